@@ -1,0 +1,193 @@
+"""Benchmark of the Metropolis sweep on one MI355X.
+
+Workload (default, `ising-sc256-i8`): three independent Ising chains on the
+simple-cubic 256^3 lattice (16.7M sites each), full PBC, J = 1, h = 0, int8
+spins, random +-1 start from seed 7, at T = 3.5, 4.5115 and 6.0, advanced by
+checkerboard sweeps (flip proposals).  One step = one sweep of all three
+chains (3 x 16,777,216 spin-flip attempts), with the per-sweep energy and
+|magnetization| of every chain returned to the host.
+
+Prints one JSON line: value = spin-flip attempts/s over the timed steps
+(spins resident in HBM), the roofline of the colour-pass kernel (algorithmic
+bytes per launch over its mean launch time from HIP events on the engine's
+stream), and the CPU baseline (the oracle's restatement of the reference's
+sequential MetropolisFlipIntegrator, one core, a bounded sample).
+"""
+import argparse
+import importlib.util
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+METRIC = "spin-flip attempts/s (Ising sc 256³); batched vs plain; p50/p99 job latency"
+
+
+def load_engine():
+    spec = importlib.util.spec_from_file_location("vegas_hip", ROOT / "vegas-rs_amd" / "__init__.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["vegas_hip"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(L_side, T, sweeps):
+    """The oracle's restatement of the reference's sequential Metropolis flip
+    loop (integrator.rs:109-138, Exchange over a CSR matrix, f64, one core)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as orc
+
+    OL = orc.lattice("sc", L_side, L_side, L_side)
+    OM = orc.model(terms=("exchange",), temperature=T)
+    t0 = time.perf_counter()
+    seq = orc.Sequential(OL, OM, 7)
+    seq.set_state(orc.ising_init(OL, 7))
+    seq.set_thermostat(T)
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    seq.sweeps(sweeps, observe=False)
+    dt = time.perf_counter() - t0
+    n = orc.num_sites(OL)
+    seq.close()
+    del np
+    return {
+        "value": n * sweeps / dt,
+        "unit": "spin-flip attempts/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{sweeps} sweeps ({n * sweeps} attempts) of sc {L_side}^3 Ising, Exchange "
+                   f"J=1 over a CSR matrix, flip proposal, T={T}, after a {build_s:.1f} s CSR "
+                   f"build; oracle restatement of MetropolisFlipIntegrator::step, 1 thread "
+                   f"on {cpu_model()} (nproc {os.cpu_count()})"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--side", type=int, default=256, help="lattice side (256 for the headline)")
+    ap.add_argument("--graph", action="store_true", help="replay sweep blocks from a hipGraph")
+    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-sweeps", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every chain "
+              f"fits one MI355X and multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    vg = load_engine()
+    temps = [3.5, 4.5115, 6.0]
+    side = args.side
+    L = vg.lattice(vg.SC, side, side, side)
+    H = vg.hamiltonian(exchange=1.0)
+    descs = [vg.chain(L, vg.ISING, vg.FLIP, H, 7) for _ in temps]
+    n_sites = side ** 3
+    attempts_per_step = n_sites * len(temps)
+    ctx = vg.Context(0)
+    batch = vg.Batch(ctx, descs)
+    batch.set_graph(args.graph)
+    batch.randomize()
+    for i, T in enumerate(temps):
+        batch.set_thermostat(i, T)
+    log(f"bench: {len(temps)} chains sc {side}^3 int8, warmup {args.warmup}")
+    if args.warmup:
+        batch.run(args.warmup)
+    log(f"bench: timing {args.steps} sweeps")
+    t0 = time.perf_counter()
+    e, m = batch.run(args.steps)
+    wall = time.perf_counter() - t0
+    device_ms = batch.last_run_ms()
+    value = attempts_per_step * args.steps / wall
+    log(f"bench: {value:.4e} attempts/s (wall {wall:.2f} s, device {device_ms / 1e3:.2f} s)")
+
+    # Per-launch kernel time of the colour pass (HIP events on the engine's stream).
+    batch.set_graph(False)
+    batch.set_timing(True)
+    tsw = max(1, min(args.timing_sweeps, args.steps))
+    batch.run(tsw, observe=False)
+    pass_ms, launches = batch.pass_stats(vg.ISING)
+    batch.set_timing(False)
+    avg_ms = pass_ms / launches
+    attempts_per_launch = attempts_per_step // 2
+    alg_bytes = 2 * attempts_per_launch  # read + write of each site once per sweep
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = ROOT / "profiles" / f"ising_sc{side}_pmc.json"
+    if pmc.exists():
+        traffic = json.loads(pmc.read_text()).get("traffic_bytes_per_launch")
+    log(f"bench: pass kernel {avg_ms * 1e3:.1f} us/launch -> {achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"bench: CPU baseline ({args.cpu_sweeps} sweeps at sc {side}^3)")
+        cpu = cpu_baseline(side, 4.5115, args.cpu_sweeps)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "spin-flip attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic (Philox random +-1 start, seed 7)",
+        "config": {
+            "workload": (f"ising-sc{side}-i8: {len(temps)} chains sc {side}x{side}x{side}, full "
+                         f"PBC, J=1, h=0, int8 spins, seed 7, T={temps}, checkerboard flip "
+                         f"sweeps; one step = one sweep of every chain"),
+            "lattice": f"sc {side}^3",
+            "chains": len(temps),
+            "attempts_per_step": attempts_per_step,
+            "graph": bool(args.graph),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "ising_sc_pass",
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "mean_launch_us": avg_ms * 1e3,
+            "launches_timed": launches,
+        },
+        "cpu_baseline": cpu,
+        "observables_last_sweep": {
+            f"T={T}": {"E/N": float(e[i, -1]) / n_sites, "|M|/N": float(m[i, -1]) / n_sites}
+            for i, T in enumerate(temps)
+        },
+    }
+    batch.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

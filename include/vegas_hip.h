@@ -1,0 +1,205 @@
+/*
+ * vegas_hip.h -- C ABI of the MI355X Metropolis engine (libvegas_hip.so).
+ *
+ * The reference (odarbelaeze/vegas-rs, crate `vegas` 0.9.0) has no FFI: its
+ * hot path is the Rust trait call
+ *     Integrator<S>::step(&self, rng, &Thermostat<S>, &H, State<S>) -> State<S>
+ * (src/integrator.rs:40-49) made once per sweep by Machine::run
+ * (src/machine.rs:91-101), followed by every Instrument recomputing
+ * H.total_energy and State::magnetization (src/instrument.rs:133-141,254-262).
+ * This header is the boundary that replaces that loop: a chain (one lattice,
+ * one spin state, one Hamiltonian) lives on the GPU and a call advances it by
+ * many sweeps, returning the per-sweep energy and |magnetization| that the
+ * reference's instruments would have computed.  Each entry point cites the
+ * reference item it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status (vg_status); 0 is success.  On
+ *    failure vg_last_error() returns a thread-local message.
+ *  - Handles are opaque and owned by the library; device memory is owned by
+ *    the library.  A handle is used by one host thread at a time.
+ *  - Host buffers are caller-owned plain arrays.  Site order at the boundary
+ *    is the lattice's natural order (vg_lattice, below).
+ */
+#ifndef VEGAS_HIP_H
+#define VEGAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes.  1-6 mirror ProgramError (src/error.rs:30-46), 10      */
+/* mirrors VegasError::NotImplementedError (src/error.rs:25-26).        */
+/* ------------------------------------------------------------------ */
+typedef enum vg_status {
+    VG_OK = 0,
+    VG_ERR_TEMPERATURE_MAX_LESS_THAN_MIN = 1,
+    VG_ERR_NO_STEPS = 2,
+    VG_ERR_ZERO_TEMPERATURE = 3,
+    VG_ERR_ZERO_COOL_RATE = 4,
+    VG_ERR_ZERO_FIELD = 5,
+    VG_ERR_ZERO_FIELD_STEP = 6,
+    VG_ERR_NOT_IMPLEMENTED = 10,
+    VG_ERR_INVALID_ARGUMENT = 20,
+    VG_ERR_UNSUPPORTED_LATTICE = 21,
+    VG_ERR_DEVICE = 30,
+    VG_ERR_OUT_OF_MEMORY = 31,
+    VG_ERR_NO_DEVICE = 32,
+    VG_ERR_ADMISSION = 40,
+    VG_ERR_NOT_FOUND = 41,
+    VG_ERR_CANCELLED = 42
+} vg_status;
+
+/* ------------------------------------------------------------------ */
+/* Lattice.  Replaces the vegas-lattice value built in Input::lattice  */
+/* (src/input.rs:296-322: Lattice::sc/bcc(1.0).expand(x,y,z), drop_*)   */
+/* and the bench lattice (src/main.rs:25).                              */
+/*   sc : site id = x + lx*(y + ly*z); 6 neighbours (+-x,+-y,+-z).      */
+/*        lx must be even; a periodic axis must have even length.       */
+/*   bcc: site id = 2*(x + lx*(y + ly*z)) + sub; sub 0 = cell corner,   */
+/*        sub 1 = body centre; 8 neighbours on the other sublattice.    */
+/* A periodic axis wraps; an open axis (drop_x/y/z) has no bonds across */
+/* its ends.  Both lattices are bipartite, which the two-colour sweep   */
+/* needs.                                                               */
+/* ------------------------------------------------------------------ */
+#define VG_LATTICE_SC 0
+#define VG_LATTICE_BCC 1
+
+#define VG_PBC_X 1u
+#define VG_PBC_Y 2u
+#define VG_PBC_Z 4u
+
+typedef struct vg_lattice {
+    int32_t kind;      /* VG_LATTICE_SC or VG_LATTICE_BCC */
+    uint32_t lx, ly, lz; /* unit cells along x, y, z (Lattice::expand) */
+    uint32_t pbc;      /* VG_PBC_* bits of the periodic axes */
+} vg_lattice;
+
+/* ------------------------------------------------------------------ */
+/* Hamiltonian.  Replaces the Hamiltonian value H of energy.rs:         */
+/*   Exchange (energy.rs:165-214)      e_i = -J sum_j s_i.s_j           */
+/*   Zeeman (energy.rs:125-161)        e_i = +|h| s_i.h_dir             */
+/*   UniaxialAnisotropy (84-121)       e_i = +K (s_i.axis)^2            */
+/*   Gauge (64-80)                     e_i = g                          */
+/*   Compound / hamiltonian! (221-290) e_i = sum of the terms           */
+/* The observable energy follows the reference's total_energy exactly:  */
+/* a Compound uses the trait default sum_i e_i (energy.rs:55-59) so     */
+/* exchange is counted twice; a single bare term uses its own           */
+/* total_energy (Exchange: sum/2; Zeeman: -|h| sum s.h_dir;             */
+/* Anisotropy: sum (s.axis)^2 without K; Gauge: N g).                   */
+/* The Zeeman field itself is a thermostat parameter (thermostat.rs).   */
+/* ------------------------------------------------------------------ */
+#define VG_TERM_EXCHANGE 1u
+#define VG_TERM_ZEEMAN 2u
+#define VG_TERM_ANISOTROPY 4u
+#define VG_TERM_GAUGE 8u
+
+typedef struct vg_hamiltonian {
+    uint32_t terms;             /* VG_TERM_* bits present */
+    int32_t compound;           /* 1: Compound of the terms; 0: one bare term */
+    double exchange;            /* J, Exchange::from_lattice(exchange, ..) */
+    double anisotropy;          /* K, UniaxialAnisotropy::new(_, k) */
+    double anisotropy_axis[3];  /* reference spin of the anisotropy */
+    double gauge;               /* Gauge::new(value) */
+} vg_hamiltonian;
+
+/* ------------------------------------------------------------------ */
+/* Chain = lattice + spin kind + proposal + Hamiltonian + seed.          */
+/* ------------------------------------------------------------------ */
+#define VG_SPIN_ISING 0          /* IsingSpin (state.rs:61-130), int8 arena */
+#define VG_SPIN_ISING_BITS 1     /* IsingSpin, 1 bit per site arena */
+#define VG_SPIN_HEISENBERG 2     /* HeisenbergSpin (state.rs:134-192), fp32 xyz */
+
+#define VG_PROPOSAL_FLIP 0       /* MetropolisFlipIntegrator (integrator.rs:100-138) */
+#define VG_PROPOSAL_RANDOM 1     /* MetropolisIntegrator (integrator.rs:57-92) */
+
+typedef struct vg_chain_desc {
+    vg_lattice lattice;
+    int32_t spin;               /* VG_SPIN_* */
+    int32_t proposal;           /* VG_PROPOSAL_* */
+    vg_hamiltonian hamiltonian;
+    uint64_t seed;              /* Philox key of this chain */
+} vg_chain_desc;
+
+typedef struct vg_ctx vg_ctx;
+typedef struct vg_batch vg_batch;
+
+/* Per-sweep integer sums the kernels reduce (fixed point for Heisenberg,
+ * scale vg_sums_scale()).  bond = sum over bonds of s_i.s_j counted once;
+ * mag = sum_i s_i; axis2 = sum_i (s_i.axis)^2. */
+typedef struct vg_sweep_sums {
+    int64_t bond;
+    int64_t mag[3];
+    int64_t axis2;
+} vg_sweep_sums;
+
+/* ---------------------------- context ------------------------------ */
+int vg_device_count(int* count);
+int vg_ctx_create(int device, vg_ctx** out);
+int vg_ctx_destroy(vg_ctx* ctx);
+const char* vg_last_error(void);
+const char* vg_version(void);
+/* 1/scale of the fixed-point sums in vg_sweep_sums for a spin kind. */
+double vg_sums_scale(int32_t spin);
+/* The kernels' Philox4x32-10 evaluated on the host (same source), for
+ * inspection: out = Philox(counter, key). */
+void vg_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ------------------------------------------------------------------ */
+/* Batch of independent chains packed in one device arena.  One chain is */
+/* a batch of size 1.  Replaces Machine<H,I,S> (machine.rs:44-125) for   */
+/* each chain: the state stays on the device between calls.              */
+/* ------------------------------------------------------------------ */
+int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs,
+                    vg_batch** out);
+int vg_batch_destroy(vg_batch* batch);
+int vg_batch_size(const vg_batch* batch, uint32_t* n_chains);
+int vg_batch_num_sites(const vg_batch* batch, uint32_t chain, uint64_t* n_sites);
+/* Machine::set_thermostat / Thermostat::new (thermostat.rs:29-39, 50-68):
+ * temperature below DBL_EPSILON is clamped to DBL_EPSILON; the field
+ * magnitude is used as |magnitude| (Field::magnitude, state.rs:219-221);
+ * orientation is the field's spin (Ising: sign of orientation[2]). */
+int vg_batch_set_thermostat(vg_batch* batch, uint32_t chain, double temperature,
+                            double field_magnitude, const double field_orientation[3]);
+/* State::rand_with_size (state.rs:260-262): i.i.d. spins from the chain's
+ * seed (Ising: Up with probability 1/2; Heisenberg: uniform on S^2 by
+ * Marsaglia's method, util.rs:21-34).  Resets the chain's sweep counter. */
+int vg_batch_randomize(vg_batch* batch);
+/* State upload / download in natural site order.  Ising: int8 +1 (Up) or
+ * -1 (Down).  Heisenberg: 3 doubles per site, stored as fp32 on device. */
+int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins);
+int vg_batch_get_ising(vg_batch* batch, uint32_t chain, int8_t* spins);
+int vg_batch_set_heisenberg(vg_batch* batch, uint32_t chain, const double* xyz);
+int vg_batch_get_heisenberg(vg_batch* batch, uint32_t chain, double* xyz);
+int vg_batch_sweep_count(const vg_batch* batch, uint32_t chain, uint64_t* sweeps);
+int vg_batch_set_sweep_count(vg_batch* batch, uint32_t chain, uint64_t sweeps);
+/* Run `sweeps` sweeps of every chain (Machine::run, machine.rs:91-101, with
+ * Integrator::step = one checkerboard sweep of N single-site attempts).
+ * energy / magnetization, if not NULL, receive n_chains*sweeps values,
+ * chain-major: the total energy (Hamiltonian::total_energy convention, see
+ * vg_hamiltonian) and |sum_i s_i| (State::magnetization().magnitude(),
+ * state.rs:291-296) after each sweep -- what StatSensor/ObservableSensor
+ * record (instrument.rs:133-141, 254-262). */
+int vg_batch_run(vg_batch* batch, uint64_t sweeps, double* energy, double* magnetization);
+/* Same sweeps, returning the raw per-sweep integer sums (n_chains*sweeps). */
+int vg_batch_run_sums(vg_batch* batch, uint64_t sweeps, vg_sweep_sums* sums);
+/* Execution options: 1 = capture sweep blocks into a hipGraph and replay. */
+int vg_batch_set_graph(vg_batch* batch, int enabled);
+/* Last run's device time in milliseconds (hipEvent pair around the sweeps). */
+int vg_batch_last_run_ms(const vg_batch* batch, double* ms);
+/* Per-launch timing of the colour-pass kernels: when enabled (plain launches
+ * only), every pass launch is bracketed by a hipEvent pair on the batch
+ * stream; vg_batch_pass_stats returns the accumulated kernel time and launch
+ * count of the pass kernel used by `spin` chains.  Enabling resets them. */
+int vg_batch_set_timing(vg_batch* batch, int enabled);
+int vg_batch_pass_stats(const vg_batch* batch, int32_t spin, double* total_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VEGAS_HIP_H */

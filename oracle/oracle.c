@@ -1,0 +1,712 @@
+/*
+ * CPU oracle -- TEST INFRASTRUCTURE ONLY (see oracle.h for who may use it).
+ * Build: oracle/Makefile (gcc, -ffp-contract=off so every fp32 operation is
+ * the single IEEE operation the checkerboard semantics name).
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon et al., SC'11): 10 rounds of                        */
+/*   (c0,c1,c2,c3) <- (hi(M1*c2)^c1^k0, lo(M1*c2), hi(M0*c0)^c3^k1, lo(M0*c0))  */
+/* with the key bumped by the Weyl constants between rounds.                 */
+/* ------------------------------------------------------------------------ */
+void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t a = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t b = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)b;
+        uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)a;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+static void draw(uint64_t seed, uint32_t x, uint32_t y, uint32_t z, uint32_t w, uint32_t out[4]) {
+    const uint32_t ctr[4] = {x, y, z, w};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    or_philox(ctr, key, out);
+}
+
+/* counter word w: colour | purpose << 1 | (sweep >> 32) << 8 */
+static uint32_t wword(uint64_t sweep, uint32_t purpose, uint32_t colour) {
+    return (uint32_t)((sweep >> 32) << 8) | (purpose << 1) | colour;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Lattice                                                                   */
+/* ------------------------------------------------------------------------ */
+uint64_t or_num_sites(const or_lattice* L) {
+    uint64_t cells = (uint64_t)L->lx * L->ly * L->lz;
+    return L->kind == 0 ? cells : 2 * cells;
+}
+
+int or_coordination(const or_lattice* L) { return L->kind == 0 ? 6 : 8; }
+
+/* coordinate a+d along an axis of length n, or -1 when it leaves an open axis */
+static int64_t step_axis(int64_t a, int64_t d, int64_t n, int periodic) {
+    int64_t b = a + d;
+    if (b >= 0 && b < n) return b;
+    if (!periodic) return -1;
+    return ((b % n) + n) % n;
+}
+
+void or_neighbours(const or_lattice* L, uint64_t site, int64_t* slots) {
+    const int px = L->pbc & 1, py = (L->pbc >> 1) & 1, pz = (L->pbc >> 2) & 1;
+    if (L->kind == 0) {
+        int64_t x = (int64_t)(site % L->lx);
+        int64_t y = (int64_t)((site / L->lx) % L->ly);
+        int64_t z = (int64_t)(site / ((uint64_t)L->lx * L->ly));
+        const int64_t d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        for (int k = 0; k < 6; ++k) {
+            int64_t a = step_axis(x, d[k][0], L->lx, px);
+            int64_t b = step_axis(y, d[k][1], L->ly, py);
+            int64_t c = step_axis(z, d[k][2], L->lz, pz);
+            slots[k] = (a < 0 || b < 0 || c < 0) ? -1 : a + (int64_t)L->lx * (b + (int64_t)L->ly * c);
+        }
+    } else {
+        /* bcc: sublattice 0 (corner of cell x,y,z) neighbours the body centres
+         * of cells (x-dx, y-dy, z-dz); sublattice 1 (centre of x,y,z) the
+         * corners of cells (x+dx, y+dy, z+dz); dx,dy,dz in {0,1}. */
+        uint64_t cell = site >> 1;
+        int sub = (int)(site & 1);
+        int64_t x = (int64_t)(cell % L->lx);
+        int64_t y = (int64_t)((cell / L->lx) % L->ly);
+        int64_t z = (int64_t)(cell / ((uint64_t)L->lx * L->ly));
+        int64_t sgn = sub == 0 ? -1 : 1;
+        int k = 0;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    int64_t a = step_axis(x, sgn * dx, L->lx, px);
+                    int64_t b = step_axis(y, sgn * dy, L->ly, py);
+                    int64_t c = step_axis(z, sgn * dz, L->lz, pz);
+                    slots[k++] = (a < 0 || b < 0 || c < 0)
+                                     ? -1
+                                     : 2 * (a + (int64_t)L->lx * (b + (int64_t)L->ly * c)) + (1 - sub);
+                }
+    }
+}
+
+void or_place(const or_lattice* L, uint64_t site, uint32_t* colour, uint32_t* row, uint32_t* pos) {
+    if (L->kind == 0) {
+        uint32_t x = (uint32_t)(site % L->lx);
+        uint32_t y = (uint32_t)((site / L->lx) % L->ly);
+        uint32_t z = (uint32_t)(site / ((uint64_t)L->lx * L->ly));
+        *colour = (x + y + z) & 1u;
+        *row = y + L->ly * z;
+        *pos = x / 2;
+    } else {
+        uint64_t cell = site >> 1;
+        uint32_t x = (uint32_t)(cell % L->lx);
+        uint32_t y = (uint32_t)((cell / L->lx) % L->ly);
+        uint32_t z = (uint32_t)(cell / ((uint64_t)L->lx * L->ly));
+        *colour = (uint32_t)(site & 1);
+        *row = y + L->ly * z;
+        *pos = x;
+    }
+}
+
+/* natural site of (colour, row, position) */
+static uint64_t site_of(const or_lattice* L, uint32_t colour, uint32_t row, uint32_t pos) {
+    uint32_t y = row % L->ly, z = row / L->ly;
+    if (L->kind == 0) {
+        uint32_t x = 2 * pos + ((colour + y + z) & 1u);
+        return x + (uint64_t)L->lx * (y + (uint64_t)L->ly * z);
+    }
+    return 2 * (pos + (uint64_t)L->lx * (y + (uint64_t)L->ly * z)) + colour;
+}
+
+static uint32_t row_length(const or_lattice* L) { return L->kind == 0 ? L->lx / 2 : L->lx; }
+
+/* ------------------------------------------------------------------------ */
+/* Ising checkerboard                                                        */
+/* ------------------------------------------------------------------------ */
+void or_ising_table(const or_lattice* L, const or_model* M, uint32_t* thr, uint32_t* always) {
+    const int z = or_coordination(L);
+    const double J = (M->terms & 1u) ? M->exchange : 0.0;
+    const double hm = (M->terms & 2u) ? fabs(M->field) : 0.0;
+    const double hz = M->field_dir[2] >= 0.0 ? 1.0 : -1.0;
+    const double T = M->temperature < DBL_EPSILON ? DBL_EPSILON : M->temperature;
+    for (int b = 0; b < 2; ++b) {
+        always[b] = 0;
+        double s = b ? 1.0 : -1.0;
+        for (int S = 0; S < 17; ++S) {
+            thr[b * 17 + S] = 0;
+            if (S > 2 * z) continue;
+            double sigma = (double)(S - z);
+            /* flip s -> -s: exchange term 2 J s sigma, Zeeman term -2 |h| hz s */
+            double dE = 2.0 * J * s * sigma - 2.0 * hm * hz * s;
+            if (dE <= 0.0) {
+                always[b] |= 1u << S;
+                continue;
+            }
+            double p = exp(-dE / T);
+            double th = ceil(p * 4294967296.0);
+            if (th >= 4294967296.0)
+                always[b] |= 1u << S;
+            else
+                thr[b * 17 + S] = (uint32_t)th;
+        }
+    }
+}
+
+void or_ising_init(const or_lattice* L, uint64_t seed, int8_t* spins) {
+    uint64_t n = or_num_sites(L);
+    for (uint64_t site = 0; site < n; ++site) {
+        uint32_t c, row, pos, r[4];
+        or_place(L, site, &c, &row, &pos);
+        draw(seed, pos >> 2, row, 0, wword(0, 3, c), r);
+        spins[site] = (r[pos & 3] >> 31) == 0 ? 1 : -1;
+    }
+}
+
+void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t sweep0,
+                     uint64_t nsweeps, int8_t* spins, int64_t* sums) {
+    const uint64_t n = or_num_sites(L);
+    const int z = or_coordination(L);
+    uint32_t thr[34], always[2];
+    or_ising_table(L, M, thr, always);
+    int64_t nb[8];
+    for (uint64_t k = 0; k < nsweeps; ++k) {
+        const uint64_t t = sweep0 + k;
+        for (uint32_t c = 0; c < 2; ++c) {
+            for (uint64_t site = 0; site < n; ++site) {
+                uint32_t cc, row, pos;
+                or_place(L, site, &cc, &row, &pos);
+                if (cc != c) continue;
+                or_neighbours(L, site, nb);
+                uint32_t S = 0;
+                for (int j = 0; j < z; ++j) S += nb[j] < 0 ? 1u : (spins[nb[j]] > 0 ? 2u : 0u);
+                uint32_t b = spins[site] > 0;
+                if (M->proposal == 1) {
+                    uint32_t r[4];
+                    draw(seed, pos >> 7, row, (uint32_t)t, wword(t, 2, c), r);
+                    uint32_t bit = (r[(pos >> 5) & 3] >> (pos & 31)) & 1u;
+                    if (bit != b) continue; /* proposal equals the current spin */
+                }
+                int flip;
+                if ((always[b] >> S) & 1u) {
+                    flip = 1;
+                } else {
+                    uint32_t ra[4], rb[4];
+                    draw(seed, pos >> 4, row, (uint32_t)t, wword(t, 0, c), ra);
+                    draw(seed, pos >> 2, row, (uint32_t)t, wword(t, 1, c), rb);
+                    uint32_t hi8 = (ra[(pos & 15) >> 2] >> (8 * (pos & 3))) & 0xFFu;
+                    uint32_t lo24 = rb[pos & 3] >> 8;
+                    uint32_t draw32 = (hi8 << 24) | lo24;
+                    flip = draw32 < thr[b * 17 + S];
+                }
+                if (flip) spins[site] = (int8_t)-spins[site];
+            }
+        }
+        if (sums) {
+            int64_t bond = 0, mag = 0;
+            for (uint64_t site = 0; site < n; ++site) {
+                mag += spins[site];
+                uint32_t cc, row, pos;
+                or_place(L, site, &cc, &row, &pos);
+                if (cc != 1) continue;
+                or_neighbours(L, site, nb);
+                for (int j = 0; j < z; ++j)
+                    if (nb[j] >= 0) bond += (int64_t)spins[site] * spins[nb[j]];
+            }
+            int64_t* o = sums + 5 * k;
+            o[0] = bond;
+            o[1] = 0;
+            o[2] = 0;
+            o[3] = mag;
+            o[4] = (int64_t)n;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Heisenberg checkerboard (fp32; every line is one IEEE operation)          */
+/* ------------------------------------------------------------------------ */
+float or_exp_nonpos(float x) {
+    if (x < -86.0f) return 0.0f;
+    const float log2e = (float)1.4426950408889634;
+    const float ln2_hi = 0.693145751953125f;
+    const float ln2_lo = (float)(0.6931471805599453 - 0.693145751953125);
+    float n = rintf(x * log2e);
+    float r = fmaf(n, -ln2_hi, x);
+    r = fmaf(n, -ln2_lo, r);
+    const float c[8] = {1.0f, 1.0f, 0.5f, (float)(1.0 / 6.0), (float)(1.0 / 24.0),
+                        (float)(1.0 / 120.0), (float)(1.0 / 720.0), (float)(1.0 / 5040.0)};
+    float p = c[7];
+    for (int k = 6; k >= 0; --k) p = fmaf(p, r, c[k]);
+    return ldexpf(p, (int)n);
+}
+
+static float dot3f(const float* a, const float* b) {
+    float t = a[0] * b[0];
+    float u = a[1] * b[1];
+    float v = t + u;
+    float w = a[2] * b[2];
+    return v + w;
+}
+
+static float unit24f(uint32_t w) { return (float)(w >> 8) * 5.9604644775390625e-08f; }
+static float sym24f(uint32_t w) { return (float)(w >> 8) * 1.1920928955078125e-07f - 1.0f; }
+
+static int32_t fix27(float v) { return (int32_t)lrintf(v * 134217728.0f); }
+
+/* Marsaglia (1972) in fp32 from a pair of words; returns 0 when rejected. */
+static int marsaglia_f32(uint32_t w1, uint32_t w2, float* out) {
+    float x1 = sym24f(w1), x2 = sym24f(w2);
+    float a = x1 * x1;
+    float b = x2 * x2;
+    float q = a + b;
+    if (!(q < 1.0f)) return 0;
+    float root = sqrtf(1.0f - q);
+    out[0] = (2.0f * x1) * root;
+    out[1] = (2.0f * x2) * root;
+    out[2] = 1.0f - 2.0f * q;
+    return 1;
+}
+
+void or_heis_init(const or_lattice* L, uint64_t seed, float* xyz) {
+    uint64_t n = or_num_sites(L);
+    for (uint64_t site = 0; site < n; ++site) {
+        uint32_t c, row, pos;
+        or_place(L, site, &c, &row, &pos);
+        float s[3] = {0.0f, 0.0f, 1.0f};
+        int done = 0;
+        for (uint32_t k = 0; k < 64 && !done; ++k) {
+            uint32_t r[4];
+            draw(seed, pos, row, k, wword(0, 4, c), r);
+            for (int t = 0; t < 2 && !done; ++t) done = marsaglia_f32(r[2 * t], r[2 * t + 1], s);
+        }
+        memcpy(xyz + 3 * site, s, sizeof s);
+    }
+}
+
+typedef struct {
+    uint64_t seed;
+    uint32_t x, y, z, whi;
+    uint32_t call, used;
+    uint32_t words[4];
+} stream_t;
+
+static void stream_start(stream_t* st, uint64_t seed, uint32_t x, uint32_t y, uint64_t t, uint32_t colour) {
+    st->seed = seed;
+    st->x = x;
+    st->y = y;
+    st->z = (uint32_t)t;
+    st->whi = (uint32_t)((t >> 32) << 8) | colour;
+    st->call = 0;
+    st->used = 0;
+    draw(seed, x, y, st->z, st->whi | (16u << 1), st->words);
+}
+
+/* next word of the stream; 0 once 112 calls are used up */
+static int stream_next(stream_t* st, uint32_t* w) {
+    if (st->used == 4) {
+        if (st->call + 1 >= 112) return 0;
+        st->call++;
+        draw(st->seed, st->x, st->y, st->z, st->whi | ((16u + st->call) << 1), st->words);
+        st->used = 0;
+    }
+    *w = st->words[st->used++];
+    return 1;
+}
+
+void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t sweep0,
+                    uint64_t nsweeps, float* xyz, int64_t* sums) {
+    const int z = or_coordination(L);
+    const uint32_t rows = L->ly * L->lz, w = row_length(L);
+    const float jn = (M->terms & 1u) ? (float)(-M->exchange) : 0.0f;
+    const float kk = (M->terms & 4u) ? (float)M->anisotropy : 0.0f;
+    const float hm = (M->terms & 2u) ? (float)fabs(M->field) : 0.0f;
+    const double T = M->temperature < DBL_EPSILON ? DBL_EPSILON : M->temperature;
+    const float beta = (float)(1.0 / T);
+    const float axis[3] = {(float)M->axis[0], (float)M->axis[1], (float)M->axis[2]};
+    const float fdir[3] = {(float)M->field_dir[0], (float)M->field_dir[1], (float)M->field_dir[2]};
+    int64_t nb[8];
+    for (uint64_t k = 0; k < nsweeps; ++k) {
+        const uint64_t t = sweep0 + k;
+        int64_t acc[5] = {0, 0, 0, 0, 0};
+        for (uint32_t c = 0; c < 2; ++c) {
+            for (uint32_t row = 0; row < rows; ++row) {
+                for (uint32_t g = 0; 4 * g < w; ++g) {
+                    stream_t st;
+                    stream_start(&st, seed, g, row, t, c);
+                    int live = 1;
+                    for (uint32_t pos = 4 * g; pos < 4 * g + 4 && pos < w; ++pos) {
+                        const uint64_t site = site_of(L, c, row, pos);
+                        float* s = xyz + 3 * site;
+                        or_neighbours(L, site, nb);
+                        float h[3];
+                        for (int a = 0; a < 3; ++a) {
+                            float v0 = nb[0] < 0 ? 0.0f : xyz[3 * nb[0] + a];
+                            float v1 = nb[1] < 0 ? 0.0f : xyz[3 * nb[1] + a];
+                            float acc_h = v0 + v1;
+                            for (int j = 2; j < z; ++j)
+                                acc_h = acc_h + (nb[j] < 0 ? 0.0f : xyz[3 * nb[j] + a]);
+                            h[a] = acc_h;
+                        }
+                        uint32_t wu = 0;
+                        if (live) live = stream_next(&st, &wu);
+                        const float u = unit24f(wu);
+                        float p[3] = {-s[0], -s[1], -s[2]};
+                        if (M->proposal == 1) {
+                            p[0] = 0.0f;
+                            p[1] = 0.0f;
+                            p[2] = 1.0f;
+                            while (live) {
+                                uint32_t w1, w2;
+                                if (!stream_next(&st, &w1) || !stream_next(&st, &w2)) {
+                                    live = 0;
+                                    break;
+                                }
+                                if (marsaglia_f32(w1, w2, p)) break;
+                            }
+                        }
+                        const float d[3] = {p[0] - s[0], p[1] - s[1], p[2] - s[2]};
+                        const float e_ex = jn * dot3f(d, h);
+                        const float pe = dot3f(p, axis), oe = dot3f(s, axis);
+                        const float pe2 = pe * pe, oe2 = oe * oe;
+                        const float e_an = kk * (pe2 - oe2);
+                        const float e_ze = hm * dot3f(d, fdir);
+                        const float e1 = e_ex + e_an;
+                        const float dE = e1 + e_ze;
+                        int accept = dE < 0.0f;
+                        if (!accept) {
+                            const float x = -(dE * beta);
+                            accept = u < or_exp_nonpos(x);
+                        }
+                        if (accept) memcpy(s, p, sizeof p);
+                        acc[1] += fix27(s[0]);
+                        acc[2] += fix27(s[1]);
+                        acc[3] += fix27(s[2]);
+                        const float se = dot3f(s, axis);
+                        acc[4] += fix27(se * se);
+                        if (c == 1) acc[0] += fix27(dot3f(s, h));
+                    }
+                }
+            }
+        }
+        if (sums) memcpy(sums + 5 * k, acc, sizeof acc);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PCG64: 128-bit LCG with multiplier 0x2360ED051FC65DA44385DF649FCCF645,   */
+/* output XSL-RR of the stepped state (O'Neill, PCG, 2014).                  */
+/* ------------------------------------------------------------------------ */
+typedef unsigned __int128 u128;
+
+static u128 pcg_mult(void) {
+    return ((u128)0x2360ED051FC65DA4ull << 64) | (u128)0x4385DF649FCCF645ull;
+}
+
+uint64_t or_pcg64_next(or_pcg64* g) {
+    u128 state = ((u128)g->state_hi << 64) | g->state_lo;
+    u128 inc = ((u128)g->inc_hi << 64) | g->inc_lo;
+    state = state * pcg_mult() + inc;
+    g->state_hi = (uint64_t)(state >> 64);
+    g->state_lo = (uint64_t)state;
+    uint64_t xsl = (uint64_t)(state >> 64) ^ (uint64_t)state;
+    unsigned rot = (unsigned)(state >> 122);
+    return (xsl >> rot) | (xsl << ((64 - rot) & 63));
+}
+
+static uint64_t splitmix64(uint64_t* x) {
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void or_pcg64_seed(or_pcg64* g, uint64_t seed) {
+    uint64_t x = seed;
+    g->state_hi = splitmix64(&x);
+    g->state_lo = splitmix64(&x);
+    g->inc_hi = splitmix64(&x);
+    g->inc_lo = splitmix64(&x) | 1u;
+}
+
+static double pcg_f64(or_pcg64* g) { return (double)(or_pcg64_next(g) >> 11) * 0x1.0p-53; }
+
+/* unbiased integer in [0, n) by Lemire's multiply-and-reject method */
+static uint64_t pcg_below(or_pcg64* g, uint64_t n) {
+    u128 m = (u128)or_pcg64_next(g) * n;
+    uint64_t l = (uint64_t)m;
+    if (l < n) {
+        uint64_t t = (0 - n) % n;
+        while (l < t) {
+            m = (u128)or_pcg64_next(g) * n;
+            l = (uint64_t)m;
+        }
+    }
+    return (uint64_t)(m >> 64);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sequential reference restatement                                         */
+/* ------------------------------------------------------------------------ */
+struct or_seq {
+    or_lattice L;
+    or_model M;
+    uint64_t n;
+    uint64_t* indptr;  /* CSR of the exchange matrix: usize indices, f64 values */
+    uint64_t* indices;
+    double* values;
+    int8_t* ising;
+    double* heis;
+    or_pcg64 rng;
+};
+
+static int cmp_i64(const void* a, const void* b) {
+    int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return (x > y) - (x < y);
+}
+
+or_seq* or_seq_create(const or_lattice* L, const or_model* M, uint64_t seed) {
+    or_seq* s = (or_seq*)calloc(1, sizeof(or_seq));
+    s->L = *L;
+    s->M = *M;
+    s->n = or_num_sites(L);
+    const int z = or_coordination(L);
+    /* Exchange::from_lattice: J on every bond, both directions; duplicate
+     * entries (length-2 periodic axes) summed, columns ascending. */
+    s->indptr = (uint64_t*)malloc((s->n + 1) * sizeof(uint64_t));
+    s->indices = (uint64_t*)malloc(s->n * z * sizeof(uint64_t));
+    s->values = (double*)malloc(s->n * z * sizeof(double));
+    uint64_t nnz = 0;
+    int64_t nb[8];
+    for (uint64_t i = 0; i < s->n; ++i) {
+        s->indptr[i] = nnz;
+        or_neighbours(L, i, nb);
+        int m = 0;
+        for (int j = 0; j < z; ++j)
+            if (nb[j] >= 0) nb[m++] = nb[j];
+        qsort(nb, (size_t)m, sizeof(int64_t), cmp_i64);
+        for (int j = 0; j < m; ++j) {
+            if (j > 0 && nb[j] == nb[j - 1]) {
+                s->values[nnz - 1] += M->exchange;
+            } else {
+                s->indices[nnz] = (uint64_t)nb[j];
+                s->values[nnz] = M->exchange;
+                ++nnz;
+            }
+        }
+    }
+    s->indptr[s->n] = nnz;
+    if (M->spin == 2)
+        s->heis = (double*)calloc(3 * s->n, sizeof(double));
+    else
+        s->ising = (int8_t*)calloc(s->n, 1);
+    or_pcg64_seed(&s->rng, seed);
+    return s;
+}
+
+void or_seq_destroy(or_seq* s) {
+    if (!s) return;
+    free(s->indptr);
+    free(s->indices);
+    free(s->values);
+    free(s->ising);
+    free(s->heis);
+    free(s);
+}
+
+uint64_t or_seq_nnz(const or_seq* s) { return s->indptr[s->n]; }
+
+void or_seq_set_thermostat(or_seq* s, double temperature, double field, const double dir[3]) {
+    s->M.temperature = temperature < DBL_EPSILON ? DBL_EPSILON : temperature;
+    s->M.field = field;
+    for (int k = 0; k < 3; ++k) s->M.field_dir[k] = dir ? dir[k] : (k == 2 ? 1.0 : 0.0);
+}
+
+void or_seq_set_state(or_seq* s, const void* spins) {
+    if (s->ising)
+        memcpy(s->ising, spins, s->n);
+    else
+        memcpy(s->heis, spins, 3 * s->n * sizeof(double));
+}
+
+void or_seq_get_state(const or_seq* s, void* spins) {
+    if (s->ising)
+        memcpy(spins, s->ising, s->n);
+    else
+        memcpy(spins, s->heis, 3 * s->n * sizeof(double));
+}
+
+/* s_i . v for a spin given as three components */
+static double sdot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void spin_vec(const or_seq* s, uint64_t i, double* v) {
+    if (s->ising) {
+        v[0] = 0.0;
+        v[1] = 0.0;
+        v[2] = (double)s->ising[i];
+    } else {
+        v[0] = s->heis[3 * i];
+        v[1] = s->heis[3 * i + 1];
+        v[2] = s->heis[3 * i + 2];
+    }
+}
+
+/* Ising dot products are +-1 (state.rs:95-101); the field and anisotropy
+ * reference of an Ising system is Up or Down. */
+static void ref_vec(const or_seq* s, const double* d, double* v) {
+    if (s->ising) {
+        v[0] = 0.0;
+        v[1] = 0.0;
+        v[2] = d[2] >= 0.0 ? 1.0 : -1.0;
+    } else {
+        v[0] = d[0];
+        v[1] = d[1];
+        v[2] = d[2];
+    }
+}
+
+static double e_exchange(const or_seq* s, uint64_t i) {
+    double si[3], sj[3], e = 0.0;
+    spin_vec(s, i, si);
+    for (uint64_t k = s->indptr[i]; k < s->indptr[i + 1]; ++k) {
+        spin_vec(s, s->indices[k], sj);
+        e += -s->values[k] * sdot(si, sj);
+    }
+    return e;
+}
+
+/* Compound(Compound(Compound(Exchange, Zeeman), Anisotropy), Gauge) energy of
+ * site i with the absent terms left out (energy.rs:254-256). */
+static double site_energy(const or_seq* s, uint64_t i) {
+    const or_model* M = &s->M;
+    double si[3], r[3];
+    spin_vec(s, i, si);
+    double e = 0.0;
+    int first = 1;
+    if (M->terms & 1u) {
+        e = e_exchange(s, i);
+        first = 0;
+    }
+    if (M->terms & 2u) {
+        ref_vec(s, M->field_dir, r);
+        double z = sdot(si, r) * fabs(M->field);
+        e = first ? z : e + z;
+        first = 0;
+    }
+    if (M->terms & 4u) {
+        ref_vec(s, M->axis, r);
+        double d = sdot(si, r);
+        double a = d * d * M->anisotropy;
+        e = first ? a : e + a;
+        first = 0;
+    }
+    if (M->terms & 8u) e = first ? M->gauge : e + M->gauge;
+    return e;
+}
+
+static void marsaglia_f64(or_pcg64* g, double* out) {
+    for (;;) {
+        double x1 = -1.0 + 2.0 * pcg_f64(g);
+        double x2 = -1.0 + 2.0 * pcg_f64(g);
+        if (x1 * x1 + x2 * x2 >= 1.0) continue;
+        out[0] = 2.0 * x1 * sqrt(1.0 - x1 * x1 - x2 * x2);
+        out[1] = 2.0 * x2 * sqrt(1.0 - x1 * x1 - x2 * x2);
+        out[2] = 1.0 - 2.0 * (x1 * x1 + x2 * x2);
+        return;
+    }
+}
+
+double or_seq_total_energy(const or_seq* s) {
+    const or_model* M = &s->M;
+    double total = 0.0;
+    if (M->compound) {
+        for (uint64_t i = 0; i < s->n; ++i) total += site_energy(s, i);
+        return total;
+    }
+    double si[3], r[3];
+    if (M->terms & 1u) {
+        for (uint64_t i = 0; i < s->n; ++i) total += e_exchange(s, i);
+        return total / 2.0;
+    }
+    if (M->terms & 2u) {
+        ref_vec(s, M->field_dir, r);
+        for (uint64_t i = 0; i < s->n; ++i) {
+            spin_vec(s, i, si);
+            total += sdot(si, r);
+        }
+        return -fabs(M->field) * total;
+    }
+    if (M->terms & 4u) {
+        ref_vec(s, M->axis, r);
+        for (uint64_t i = 0; i < s->n; ++i) {
+            spin_vec(s, i, si);
+            double d = sdot(si, r);
+            total += d * d;
+        }
+        return total;
+    }
+    for (uint64_t i = 0; i < s->n; ++i) total += M->gauge;
+    return total;
+}
+
+double or_seq_magnetization(const or_seq* s) {
+    double m[3] = {0.0, 0.0, 0.0}, si[3];
+    for (uint64_t i = 0; i < s->n; ++i) {
+        spin_vec(s, i, si);
+        m[0] += si[0];
+        m[1] += si[1];
+        m[2] += si[2];
+    }
+    if (s->ising) return fabs(m[2]);
+    double mag = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    return fabs(mag) < DBL_EPSILON ? 0.0 : mag;
+}
+
+void or_seq_sweeps(or_seq* s, uint64_t nsweeps, double* energy, double* mag) {
+    const double T = s->M.temperature;
+    for (uint64_t k = 0; k < nsweeps; ++k) {
+        for (uint64_t a = 0; a < s->n; ++a) {
+            uint64_t i = pcg_below(&s->rng, s->n);
+            double old_e = site_energy(s, i);
+            int8_t old_i = 0;
+            double old_h[3] = {0.0, 0.0, 0.0};
+            if (s->ising) {
+                old_i = s->ising[i];
+                if (s->M.proposal == 1)
+                    s->ising[i] = pcg_f64(&s->rng) < 0.5 ? 1 : -1;
+                else
+                    s->ising[i] = (int8_t)-old_i;
+            } else {
+                memcpy(old_h, s->heis + 3 * i, sizeof old_h);
+                if (s->M.proposal == 1) {
+                    marsaglia_f64(&s->rng, s->heis + 3 * i);
+                } else {
+                    for (int c = 0; c < 3; ++c) s->heis[3 * i + c] = -old_h[c];
+                }
+            }
+            double delta = site_energy(s, i) - old_e;
+            if (delta < 0.0) continue;
+            if (pcg_f64(&s->rng) < exp(-delta / T)) continue;
+            if (s->ising)
+                s->ising[i] = old_i;
+            else
+                memcpy(s->heis + 3 * i, old_h, sizeof old_h);
+        }
+        if (energy) energy[k] = or_seq_total_energy(s);
+        if (mag) mag[k] = or_seq_magnetization(s);
+    }
+}

@@ -1,0 +1,327 @@
+"""Pins of the CPU oracle that do not need a GPU.
+
+Each check is chosen so that a plausible slip in the oracle (a wrong sign,
+a dropped term, a transposed index, a wrong neighbour) fails it: known
+answers from independent implementations (numpy's PCG64), closed forms,
+exact enumeration of tiny lattices, and the reference's own unit tests
+(energy.rs:300-375, state.rs:330-382) with two-sided expectations.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from _stats import binned, ising_exact, z_score
+
+
+# ---------------------------------------------------------------- PCG64 ----
+def test_pcg64_matches_numpy(orc):
+    """The sequential oracle's generator is PCG64 XSL-RR 128/64, the algorithm
+    of rand_pcg::Lcg128Xsl64 (Cargo.toml:14) and numpy's PCG64."""
+    g = orc.pcg64(12345)
+    state = (g.state_hi << 64) | g.state_lo
+    inc = (g.inc_hi << 64) | g.inc_lo
+    bg = np.random.PCG64()
+    bg.state = {"bit_generator": "PCG64", "state": {"state": state, "inc": inc},
+                "has_uint32": 0, "uinteger": 0}
+    expected = bg.random_raw(1000)
+    got = [orc.pcg64_next(g) for _ in range(1000)]
+    assert got == [int(v) for v in expected]
+
+
+# -------------------------------------------------------------- lattice ----
+@pytest.mark.parametrize("kind,dims,pbc", [
+    ("sc", (4, 4, 4), (1, 1, 1)), ("sc", (10, 10, 1), (1, 1, 0)), ("sc", (2, 4, 6), (1, 1, 1)),
+    ("sc", (6, 3, 5), (1, 0, 0)), ("bcc", (3, 3, 3), (1, 1, 1)), ("bcc", (2, 3, 1), (0, 1, 0)),
+])
+def test_neighbours_symmetric_and_bipartite(orc, kind, dims, pbc):
+    L = orc.lattice(kind, *dims, pbc=pbc)
+    n = orc.num_sites(L)
+    counts = {}
+    for i in range(n):
+        ci, _, _ = orc.place(L, i)
+        for j in orc.neighbours(L, i):
+            if j < 0:
+                continue
+            assert orc.place(L, j)[0] != ci  # two-colourable
+            counts[(i, j)] = counts.get((i, j), 0) + 1
+    for (i, j), c in counts.items():
+        assert counts.get((j, i)) == c  # symmetric with multiplicity
+
+
+def test_neighbour_counts(orc):
+    # sc periodic: 6 distinct neighbours; open 10x10x1: 4 in the bulk, 0 along z
+    L = orc.lattice("sc", 4, 4, 4)
+    assert all(len([j for j in orc.neighbours(L, i) if j >= 0]) == 6 for i in range(64))
+    L = orc.lattice("sc", 10, 10, 1, pbc=(1, 1, 0))
+    for i in range(100):
+        nb = orc.neighbours(L, i)
+        assert nb[4] == -1 and nb[5] == -1 and min(nb[:4]) >= 0
+    # site (1,2,3) of a 4x5x6 sc lattice: explicit neighbours
+    L = orc.lattice("sc", 4, 5, 6)
+    site = 1 + 4 * (2 + 5 * 3)
+    expect = [0 + 4 * (2 + 5 * 3), 2 + 4 * (2 + 5 * 3), 1 + 4 * (1 + 5 * 3), 1 + 4 * (3 + 5 * 3),
+              1 + 4 * (2 + 5 * 2), 1 + 4 * (2 + 5 * 4)]
+    assert orc.neighbours(L, site) == expect
+    # bcc periodic: 8 distinct neighbours on the other sublattice
+    L = orc.lattice("bcc", 3, 3, 3)
+    for i in range(orc.num_sites(L)):
+        nb = orc.neighbours(L, i)
+        assert len(set(nb)) == 8 and all((j & 1) != (i & 1) for j in nb)
+
+
+def test_place_round_trip(orc):
+    for kind, dims in [("sc", (6, 4, 2)), ("bcc", (3, 2, 2))]:
+        L = orc.lattice(kind, *dims)
+        seen = set()
+        for i in range(orc.num_sites(L)):
+            seen.add(orc.place(L, i))
+        assert len(seen) == orc.num_sites(L)
+
+
+# ---------------------------------------------- reference energy quirks ----
+def _heis_seq(orc, n, terms, **kw):
+    L = orc.lattice("sc", n, 1, 1, pbc=(0, 0, 0))
+    M = orc.model(spin="heisenberg", proposal="random", terms=terms, **kw)
+    return orc.Sequential(L, M, 1)
+
+
+def test_reference_energy_unit_tests(orc):
+    """energy.rs:300-375 with the reference's values computed, not its
+    one-sided asserts: Gauge(10) -> 100; Anisotropy total omits K -> 10;
+    Zeeman total is -|h| sum s.h -> -10/+10 and -20/+20; Compound(Gauge(10),
+    Anisotropy(1)) uses the default sum of site energies -> 110;
+    hamiltonian!(Anisotropy(1), Gauge(1)) -> 20."""
+    up = np.tile([0.0, 0.0, 1.0], (10, 1))
+    down = -up
+    s = _heis_seq(orc, 10, ("gauge",), gauge=10.0)
+    s.set_state(up)
+    assert s.total_energy() == pytest.approx(100.0, abs=1e-12)
+    s = _heis_seq(orc, 10, ("anisotropy",), anisotropy=1.0)
+    for st in (up, down):
+        s.set_state(st)
+        assert s.total_energy() == pytest.approx(10.0, abs=1e-12)
+    s = _heis_seq(orc, 10, ("anisotropy",), anisotropy=7.0)
+    s.set_state(up)
+    assert s.total_energy() == pytest.approx(10.0, abs=1e-12)  # K omitted (Q3)
+    for mag in (1.0, 2.0):
+        s = _heis_seq(orc, 10, ("zeeman",))
+        s.set_thermostat(1e-300, mag, (0.0, 0.0, 1.0))
+        s.set_state(up)
+        assert s.total_energy() == pytest.approx(-10.0 * mag, abs=1e-12)
+        s.set_state(down)
+        assert s.total_energy() == pytest.approx(10.0 * mag, abs=1e-12)
+    s = _heis_seq(orc, 10, ("anisotropy", "gauge"), anisotropy=1.0, gauge=10.0)
+    s.set_state(up)
+    assert s.total_energy() == pytest.approx(110.0, abs=1e-12)
+    s = _heis_seq(orc, 10, ("anisotropy", "gauge"), anisotropy=1.0, gauge=1.0)
+    s.set_state(up)
+    assert s.total_energy() == pytest.approx(20.0, abs=1e-12)
+
+
+def test_ground_state_energies(orc):
+    """Compound(Exchange, Zeeman) double counts exchange (Q4): the 10x10x1
+    CLI lattice (x,y periodic, z open) has E/N = -4 all up; Exchange alone on
+    sc has E/N = -3 (the `vegas bench` Hamiltonian); bcc E/N = -4."""
+    L = orc.lattice("sc", 10, 10, 1, pbc=(1, 1, 0))
+    s = orc.Sequential(L, orc.model(terms=("exchange", "zeeman")), 1)
+    s.set_state(np.ones(100, dtype=np.int8))
+    assert s.total_energy() / 100 == pytest.approx(-4.0)
+    assert s.magnetization() == 100.0
+    assert s.nnz() == 400
+    L = orc.lattice("sc", 6, 6, 6)
+    s = orc.Sequential(L, orc.model(terms=("exchange",)), 1)
+    s.set_state(-np.ones(216, dtype=np.int8))
+    assert s.total_energy() / 216 == pytest.approx(-3.0)
+    assert s.magnetization() == 216.0
+    L = orc.lattice("bcc", 3, 3, 3)
+    s = orc.Sequential(L, orc.model(terms=("exchange",)), 1)
+    s.set_state(np.ones(54, dtype=np.int8))
+    assert s.total_energy() / 54 == pytest.approx(-4.0)
+    # Neel state: every bond broken
+    L = orc.lattice("sc", 4, 4, 4)
+    st = np.array([1 if orc.place(L, i)[0] == 0 else -1 for i in range(64)], dtype=np.int8)
+    s = orc.Sequential(L, orc.model(terms=("exchange",)), 1)
+    s.set_state(st)
+    assert s.total_energy() / 64 == pytest.approx(3.0)
+    assert s.magnetization() == 0.0
+
+
+def test_length_two_axis_counts_bond_twice(orc):
+    """On a periodic axis of length 2 the +x and -x neighbours coincide; the
+    CSR sums the duplicate, so the bond carries 2J."""
+    L = orc.lattice("sc", 2, 4, 4)
+    s = orc.Sequential(L, orc.model(terms=("exchange",)), 1)
+    s.set_state(np.ones(32, dtype=np.int8))
+    assert s.total_energy() / 32 == pytest.approx(-3.0)
+    assert s.nnz() == 32 * 5
+
+
+# ---------------------------------------------------- acceptance table ----
+def test_ising_table_closed_forms(orc):
+    L = orc.lattice("sc", 4, 4, 4)
+    T = 3.0
+    thr, alw = orc.ising_table(L, orc.model(terms=("exchange",), temperature=T))
+    # h = 0: a flip is free or downhill iff at most 3 of 6 neighbours align
+    # (S = 2 * up neighbours when none is absent)
+    for S in range(0, 13, 2):
+        up_aligned = S // 2
+        assert bool(alw[1] >> S & 1) == (up_aligned <= 3)
+        assert bool(alw[0] >> S & 1) == (6 - up_aligned <= 3)
+    # odd S: one absent neighbour slot counts 1, so sum_j s_j = S - 6
+    for S in range(1, 13, 2):
+        assert bool(alw[1] >> S & 1) == (S - 6 <= 0)
+        assert bool(alw[0] >> S & 1) == (6 - S <= 0)
+    # all six neighbours aligned: dE = 12 J
+    assert thr[1, 12] == math.ceil(math.exp(-12.0 / T) * 2 ** 32)
+    assert thr[0, 0] == thr[1, 12]
+    assert thr[1, 10] == math.ceil(math.exp(-8.0 / T) * 2 ** 32)
+    # Zeeman sign (Q1): the per-site energy +|h| s.h makes Up costly under an
+    # Up field, so flipping Up -> Down with no exchange bias is downhill.
+    thr, alw = orc.ising_table(L, orc.model(terms=("exchange", "zeeman"), temperature=T,
+                                            field=0.5))
+    assert alw[1] >> 6 & 1
+    assert thr[0, 6] == math.ceil(math.exp(-1.0 / T) * 2 ** 32)
+    thr2, alw2 = orc.ising_table(L, orc.model(terms=("exchange", "zeeman"), temperature=T,
+                                              field=-0.5))
+    assert (thr2 == thr).all() and alw2 == alw  # |magnitude| is used (Q2)
+    thr3, alw3 = orc.ising_table(L, orc.model(terms=("exchange", "zeeman"), temperature=T,
+                                              field=0.5, field_dir=(0, 0, -1)))
+    assert alw3[0] >> 6 & 1 and thr3[1, 6] == thr[0, 6]
+
+
+# ------------------------------------------------------------- exp(x) ----
+def test_exp_nonpos_accuracy(orc):
+    xs = np.concatenate([-np.geomspace(1e-8, 86.0, 4000), [0.0, -1e-30, -0.5, -86.0]])
+    worst = 0.0
+    for x in xs:
+        x32 = float(np.float32(x))
+        got = orc.exp_nonpos(x32)
+        ref = math.exp(x32)
+        worst = max(worst, abs(got - ref) / ref)
+    assert worst < 3e-7
+    assert orc.exp_nonpos(0.0) == 1.0
+    assert orc.exp_nonpos(-87.0) == 0.0
+
+
+# ------------------------------------------------- Heisenberg sampling ----
+def test_heisenberg_init_uniform_on_sphere(orc):
+    L = orc.lattice("sc", 32, 32, 32)
+    s = orc.heis_init(L, 7).astype(np.float64)
+    norm = np.linalg.norm(s, axis=1)
+    assert np.abs(norm - 1.0).max() < 1e-6
+    n = len(s)
+    assert np.abs(s.mean(axis=0)).max() < 5 * np.sqrt(1 / 3 / n)
+    assert np.abs((s ** 2).mean(axis=0) - 1 / 3).max() < 5 * np.sqrt(4 / 45 / n)
+    # the two colour planes are independent streams
+    assert not np.array_equal(s[0], s[1])
+
+
+def test_ising_init_fair(orc):
+    L = orc.lattice("sc", 64, 64, 16)
+    s = orc.ising_init(L, 42)
+    n = len(s)
+    assert set(np.unique(s)) == {-1, 1}
+    assert abs(s.mean()) < 5 / np.sqrt(n)
+    assert not np.array_equal(s, orc.ising_init(L, 43))
+
+
+# --------------------------------------- checkerboard vs exact and closed ----
+def test_checkerboard_ising_matches_exact_enumeration(orc):
+    """4x4x1 (x,y periodic, z open) Compound(Exchange, Zeeman): the
+    checkerboard oracle's long-run averages equal the exact Boltzmann ones."""
+    L = orc.lattice("sc", 4, 4, 1, pbc=(1, 1, 0))
+    n = 16
+    neigh = [[j for j in orc.neighbours(L, i) if j >= 0] for i in range(n)]
+    for T, h in [(2.5, 0.0), (1.8, 0.4), (4.0, -0.7)]:
+        M = orc.model(terms=("exchange", "zeeman"), temperature=T, field=h)
+        spins = orc.ising_init(L, 3)
+        orc.ising_sweeps(L, M, 11, 0, 2000, spins)
+        sums = orc.ising_sweeps(L, M, 11, 2000, 200000, spins)
+        bond, mag = sums[:, 0].astype(float), sums[:, 3].astype(float)
+        e = -2.0 * bond + abs(h) * mag
+        ee, em, _, _ = ising_exact(neigh, n, 1.0, h, 1.0, T)
+        me, se = binned(e)
+        mm, sm = binned(np.abs(mag))
+        assert abs(me - ee) < 4 * se, (T, h, me, ee, se)
+        assert abs(mm - em) < 4 * sm, (T, h, mm, em, sm)
+
+
+def test_sequential_matches_exact_enumeration(orc):
+    L = orc.lattice("sc", 4, 4, 1, pbc=(1, 1, 0))
+    n = 16
+    neigh = [[j for j in orc.neighbours(L, i) if j >= 0] for i in range(n)]
+    for T, h, proposal in [(2.5, 0.0, "flip"), (1.8, 0.4, "random")]:
+        M = orc.model(terms=("exchange", "zeeman"), temperature=T, field=h, proposal=proposal)
+        s = orc.Sequential(L, M, 5)
+        s.set_state(np.ones(n, dtype=np.int8))
+        s.set_thermostat(T, h)
+        s.sweeps(2000, observe=False)
+        e, m = s.sweeps(100000)
+        ee, em, _, _ = ising_exact(neigh, n, 1.0, h, 1.0, T)
+        me, se = binned(e)
+        mm, sm = binned(m)
+        assert abs(me - ee) < 4 * se, (T, h, me, ee, se)
+        assert abs(mm - em) < 4 * sm, (T, h, mm, em, sm)
+
+
+def test_checkerboard_ising_random_proposal_matches_exact(orc):
+    L = orc.lattice("sc", 4, 4, 1, pbc=(1, 1, 0))
+    neigh = [[j for j in orc.neighbours(L, i) if j >= 0] for i in range(16)]
+    M = orc.model(terms=("exchange", "zeeman"), temperature=2.2, field=0.3, proposal="random")
+    spins = orc.ising_init(L, 8)
+    sums = orc.ising_sweeps(L, M, 12, 0, 150000, spins)[1000:]
+    e = -2.0 * sums[:, 0] + 0.3 * sums[:, 3]
+    ee, em, _, _ = ising_exact(neigh, 16, 1.0, 0.3, 1.0, 2.2)
+    me, se = binned(e)
+    mm, sm = binned(np.abs(sums[:, 3]))
+    assert abs(me - ee) < 4 * se and abs(mm - em) < 4 * sm
+
+
+def _langevin(a):
+    return 1.0 / math.tanh(a) - 1.0 / a
+
+
+def test_checkerboard_heisenberg_single_spin_closed_forms(orc):
+    """With only Zeeman, sites are independent: <s.h> = -L(|h|/T) (the
+    per-site energy is +|h| s.h, Q1).  With only anisotropy K along z,
+    <s_z^2> = int x^2 e^{-K x^2/T} / int e^{-K x^2/T} over [-1,1]."""
+    L = orc.lattice("sc", 8, 8, 8)
+    h, T = 0.6, 0.8
+    M = orc.model(spin="heisenberg", proposal="random", terms=("zeeman",), temperature=T,
+                  field=h)
+    xyz = orc.heis_init(L, 1)
+    sums = orc.heis_sweeps(L, M, 2, 0, 3000, xyz)[100:]
+    mz = sums[:, 3] / 2 ** 27 / 512
+    mean, err = binned(mz)
+    assert z_score(mean, err, -_langevin(h / T), 0.0) < 4
+    K = 1.5
+    M = orc.model(spin="heisenberg", proposal="random", terms=("anisotropy",), anisotropy=K,
+                  temperature=T)
+    sums = orc.heis_sweeps(L, M, 3, 0, 3000, xyz)[100:]
+    a2 = sums[:, 4] / 2 ** 27 / 512
+    num = integrate.quad(lambda x: x * x * math.exp(-K * x * x / T), -1, 1)[0]
+    den = integrate.quad(lambda x: math.exp(-K * x * x / T), -1, 1)[0]
+    mean, err = binned(a2)
+    assert z_score(mean, err, num / den, 0.0) < 4
+
+
+def test_checkerboard_ising_zeeman_only_closed_form(orc):
+    L = orc.lattice("sc", 8, 8, 8)
+    h, T = 0.5, 1.3
+    M = orc.model(terms=("zeeman",), temperature=T, field=h)
+    spins = orc.ising_init(L, 4)
+    sums = orc.ising_sweeps(L, M, 9, 0, 4000, spins)[50:]
+    mean, err = binned(sums[:, 3] / 512)
+    assert z_score(mean, err, -math.tanh(h / T), 0.0) < 4
+
+
+def test_heisenberg_flip_proposal_conserves_norm(orc):
+    L = orc.lattice("sc", 6, 6, 6)
+    M = orc.model(spin="heisenberg", proposal="flip", terms=("exchange", "zeeman"),
+                  temperature=1.0, field=0.2)
+    xyz = orc.heis_init(L, 5)
+    before = np.linalg.norm(xyz, axis=1)
+    orc.heis_sweeps(L, M, 6, 0, 20, xyz)
+    assert np.array_equal(np.linalg.norm(xyz, axis=1), before)

@@ -1,0 +1,871 @@
+// C ABI of the engine (declared in include/vegas_hip.h): contexts, batches of
+// chains packed in per-kernel arenas, the sweep-block driver (plain launches
+// or a replayed hipGraph), and the host side of the observables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vegas_hip.h"
+#include "vg_device.hpp"
+
+namespace vg {
+__global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
+                              uint32_t, uint32_t, uint32_t);
+__global__ void ising_reduce(const ChainDev*, const int32_t*, uint32_t, uint32_t, int64_t*);
+__global__ void ising_init(const ChainDev*, uint8_t*);
+__global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t);
+__global__ void ising_gather(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
+__global__ void heis_sc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
+                             uint32_t, uint32_t, uint32_t);
+__global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t, int64_t*);
+__global__ void heis_clear(const ChainDev*, float*);
+__global__ void heis_init(const ChainDev*, float*);
+__global__ void heis_scatter(const ChainDev*, float*, const double*, uint64_t);
+__global__ void heis_gather(const ChainDev*, const float*, double*, uint64_t);
+__global__ void zero_i64(int64_t*, uint64_t);
+}  // namespace vg
+
+using vg::ChainDev;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define VG_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE, \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,  \
+                        __LINE__);                                                        \
+    } while (0)
+
+uint64_t round_up(uint64_t v, uint64_t m) { return (v + m - 1) / m * m; }
+
+// Best-effort cleanup: errors there cannot be reported to anyone.
+inline void quiet(hipError_t) {}
+
+// Kernel groups: chains of one group share an arena and one pass kernel.
+enum GroupKind { kIsingSc = 0, kHeisSc = 1, kNumGroups = 2 };
+
+struct Group {
+    std::vector<uint32_t> members;  // batch chain indices
+    std::vector<ChainDev> host;     // descriptors, uploaded each sweep block
+    ChainDev* d_chains = nullptr;
+    ChainDev* h_staging = nullptr;  // pinned copy used for the upload
+    uint32_t* d_block_chain = nullptr;
+    uint32_t total_blocks = 0;
+    uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
+    void* arena = nullptr;
+    uint64_t arena_bytes = 0;
+    void* partials = nullptr;  // [sweep][colour][block][quantity]
+    uint64_t partial_bytes_per_sweep = 0;
+};
+
+struct ChainHost {
+    vg_chain_desc desc;
+    uint64_t n_sites = 0;
+    uint64_t sweeps = 0;
+    double temperature = 2.8;  // Thermostat::new(2.8, Field::zero()) of input.rs:274
+    double field_mag = 0.0;
+    double field_dir[3] = {0.0, 0.0, 1.0};
+    int group = 0;
+    uint32_t index = 0;  // index in its group
+};
+
+}  // namespace
+
+struct vg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct vg_batch {
+    vg_ctx* ctx = nullptr;
+    std::vector<ChainHost> chains;
+    Group groups[kNumGroups];
+    uint32_t max_block_sweeps = 256;
+    int64_t* d_records = nullptr;  // [chain][sweep][5]
+    int64_t* h_records = nullptr;  // pinned
+    bool use_graph = false;
+    std::map<uint32_t, hipGraphExec_t> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    // Per-launch timing of the colour-pass kernels (plain launches only).
+    bool timing = false;
+    std::vector<hipEvent_t> pass_events;  // start/stop pairs
+    uint32_t pass_events_used = 0;
+    double pass_ms[kNumGroups] = {0.0, 0.0};
+    uint64_t pass_launches[kNumGroups] = {0, 0};
+};
+
+namespace {
+
+int coordination(int kind) { return kind == VG_LATTICE_SC ? 6 : 8; }
+
+int validate_desc(const vg_chain_desc& d) {
+    const vg_lattice& L = d.lattice;
+    if (L.kind != VG_LATTICE_SC && L.kind != VG_LATTICE_BCC)
+        return fail(VG_ERR_UNSUPPORTED_LATTICE, "unknown lattice kind %d", L.kind);
+    if (L.lx == 0 || L.ly == 0 || L.lz == 0)
+        return fail(VG_ERR_INVALID_ARGUMENT, "lattice dimensions must be positive");
+    if (L.pbc & ~7u) return fail(VG_ERR_INVALID_ARGUMENT, "unknown pbc bits 0x%x", L.pbc);
+    if (L.kind == VG_LATTICE_SC) {
+        if (L.lx % 2)
+            return fail(VG_ERR_UNSUPPORTED_LATTICE,
+                        "sc lattice needs an even x length for the two-colour layout (lx=%u)",
+                        L.lx);
+        const uint32_t dims[3] = {L.lx, L.ly, L.lz};
+        for (int a = 0; a < 3; ++a)
+            if ((L.pbc >> a) & 1u && dims[a] % 2)
+                return fail(VG_ERR_UNSUPPORTED_LATTICE,
+                            "periodic sc axis %c has odd length %u: not bipartite", "xyz"[a],
+                            dims[a]);
+    } else {
+        return fail(VG_ERR_NOT_IMPLEMENTED, "bcc lattices are not implemented yet");
+    }
+    const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
+    if ((uint64_t)L.ly * L.lz > 0xFFFFFFFFull || cells > (1ull << 34))
+        return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
+    if (d.spin != VG_SPIN_ISING && d.spin != VG_SPIN_HEISENBERG) {
+        if (d.spin == VG_SPIN_ISING_BITS)
+            return fail(VG_ERR_NOT_IMPLEMENTED, "bit-packed Ising chains are not implemented yet");
+        return fail(VG_ERR_INVALID_ARGUMENT, "unknown spin kind %d", d.spin);
+    }
+    if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM)
+        return fail(VG_ERR_INVALID_ARGUMENT, "unknown proposal %d", d.proposal);
+    const vg_hamiltonian& H = d.hamiltonian;
+    if (H.terms & ~15u) return fail(VG_ERR_INVALID_ARGUMENT, "unknown term bits 0x%x", H.terms);
+    if (H.terms == 0) return fail(VG_ERR_INVALID_ARGUMENT, "hamiltonian has no terms");
+    const int nterms = __builtin_popcount(H.terms);
+    if (nterms > 1 && !H.compound)
+        return fail(VG_ERR_INVALID_ARGUMENT, "several terms need compound = 1");
+    if (nterms == 1 && H.compound)
+        return fail(VG_ERR_INVALID_ARGUMENT, "a Compound needs at least two terms");
+    return VG_OK;
+}
+
+// Acceptance thresholds of an Ising chain for its current thermostat.
+// Class (b, S): current spin s = 2b-1, neighbour sum sigma = S - z.
+// dE = ((2*J)*s)*sigma - ((2*|h|)*hz)*s, evaluated in f64 left to right;
+// dE <= 0 is always accepted (the reference accepts dE < 0 outright and
+// dE = 0 with probability 1), otherwise the draw k (uniform 32-bit) is
+// accepted iff k < ceil(exp(-dE/T) * 2^32).
+void ising_thresholds(const ChainHost& ch, ChainDev& d) {
+    const vg_hamiltonian& H = ch.desc.hamiltonian;
+    const int z = coordination(ch.desc.lattice.kind);
+    const double J = (H.terms & VG_TERM_EXCHANGE) ? H.exchange : 0.0;
+    const double hm = (H.terms & VG_TERM_ZEEMAN) ? std::fabs(ch.field_mag) : 0.0;
+    const double hz = ch.field_dir[2] >= 0.0 ? 1.0 : -1.0;
+    const double T = ch.temperature;
+    for (int b = 0; b < 2; ++b) {
+        d.always[b] = 0;
+        const double s = b ? 1.0 : -1.0;
+        for (int S = 0; S < 17; ++S) {
+            d.thr[b][S] = 0;
+            if (S > 2 * z) continue;
+            const double sigma = (double)(S - z);
+            const double dE = 2.0 * J * s * sigma - 2.0 * hm * hz * s;
+            if (dE <= 0.0) {
+                d.always[b] |= 1u << S;
+                continue;
+            }
+            const double th = std::ceil(std::ldexp(std::exp(-dE / T), 32));
+            if (th >= 4294967296.0)
+                d.always[b] |= 1u << S;
+            else
+                d.thr[b][S] = (uint32_t)th;
+        }
+    }
+}
+
+void heis_params(const ChainHost& ch, ChainDev& d) {
+    const vg_hamiltonian& H = ch.desc.hamiltonian;
+    d.jn = (H.terms & VG_TERM_EXCHANGE) ? (float)(-H.exchange) : 0.0f;
+    d.kk = (H.terms & VG_TERM_ANISOTROPY) ? (float)H.anisotropy : 0.0f;
+    d.hm = (H.terms & VG_TERM_ZEEMAN) ? (float)std::fabs(ch.field_mag) : 0.0f;
+    d.beta = (float)(1.0 / ch.temperature);
+    d.ax = (float)H.anisotropy_axis[0];
+    d.ay = (float)H.anisotropy_axis[1];
+    d.az = (float)H.anisotropy_axis[2];
+    d.fx = (float)ch.field_dir[0];
+    d.fy = (float)ch.field_dir[1];
+    d.fz = (float)ch.field_dir[2];
+}
+
+void refresh_params(vg_batch* B, uint32_t i) {
+    ChainHost& ch = B->chains[i];
+    ChainDev& d = B->groups[ch.group].host[ch.index];
+    if (ch.group == kIsingSc)
+        ising_thresholds(ch, d);
+    else
+        heis_params(ch, d);
+}
+
+uint32_t quantities(int group) { return group == kIsingSc ? 2u : 5u; }
+uint32_t quantity_bytes(int group) { return group == kIsingSc ? 4u : 8u; }
+
+int upload_descriptors(vg_batch* B, int g) {
+    Group& G = B->groups[g];
+    if (G.members.empty()) return VG_OK;
+    std::memcpy(G.h_staging, G.host.data(), G.host.size() * sizeof(ChainDev));
+    VG_HIP(hipMemcpyAsync(G.d_chains, G.h_staging, G.host.size() * sizeof(ChainDev),
+                          hipMemcpyHostToDevice, B->ctx->stream));
+    return VG_OK;
+}
+
+int free_batch(vg_batch* B) {
+    if (!B) return VG_OK;
+    for (auto& kv : B->graphs) quiet(hipGraphExecDestroy(kv.second));
+    B->graphs.clear();
+    for (Group& G : B->groups) {
+        if (G.d_chains) quiet(hipFree(G.d_chains));
+        if (G.h_staging) quiet(hipHostFree(G.h_staging));
+        if (G.d_block_chain) quiet(hipFree(G.d_block_chain));
+        if (G.arena) quiet(hipFree(G.arena));
+        if (G.partials) quiet(hipFree(G.partials));
+    }
+    if (B->d_records) quiet(hipFree(B->d_records));
+    if (B->h_records) quiet(hipHostFree(B->h_records));
+    if (B->ev0) quiet(hipEventDestroy(B->ev0));
+    if (B->ev1) quiet(hipEventDestroy(B->ev1));
+    delete B;
+    return VG_OK;
+}
+
+// Enqueues the sweeps of one block (nsw sweeps, both colours of every group,
+// then the per-chain reductions) on the batch stream.
+int enqueue_block(vg_batch* B, uint32_t nsw) {
+    hipStream_t st = B->ctx->stream;
+    const uint64_t nrec = (uint64_t)B->chains.size() * nsw * 5u;
+    hipLaunchKernelGGL(vg::zero_i64, dim3((uint32_t)((nrec + 255) / 256)), dim3(256), 0, st,
+                       B->d_records, nrec);
+    for (uint32_t s = 0; s < nsw; ++s) {
+        for (int g = 0; g < kNumGroups; ++g) {
+            Group& G = B->groups[g];
+            if (G.members.empty()) continue;
+            for (uint32_t c = 0; c < 2; ++c) {
+                const bool timed = B->timing && !B->use_graph;
+                if (timed) {
+                    const uint32_t k = B->pass_events_used;
+                    if (k + 2 > B->pass_events.size()) return fail(VG_ERR_DEVICE, "timing events exhausted");
+                    VG_HIP(hipEventRecord(B->pass_events[k], st));
+                }
+                if (g == kIsingSc)
+                    hipLaunchKernelGGL(vg::ising_sc_pass, dim3(G.grid), dim3(vg::kBlock), 0, st,
+                                       G.d_chains, G.d_block_chain, (uint8_t*)G.arena,
+                                       (int32_t*)G.partials, c, s, G.total_blocks, G.grid);
+                else
+                    hipLaunchKernelGGL(vg::heis_sc_pass, dim3(G.grid), dim3(vg::kBlock), 0, st,
+                                       G.d_chains, G.d_block_chain, (float*)G.arena,
+                                       (int64_t*)G.partials, c, s, G.total_blocks, G.grid);
+                if (timed) {
+                    VG_HIP(hipEventRecord(B->pass_events[B->pass_events_used + 1], st));
+                    B->pass_events_used += 2;
+                }
+            }
+        }
+    }
+    for (int g = 0; g < kNumGroups; ++g) {
+        Group& G = B->groups[g];
+        if (G.members.empty()) continue;
+        dim3 grid((uint32_t)G.members.size(), nsw);
+        if (g == kIsingSc)
+            hipLaunchKernelGGL(vg::ising_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
+                               (const int32_t*)G.partials, G.total_blocks, nsw, B->d_records);
+        else
+            hipLaunchKernelGGL(vg::heis_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
+                               (const int64_t*)G.partials, G.total_blocks, nsw, B->d_records);
+    }
+    VG_HIP(hipGetLastError());
+    return VG_OK;
+}
+
+int launch_block(vg_batch* B, uint32_t nsw) {
+    if (!B->use_graph) return enqueue_block(B, nsw);
+    hipStream_t st = B->ctx->stream;
+    auto it = B->graphs.find(nsw);
+    if (it == B->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        VG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_block(B, nsw);
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc != VG_OK) {
+            if (graph) quiet(hipGraphDestroy(graph));
+            return rc;
+        }
+        if (e != hipSuccess) return fail(VG_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        quiet(hipGraphDestroy(graph));
+        if (e != hipSuccess)
+            return fail(VG_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+        it = B->graphs.emplace(nsw, exec).first;
+    }
+    VG_HIP(hipGraphLaunch(it->second, st));
+    return VG_OK;
+}
+
+double sums_scale(int spin) { return spin == VG_SPIN_HEISENBERG ? 1.0 / 134217728.0 : 1.0; }
+
+// Host side of the observables.  E follows Hamiltonian::total_energy of the
+// reference (energy.rs:55-59 default for Compound; each term's own override
+// for a bare term) and |M| follows State::magnetization().magnitude()
+// (state.rs:86-92, 150-160, 291-296).
+void observables(const ChainHost& ch, const vg_sweep_sums& r, double* E, double* M) {
+    const vg_hamiltonian& H = ch.desc.hamiltonian;
+    const double sc = sums_scale(ch.desc.spin);
+    const double N = (double)ch.n_sites;
+    const double bond = (double)r.bond * sc;
+    double m[3] = {(double)r.mag[0] * sc, (double)r.mag[1] * sc, (double)r.mag[2] * sc};
+    double axis2 = (double)r.axis2 * sc;
+    double hdir[3] = {ch.field_dir[0], ch.field_dir[1], ch.field_dir[2]};
+    if (ch.desc.spin != VG_SPIN_HEISENBERG) {
+        axis2 = N;  // (s.axis)^2 = 1 for Ising spins
+        hdir[0] = hdir[1] = 0.0;
+        hdir[2] = ch.field_dir[2] >= 0.0 ? 1.0 : -1.0;
+    }
+    const double hm = std::fabs(ch.field_mag);
+    const double mdoth = m[0] * hdir[0] + m[1] * hdir[1] + m[2] * hdir[2];
+    double e = 0.0;
+    if (H.compound) {
+        if (H.terms & VG_TERM_EXCHANGE) e += -2.0 * H.exchange * bond;
+        if (H.terms & VG_TERM_ZEEMAN) e += hm * mdoth;
+        if (H.terms & VG_TERM_ANISOTROPY) e += H.anisotropy * axis2;
+        if (H.terms & VG_TERM_GAUGE) e += H.gauge * N;
+    } else if (H.terms & VG_TERM_EXCHANGE) {
+        e = -H.exchange * bond;
+    } else if (H.terms & VG_TERM_ZEEMAN) {
+        e = -hm * mdoth;
+    } else if (H.terms & VG_TERM_ANISOTROPY) {
+        e = axis2;
+    } else {
+        e = H.gauge * N;
+    }
+    double mag;
+    if (ch.desc.spin == VG_SPIN_HEISENBERG) {
+        mag = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        if (std::fabs(mag) < DBL_EPSILON) mag = 0.0;
+    } else {
+        mag = std::fabs(m[2]);
+    }
+    if (E) *E = e;
+    if (M) *M = mag;
+}
+
+int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
+             double* magnetization) {
+    if (!B) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    const uint64_t nch = B->chains.size();
+    hipStream_t st = B->ctx->stream;
+    VG_HIP(hipSetDevice(B->ctx->device));
+    B->last_ms = 0.0;
+    uint64_t done = 0;
+    while (done < sweeps) {
+        const uint32_t nsw = (uint32_t)std::min<uint64_t>(B->max_block_sweeps, sweeps - done);
+        for (uint64_t i = 0; i < nch; ++i) {
+            ChainHost& ch = B->chains[i];
+            B->groups[ch.group].host[ch.index].sweep0 = ch.sweeps;
+        }
+        for (int g = 0; g < kNumGroups; ++g) {
+            int rc = upload_descriptors(B, g);
+            if (rc) return rc;
+        }
+        if (B->timing && !B->use_graph) {
+            uint32_t groups = 0;
+            for (const Group& G : B->groups) groups += G.members.empty() ? 0 : 1;
+            const size_t need = (size_t)nsw * groups * 4;
+            while (B->pass_events.size() < need) {
+                hipEvent_t e;
+                VG_HIP(hipEventCreate(&e));
+                B->pass_events.push_back(e);
+            }
+            B->pass_events_used = 0;
+        }
+        VG_HIP(hipEventRecord(B->ev0, st));
+        int rc = launch_block(B, nsw);
+        if (rc) return rc;
+        VG_HIP(hipEventRecord(B->ev1, st));
+        VG_HIP(hipMemcpyAsync(B->h_records, B->d_records, nch * nsw * 5u * sizeof(int64_t),
+                              hipMemcpyDeviceToHost, st));
+        VG_HIP(hipStreamSynchronize(st));
+        float ms = 0.0f;
+        VG_HIP(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+        B->last_ms += ms;
+        if (B->timing && !B->use_graph) {
+            // Events were recorded in launch order: sweep, group, colour.
+            uint32_t k = 0;
+            for (uint32_t sw = 0; sw < nsw; ++sw)
+                for (int g = 0; g < kNumGroups; ++g) {
+                    if (B->groups[g].members.empty()) continue;
+                    for (int c = 0; c < 2; ++c, k += 2) {
+                        float t = 0.0f;
+                        VG_HIP(hipEventElapsedTime(&t, B->pass_events[k], B->pass_events[k + 1]));
+                        B->pass_ms[g] += t;
+                        B->pass_launches[g] += 1;
+                    }
+                }
+        }
+        for (uint64_t i = 0; i < nch; ++i) {
+            ChainHost& ch = B->chains[i];
+            for (uint32_t s = 0; s < nsw; ++s) {
+                const int64_t* r = B->h_records + (i * nsw + s) * 5u;
+                vg_sweep_sums v;
+                v.bond = r[0];
+                v.mag[0] = r[1];
+                v.mag[1] = r[2];
+                v.mag[2] = r[3];
+                v.axis2 = ch.desc.spin == VG_SPIN_HEISENBERG ? r[4] : (int64_t)ch.n_sites;
+                const uint64_t o = i * sweeps + done + s;
+                if (sums) sums[o] = v;
+                if (energy || magnetization)
+                    observables(ch, v, energy ? energy + o : nullptr,
+                                magnetization ? magnetization + o : nullptr);
+            }
+            ch.sweeps += nsw;
+        }
+        done += nsw;
+    }
+    return VG_OK;
+}
+
+int check_chain(const vg_batch* B, uint32_t chain) {
+    if (!B) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    if (chain >= B->chains.size())
+        return fail(VG_ERR_INVALID_ARGUMENT, "chain %u out of range (%zu chains)", chain,
+                    B->chains.size());
+    return VG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vg_last_error(void) { return g_error.c_str(); }
+
+const char* vg_version(void) { return "vegas-hip 0.1.0 (gfx950)"; }
+
+double vg_sums_scale(int32_t spin) { return sums_scale(spin); }
+
+void vg_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
+    const vg::U4 r = vg::philox(vg::U4{counter[0], counter[1], counter[2], counter[3]}, key[0], key[1]);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+
+int vg_device_count(int* count) {
+    if (!count) return fail(VG_ERR_INVALID_ARGUMENT, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return VG_OK;
+}
+
+int vg_ctx_create(int device, vg_ctx** out) {
+    if (!out) return fail(VG_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(VG_ERR_NO_DEVICE, "no HIP device is visible");
+    if (device < 0 || device >= n)
+        return fail(VG_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, n);
+    VG_HIP(hipSetDevice(device));
+    vg_ctx* c = new vg_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(VG_ERR_DEVICE, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return VG_OK;
+}
+
+int vg_ctx_destroy(vg_ctx* ctx) {
+    if (!ctx) return VG_OK;
+    quiet(hipSetDevice(ctx->device));
+    quiet(hipStreamSynchronize(ctx->stream));
+    quiet(hipStreamDestroy(ctx->stream));
+    delete ctx;
+    return VG_OK;
+}
+
+int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, vg_batch** out) {
+    if (!ctx || !descs || !out) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (n_chains == 0) return fail(VG_ERR_INVALID_ARGUMENT, "a batch needs at least one chain");
+    for (uint32_t i = 0; i < n_chains; ++i) {
+        int rc = validate_desc(descs[i]);
+        if (rc) return fail(rc, "chain %u: %s", i, g_error.c_str());
+    }
+    VG_HIP(hipSetDevice(ctx->device));
+    vg_batch* B = new vg_batch;
+    B->ctx = ctx;
+    B->chains.resize(n_chains);
+    for (uint32_t i = 0; i < n_chains; ++i) {
+        ChainHost& ch = B->chains[i];
+        ch.desc = descs[i];
+        const vg_lattice& L = ch.desc.lattice;
+        const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
+        ch.n_sites = L.kind == VG_LATTICE_SC ? cells : 2 * cells;
+        ch.group = ch.desc.spin == VG_SPIN_HEISENBERG ? kHeisSc : kIsingSc;
+        Group& G = B->groups[ch.group];
+        ch.index = (uint32_t)G.members.size();
+        G.members.push_back(i);
+    }
+    // Lay out every group: arena offsets, work blocks, partial buffers.
+    for (int g = 0; g < kNumGroups; ++g) {
+        Group& G = B->groups[g];
+        if (G.members.empty()) continue;
+        G.host.resize(G.members.size());
+        uint64_t offset = 0;  // in elements (bytes for Ising, floats for Heisenberg)
+        std::vector<uint32_t> block_chain;
+        for (size_t k = 0; k < G.members.size(); ++k) {
+            ChainHost& ch = B->chains[G.members[k]];
+            ChainDev& d = G.host[k];
+            std::memset(&d, 0, sizeof d);
+            const vg_lattice& L = ch.desc.lattice;
+            d.seed = ch.desc.seed;
+            d.lx = L.lx;
+            d.ly = L.ly;
+            d.lz = L.lz;
+            d.kind = (uint32_t)L.kind;
+            d.pbc = L.pbc;
+            d.z = (uint32_t)coordination(L.kind);
+            d.rows = L.ly * L.lz;
+            d.w = L.kind == VG_LATTICE_SC ? L.lx / 2 : L.lx;
+            d.proposal = (uint32_t)ch.desc.proposal;
+            d.slot = G.members[k];
+            const uint32_t seg = g == kIsingSc ? vg::kIsingSeg : vg::kHeisSeg;
+            const uint32_t align = g == kIsingSc ? 64u : 16u;  // 64-byte aligned rows
+            d.pitch = (uint32_t)round_up(d.w, align);
+            d.segs = (d.w + seg - 1) / seg;
+            const uint64_t items = (uint64_t)d.rows * d.segs;
+            if (items > 0xFFFFFFFFull - vg::kBlock) {
+                free_batch(B);
+                return fail(VG_ERR_INVALID_ARGUMENT, "chain %u too large", G.members[k]);
+            }
+            d.items = (uint32_t)items;
+            d.nblocks = (uint32_t)((items + vg::kBlock - 1) / vg::kBlock);
+            d.block0 = (uint32_t)block_chain.size();
+            for (uint32_t bidx = 0; bidx < d.nblocks; ++bidx) block_chain.push_back((uint32_t)k);
+            d.plane = (uint64_t)d.rows * d.pitch;
+            const uint64_t planes_per_colour = g == kIsingSc ? 1 : 3;
+            const uint64_t elem_align = g == kIsingSc ? 256 : 64;  // 256-byte chain alignment
+            offset = round_up(offset, elem_align);
+            d.off[0] = offset;
+            offset += planes_per_colour * d.plane;
+            offset = round_up(offset, elem_align);
+            d.off[1] = offset;
+            offset += planes_per_colour * d.plane;
+            refresh_params(B, G.members[k]);
+        }
+        G.total_blocks = (uint32_t)block_chain.size();
+        G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
+        const uint64_t elem = g == kIsingSc ? 1 : 4;
+        G.arena_bytes = std::max<uint64_t>(offset * elem, 256);
+        hipError_t e = hipMalloc(&G.arena, G.arena_bytes);
+        if (e == hipSuccess) e = hipMemset(G.arena, 0, G.arena_bytes);
+        if (e == hipSuccess) e = hipMalloc(&G.d_chains, G.host.size() * sizeof(ChainDev));
+        if (e == hipSuccess) e = hipHostMalloc(&G.h_staging, G.host.size() * sizeof(ChainDev));
+        if (e == hipSuccess) e = hipMalloc(&G.d_block_chain, block_chain.size() * sizeof(uint32_t));
+        if (e == hipSuccess)
+            e = hipMemcpy(G.d_block_chain, block_chain.data(), block_chain.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            free_batch(B);
+            return fail(e == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE,
+                        "allocating the chain arena (%llu bytes): %s",
+                        (unsigned long long)G.arena_bytes, hipGetErrorString(e));
+        }
+        G.partial_bytes_per_sweep = 2ull * G.total_blocks * quantities(g) * quantity_bytes(g);
+    }
+    // Sweep-block length: keep the partial buffers within 64 MiB per group.
+    uint64_t max_per_sweep = 1;
+    for (Group& G : B->groups) max_per_sweep = std::max(max_per_sweep, G.partial_bytes_per_sweep);
+    B->max_block_sweeps =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(256, (64ull << 20) / max_per_sweep));
+    for (int g = 0; g < kNumGroups; ++g) {
+        Group& G = B->groups[g];
+        if (G.members.empty()) continue;
+        hipError_t e = hipMalloc(&G.partials, G.partial_bytes_per_sweep * B->max_block_sweeps);
+        if (e != hipSuccess) {
+            free_batch(B);
+            return fail(VG_ERR_OUT_OF_MEMORY, "allocating partial sums: %s", hipGetErrorString(e));
+        }
+    }
+    const uint64_t rec_bytes = (uint64_t)n_chains * B->max_block_sweeps * 5u * sizeof(int64_t);
+    hipError_t e = hipMalloc(&B->d_records, rec_bytes);
+    if (e == hipSuccess) e = hipHostMalloc(&B->h_records, rec_bytes);
+    if (e == hipSuccess) e = hipEventCreate(&B->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&B->ev1);
+    if (e != hipSuccess) {
+        free_batch(B);
+        return fail(VG_ERR_DEVICE, "allocating records: %s", hipGetErrorString(e));
+    }
+    for (int g = 0; g < kNumGroups; ++g) {
+        int rc = upload_descriptors(B, g);
+        if (rc) {
+            free_batch(B);
+            return rc;
+        }
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        free_batch(B);
+        return fail(VG_ERR_DEVICE, "%s", hipGetErrorString(e));
+    }
+    *out = B;
+    return VG_OK;
+}
+
+int vg_batch_destroy(vg_batch* batch) {
+    if (!batch) return VG_OK;
+    quiet(hipSetDevice(batch->ctx->device));
+    quiet(hipStreamSynchronize(batch->ctx->stream));
+    return free_batch(batch);
+}
+
+int vg_batch_size(const vg_batch* batch, uint32_t* n) {
+    if (!batch || !n) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    *n = (uint32_t)batch->chains.size();
+    return VG_OK;
+}
+
+int vg_batch_num_sites(const vg_batch* batch, uint32_t chain, uint64_t* n) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!n) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    *n = batch->chains[chain].n_sites;
+    return VG_OK;
+}
+
+int vg_batch_set_thermostat(vg_batch* batch, uint32_t chain, double temperature,
+                            double field_magnitude, const double field_orientation[3]) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (std::isnan(temperature) || std::isnan(field_magnitude))
+        return fail(VG_ERR_INVALID_ARGUMENT, "NaN thermostat parameter");
+    ChainHost& ch = batch->chains[chain];
+    ch.temperature = temperature < DBL_EPSILON ? DBL_EPSILON : temperature;
+    ch.field_mag = field_magnitude;
+    if (field_orientation) {
+        for (int k = 0; k < 3; ++k) ch.field_dir[k] = field_orientation[k];
+    } else {
+        ch.field_dir[0] = ch.field_dir[1] = 0.0;
+        ch.field_dir[2] = 1.0;
+    }
+    refresh_params(batch, chain);
+    return VG_OK;
+}
+
+int vg_batch_randomize(vg_batch* batch) {
+    if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    hipStream_t st = batch->ctx->stream;
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    for (int g = 0; g < kNumGroups; ++g) {
+        int rc = upload_descriptors(batch, g);
+        if (rc) return rc;
+    }
+    for (ChainHost& ch : batch->chains) {
+        Group& G = batch->groups[ch.group];
+        const ChainDev& d = G.host[ch.index];
+        const ChainDev* dp = G.d_chains + ch.index;
+        if (ch.group == kIsingSc) {
+            const uint64_t n = 2ull * d.plane;
+            hipLaunchKernelGGL(vg::ising_init, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                               dp, (uint8_t*)G.arena);
+        } else {
+            const uint64_t n = 2ull * d.rows * d.w;
+            hipLaunchKernelGGL(vg::heis_init, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                               dp, (float*)G.arena);
+        }
+        ch.sweeps = 0;
+    }
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(st));
+    return VG_OK;
+}
+
+int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!spins) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
+    ChainHost& ch = batch->chains[chain];
+    if (ch.group != kIsingSc) return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Ising", chain);
+    for (uint64_t n = 0; n < ch.n_sites; ++n)
+        if (spins[n] != 1 && spins[n] != -1)
+            return fail(VG_ERR_INVALID_ARGUMENT, "spin %llu is %d, not +1 or -1",
+                        (unsigned long long)n, (int)spins[n]);
+    Group& G = batch->groups[ch.group];
+    hipStream_t st = batch->ctx->stream;
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    rc = upload_descriptors(batch, ch.group);
+    if (rc) return rc;
+    int8_t* tmp = nullptr;
+    VG_HIP(hipMalloc(&tmp, ch.n_sites));
+    hipError_t e = hipMemcpyAsync(tmp, spins, ch.n_sites, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(vg::ising_scatter, dim3((uint32_t)((ch.n_sites + 255) / 256)),
+                           dim3(256), 0, st, G.d_chains + ch.index, (uint8_t*)G.arena, tmp,
+                           ch.n_sites);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    quiet(hipFree(tmp));
+    if (e != hipSuccess) return fail(VG_ERR_DEVICE, "uploading spins: %s", hipGetErrorString(e));
+    return VG_OK;
+}
+
+int vg_batch_get_ising(vg_batch* batch, uint32_t chain, int8_t* spins) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!spins) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
+    ChainHost& ch = batch->chains[chain];
+    if (ch.group != kIsingSc) return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Ising", chain);
+    Group& G = batch->groups[ch.group];
+    hipStream_t st = batch->ctx->stream;
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    rc = upload_descriptors(batch, ch.group);
+    if (rc) return rc;
+    int8_t* tmp = nullptr;
+    VG_HIP(hipMalloc(&tmp, ch.n_sites));
+    hipLaunchKernelGGL(vg::ising_gather, dim3((uint32_t)((ch.n_sites + 255) / 256)), dim3(256), 0,
+                       st, G.d_chains + ch.index, (const uint8_t*)G.arena, tmp, ch.n_sites);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(spins, tmp, ch.n_sites, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    quiet(hipFree(tmp));
+    if (e != hipSuccess) return fail(VG_ERR_DEVICE, "downloading spins: %s", hipGetErrorString(e));
+    return VG_OK;
+}
+
+int vg_batch_set_heisenberg(vg_batch* batch, uint32_t chain, const double* xyz) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!xyz) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
+    ChainHost& ch = batch->chains[chain];
+    if (ch.group != kHeisSc)
+        return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Heisenberg", chain);
+    Group& G = batch->groups[ch.group];
+    hipStream_t st = batch->ctx->stream;
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    rc = upload_descriptors(batch, ch.group);
+    if (rc) return rc;
+    double* tmp = nullptr;
+    const uint64_t bytes = ch.n_sites * 3 * sizeof(double);
+    VG_HIP(hipMalloc(&tmp, bytes));
+    hipError_t e = hipMemcpyAsync(tmp, xyz, bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(vg::heis_scatter, dim3((uint32_t)((ch.n_sites + 255) / 256)), dim3(256),
+                           0, st, G.d_chains + ch.index, (float*)G.arena, tmp, ch.n_sites);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    quiet(hipFree(tmp));
+    if (e != hipSuccess) return fail(VG_ERR_DEVICE, "uploading spins: %s", hipGetErrorString(e));
+    return VG_OK;
+}
+
+int vg_batch_get_heisenberg(vg_batch* batch, uint32_t chain, double* xyz) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!xyz) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
+    ChainHost& ch = batch->chains[chain];
+    if (ch.group != kHeisSc)
+        return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Heisenberg", chain);
+    Group& G = batch->groups[ch.group];
+    hipStream_t st = batch->ctx->stream;
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    rc = upload_descriptors(batch, ch.group);
+    if (rc) return rc;
+    double* tmp = nullptr;
+    const uint64_t bytes = ch.n_sites * 3 * sizeof(double);
+    VG_HIP(hipMalloc(&tmp, bytes));
+    hipLaunchKernelGGL(vg::heis_gather, dim3((uint32_t)((ch.n_sites + 255) / 256)), dim3(256), 0,
+                       st, G.d_chains + ch.index, (const float*)G.arena, tmp, ch.n_sites);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(xyz, tmp, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    quiet(hipFree(tmp));
+    if (e != hipSuccess) return fail(VG_ERR_DEVICE, "downloading spins: %s", hipGetErrorString(e));
+    return VG_OK;
+}
+
+int vg_batch_sweep_count(const vg_batch* batch, uint32_t chain, uint64_t* sweeps) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!sweeps) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    *sweeps = batch->chains[chain].sweeps;
+    return VG_OK;
+}
+
+int vg_batch_set_sweep_count(vg_batch* batch, uint32_t chain, uint64_t sweeps) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (sweeps >> 56) return fail(VG_ERR_INVALID_ARGUMENT, "sweep counter beyond 2^56");
+    batch->chains[chain].sweeps = sweeps;
+    return VG_OK;
+}
+
+int vg_batch_run(vg_batch* batch, uint64_t sweeps, double* energy, double* magnetization) {
+    return run_impl(batch, sweeps, nullptr, energy, magnetization);
+}
+
+int vg_batch_run_sums(vg_batch* batch, uint64_t sweeps, vg_sweep_sums* sums) {
+    if (!sums) return fail(VG_ERR_INVALID_ARGUMENT, "null sums");
+    return run_impl(batch, sweeps, sums, nullptr, nullptr);
+}
+
+int vg_batch_set_graph(vg_batch* batch, int enabled) {
+    if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    batch->use_graph = enabled != 0;
+    return VG_OK;
+}
+
+int vg_batch_set_timing(vg_batch* batch, int enabled) {
+    if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    batch->timing = enabled != 0;
+    for (int g = 0; g < kNumGroups; ++g) {
+        batch->pass_ms[g] = 0.0;
+        batch->pass_launches[g] = 0;
+    }
+    return VG_OK;
+}
+
+int vg_batch_pass_stats(const vg_batch* batch, int32_t spin, double* total_ms, uint64_t* launches) {
+    if (!batch || !total_ms || !launches) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    const int g = spin == VG_SPIN_HEISENBERG ? kHeisSc : kIsingSc;
+    *total_ms = batch->pass_ms[g];
+    *launches = batch->pass_launches[g];
+    return VG_OK;
+}
+
+int vg_batch_last_run_ms(const vg_batch* batch, double* ms) {
+    if (!batch || !ms) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    *ms = batch->last_ms;
+    return VG_OK;
+}
+
+}  // extern "C"

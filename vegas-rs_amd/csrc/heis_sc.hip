@@ -1,0 +1,286 @@
+// fp32 Heisenberg checkerboard colour pass on the simple-cubic lattice, for
+// any number of chains packed in one arena.
+//
+// Replaces MetropolisIntegrator::step (reference src/integrator.rs:66-92:
+// fresh uniform-on-sphere proposal, Spin::rand -> marsaglia, util.rs:21-34)
+// and MetropolisFlipIntegrator::step (109-138: proposal -s) with the local
+// energy of Exchange + UniaxialAnisotropy + Zeeman + Gauge
+// (src/energy.rs:64-214): one sweep updates every colour-0 site, then every
+// colour-1 site.
+//
+// Work item: 4 consecutive positions of one row (one aligned 16-byte load per
+// component per row touched).  The 4 positions share one Philox stream
+// (counter x = i/4, purpose 16 + call index), consumed in position order:
+// each site takes one word for its acceptance uniform, then (random proposal)
+// two words per Marsaglia try until a try lands inside the unit disc.
+// All arithmetic is single IEEE fp32 operations in a fixed order (no
+// contraction), so the CPU oracle reproduces every bit.
+#include "vg_device.hpp"
+
+namespace vg {
+
+__device__ __forceinline__ float fm(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float fa(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float fs(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return fa(fa(fm(ax, bx), fm(ay, by)), fm(az, bz));
+}
+__device__ __forceinline__ int64_t fix27(float v) {
+    return (int64_t)__float2int_rn(fm(v, kFixScale));
+}
+
+__device__ __forceinline__ float4 ld4(const float* p, bool present) {
+    if (!present) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return *reinterpret_cast<const float4*>(p);
+}
+
+__device__ __forceinline__ float f4get(const float4& v, uint32_t k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+// Sequential word stream of one 4-position group.
+struct Stream {
+    U4 cur;
+    uint32_t call, word;
+    uint32_t cx, cy, cz, sweep_hi_bits, k0, k1;
+    __device__ __forceinline__ void refill() {
+        cur = philox(U4{cx, cy, cz, sweep_hi_bits | ((kHeisStream + call) << 1)}, k0, k1);
+        word = 0;
+    }
+    // Returns false once the stream is exhausted (after kHeisMaxCalls calls).
+    __device__ __forceinline__ bool next(uint32_t& out) {
+        if (word == 4) {
+            if (call + 1 >= kHeisMaxCalls) return false;
+            ++call;
+            refill();
+        }
+        out = u4_word(cur, word++);
+        return true;
+    }
+};
+
+__global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__ chains,
+                                                    const uint32_t* __restrict__ block_chain,
+                                                    float* __restrict__ arena,
+                                                    int64_t* __restrict__ partials,
+                                                    uint32_t colour, uint32_t s,
+                                                    uint32_t total_blocks, uint32_t grid) {
+    __shared__ int64_t red[5][kBlock / 64];
+    const uint32_t b = xcd_block(blockIdx.x, grid);
+    if (b >= total_blocks) return;  // uniform over the workgroup
+    const ChainDev& C = chains[block_chain[b]];
+    const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
+    int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
+
+    if (local < C.items) {
+        const uint32_t row = local / C.segs;
+        const uint32_t seg = local - row * C.segs;
+        const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
+        const uint64_t plane = C.plane;
+        const uint32_t y = row % ly, z = row / ly;
+        const uint32_t q = (colour + y + z) & 1u;
+        const uint32_t i0 = seg * kHeisSeg;
+        const uint32_t nvalid = min(kHeisSeg, w - i0);
+        float* own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
+        const float* oth = arena + C.off[colour ^ 1u];
+        const float* orow = oth + (uint64_t)row * pitch;
+
+        const bool ym_ok = (y > 0) || (pbc & 2u);
+        const bool yp_ok = (y + 1 < ly) || (pbc & 2u);
+        const bool zm_ok = (z > 0) || (pbc & 4u);
+        const bool zp_ok = (z + 1 < lz) || (pbc & 4u);
+        const int64_t r_ym = y > 0 ? (int64_t)row - 1 : (int64_t)row + (ly - 1);
+        const int64_t r_yp = y + 1 < ly ? (int64_t)row + 1 : (int64_t)row - (ly - 1);
+        const int64_t r_zm = z > 0 ? (int64_t)row - ly : (int64_t)row + (int64_t)(lz - 1) * ly;
+        const int64_t r_zp =
+            z + 1 < lz ? (int64_t)row + ly : (int64_t)row - (int64_t)(lz - 1) * ly;
+
+        float4 so[3], mid[3], ym[3], yp[3], zm[3], zp[3];
+        float xe[3];  // the extra x neighbour (position i0-1 or i0+4)
+        bool xe_ok = true;
+        uint32_t xe_pos;
+        if (q == 0) {
+            xe_pos = i0 > 0 ? i0 - 1 : w - 1;
+            xe_ok = i0 > 0 || (pbc & 1u);
+        } else {
+            xe_pos = i0 + kHeisSeg < w ? i0 + kHeisSeg : 0;
+            xe_ok = i0 + kHeisSeg < w || (pbc & 1u);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            so[c] = ld4(own + c * plane, true);
+            mid[c] = ld4(orow + c * plane + i0, true);
+            ym[c] = ld4(oth + c * plane + r_ym * (int64_t)pitch + i0, ym_ok);
+            yp[c] = ld4(oth + c * plane + r_yp * (int64_t)pitch + i0, yp_ok);
+            zm[c] = ld4(oth + c * plane + r_zm * (int64_t)pitch + i0, zm_ok);
+            zp[c] = ld4(oth + c * plane + r_zp * (int64_t)pitch + i0, zp_ok);
+            xe[c] = xe_ok ? orow[c * plane + xe_pos] : 0.0f;
+        }
+        // Position w-1 with q = 1 wraps to position 0 (or has no +x bond).
+        float wrap0[3] = {0.0f, 0.0f, 0.0f};
+        if (q == 1 && i0 + kHeisSeg >= w && (pbc & 1u)) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) wrap0[c] = orow[c * plane];
+        }
+
+        const uint64_t t = C.sweep0 + s;
+        Stream st;
+        st.call = 0;
+        st.cx = i0 >> 2;
+        st.cy = row;
+        st.cz = (uint32_t)t;
+        st.sweep_hi_bits = ((uint32_t)(t >> 32) << 8) | colour;
+        st.k0 = (uint32_t)C.seed;
+        st.k1 = (uint32_t)(C.seed >> 32);
+        st.refill();
+        bool live = true;
+
+        const float jn = C.jn, kk = C.kk, hm = C.hm, beta = C.beta;
+        const float ax = C.ax, ay = C.ay, az = C.az, fx = C.fx, fy = C.fy, fz = C.fz;
+        float ns[3][4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const float ox = f4get(so[0], j), oy = f4get(so[1], j), oz = f4get(so[2], j);
+            ns[0][j] = ox;
+            ns[1][j] = oy;
+            ns[2][j] = oz;
+            if (j >= nvalid) continue;
+            const uint32_t pos = i0 + j;
+            // x neighbours, -x then +x: positions pos-1, pos (q = 0) or
+            // pos, pos+1 (q = 1) of the other plane's row.
+            float nmx[3], npx[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float here = f4get(mid[c], j);
+                if (q == 0) {
+                    nmx[c] = j == 0 ? xe[c] : f4get(mid[c], j - 1);
+                    npx[c] = here;
+                } else {
+                    nmx[c] = here;
+                    npx[c] = pos + 1 == w ? wrap0[c] : (j == 3 ? xe[c] : f4get(mid[c], j + 1));
+                }
+            }
+            float h[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float v = fa(nmx[c], npx[c]);
+                v = fa(v, f4get(ym[c], j));
+                v = fa(v, f4get(yp[c], j));
+                v = fa(v, f4get(zm[c], j));
+                v = fa(v, f4get(zp[c], j));
+                h[c] = v;
+            }
+            // Acceptance uniform, then the proposal.
+            uint32_t wu = 0;
+            float px = -ox, py = -oy, pz = -oz;
+            bool ok = live && st.next(wu);
+            live = ok;
+            const float u = unit24(wu);
+            if (C.proposal) {
+                px = 0.0f;
+                py = 0.0f;
+                pz = 1.0f;
+                while (live) {
+                    uint32_t w1, w2;
+                    if (!st.next(w1) || !st.next(w2)) {
+                        live = false;
+                        break;
+                    }
+                    const float x1 = sym24(w1), x2 = sym24(w2);
+                    const float qq = fa(fm(x1, x1), fm(x2, x2));
+                    if (qq < 1.0f) {
+                        const float root = __fsqrt_rn(fs(1.0f, qq));
+                        px = fm(fm(2.0f, x1), root);
+                        py = fm(fm(2.0f, x2), root);
+                        pz = fs(1.0f, fm(2.0f, qq));
+                        break;
+                    }
+                }
+            }
+            const float dx = fs(px, ox), dy = fs(py, oy), dz = fs(pz, oz);
+            const float e_ex = fm(jn, dot3(dx, dy, dz, h[0], h[1], h[2]));
+            const float pe = dot3(px, py, pz, ax, ay, az);
+            const float oe = dot3(ox, oy, oz, ax, ay, az);
+            const float e_an = fm(kk, fs(fm(pe, pe), fm(oe, oe)));
+            const float e_ze = fm(hm, dot3(dx, dy, dz, fx, fy, fz));
+            const float dE = fa(fa(e_ex, e_an), e_ze);
+            bool accept = dE < 0.0f;
+            if (!accept) accept = u < exp_nonpos(-fm(dE, beta));
+            float sx = ox, sy = oy, sz = oz;
+            if (accept) {
+                sx = px;
+                sy = py;
+                sz = pz;
+            }
+            ns[0][j] = sx;
+            ns[1][j] = sy;
+            ns[2][j] = sz;
+            acc[1] += fix27(sx);
+            acc[2] += fix27(sy);
+            acc[3] += fix27(sz);
+            const float se = dot3(sx, sy, sz, ax, ay, az);
+            acc[4] += fix27(fm(se, se));
+            if (colour == 1) acc[0] += fix27(dot3(sx, sy, sz, h[0], h[1], h[2]));
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            *reinterpret_cast<float4*>(own + c * plane) =
+                make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
+    }
+
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) red[k][wave] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        int64_t v = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kBlock / 64; ++k) v += red[threadIdx.x][k];
+        partials[((uint64_t)(s * 2u + colour) * total_blocks + b) * 5u + threadIdx.x] = v;
+    }
+}
+
+// record = {bond, mx, my, mz, axis2}: both colours for the on-site sums,
+// colour 1 for the bond sum.  grid = (chains, sweeps).
+__global__ __launch_bounds__(256) void heis_reduce(const ChainDev* __restrict__ chains,
+                                                   const int64_t* __restrict__ partials,
+                                                   uint32_t total_blocks, uint32_t nsw,
+                                                   int64_t* __restrict__ records) {
+    __shared__ int64_t red[5][kBlock / 64];
+    const ChainDev& C = chains[blockIdx.x];
+    const uint32_t s = blockIdx.y;
+    int64_t acc[5] = {0, 0, 0, 0, 0};
+    for (uint32_t k = threadIdx.x; k < C.nblocks; k += kBlock) {
+        const uint32_t blk = C.block0 + k;
+        const int64_t* p0 = partials + ((uint64_t)(s * 2u + 0u) * total_blocks + blk) * 5u;
+        const int64_t* p1 = partials + ((uint64_t)(s * 2u + 1u) * total_blocks + blk) * 5u;
+        acc[0] += p1[0];
+#pragma unroll
+        for (int q = 1; q < 5; ++q) acc[q] += p0[q] + p1[q];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) red[k][wave] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        int64_t v = 0;
+        for (uint32_t k = 0; k < kBlock / 64; ++k) v += red[threadIdx.x][k];
+        records[((uint64_t)C.slot * nsw + s) * 5u + threadIdx.x] = v;
+    }
+}
+
+}  // namespace vg

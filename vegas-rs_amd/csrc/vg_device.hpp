@@ -1,0 +1,168 @@
+// Shared device-side definitions of the MI355X Metropolis engine: the
+// Philox4x32-10 generator, the counter layout that keys every random draw by
+// (chain seed, sweep, colour, row, position), the chain descriptor the
+// kernels read, and the colour-split spin layout.
+//
+// Layout of one chain (sc lattice lx*ly*lz, bcc lattice lx*ly*lz cells):
+//   The lattice is bipartite.  Colour c of sc site (x,y,z) is (x+y+z)&1; of
+//   a bcc site it is its sublattice.  Each colour is stored as its own plane
+//   of rows; row r = y + ly*z; within a row, position i holds
+//     sc : the colour-c site at x = 2*i + ((c+y+z)&1)   (w = lx/2 positions)
+//     bcc: the sublattice-c site of cell x = i           (w = lx positions)
+//   Rows are `pitch` elements apart (pitch is a multiple of 64 bytes), so a
+//   row is read with aligned 16-byte loads.  Ising planes hold one byte per
+//   site (1 = Up, 0 = Down); Heisenberg chains hold three fp32 planes (x, y,
+//   z components) per colour.  Padding elements past w stay zero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define VG_HD __host__ __device__ __forceinline__
+
+namespace vg {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10, Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11.  Multipliers and Weyl key increments as published
+// there.  counter (x,y,z,w), key (k0,k1) -> 4 x 32 random bits.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+VG_HD U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c.x;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
+        U4 n;
+        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        n.y = (uint32_t)p1;
+        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        n.w = (uint32_t)p0;
+        c = n;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return c;
+}
+
+VG_HD uint32_t u4_word(const U4& v, uint32_t k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+// Counter word w: bit 0 colour, bits 1-7 purpose, bits 8-31 sweep >> 32.
+// Counter words x, y, z: position group, row, sweep & 0xffffffff (or call
+// index for the initial state, which has no sweep).
+enum Purpose : uint32_t {
+    kIsingA = 0,         // high 8 bits of the acceptance draw, 16 positions per call
+    kIsingB = 1,         // low 24 bits of the acceptance draw, 4 positions per call
+    kIsingProposal = 2,  // random-proposal bit, 128 positions per call
+    kIsingInit = 3,      // initial spin, 4 positions per call
+    kHeisInit = 4,       // initial spin stream, one position per stream
+    kHeisStream = 16,    // sweep stream call k uses purpose 16 + k, k < 112
+};
+constexpr uint32_t kHeisMaxCalls = 112;
+constexpr uint32_t kHeisInitMaxCalls = 64;
+
+VG_HD uint32_t ctr_w(uint64_t sweep, uint32_t purpose, uint32_t colour) {
+    return ((uint32_t)(sweep >> 32) << 8) | (purpose << 1) | colour;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed point used for the Heisenberg per-sweep sums: every per-site fp32
+// contribution (|v| <= 8) is rounded to an int32 multiple of 2^-27 and summed
+// in int64, so the sums do not depend on reduction order.
+// ---------------------------------------------------------------------------
+constexpr float kFixScale = 134217728.0f;  // 2^27
+
+// ---------------------------------------------------------------------------
+// Chain descriptor as the kernels see it (one per chain of a kernel group).
+// ---------------------------------------------------------------------------
+struct ChainDev {
+    uint64_t seed;         // Philox key (k0 = low word, k1 = high word)
+    uint64_t sweep0;       // sweep counter at the start of the current block
+    uint64_t off[2];       // element offset of colour plane c in the arena
+    uint64_t plane;        // elements per plane (rows * pitch); Heisenberg y/z planes follow x
+    uint32_t lx, ly, lz, kind;
+    uint32_t w, pitch, rows, pbc;
+    uint32_t segs;         // work items per row
+    uint32_t items;        // work items per pass (rows * segs)
+    uint32_t block0;       // first block of this chain in the pass grid
+    uint32_t nblocks;      // blocks of this chain
+    uint32_t proposal;     // 0 flip, 1 random
+    uint32_t slot;         // chain index in its batch (record row)
+    uint32_t z;            // coordination number: 6 (sc) or 8 (bcc)
+    uint32_t pad0;
+    // Ising acceptance: class (b, S) with b the current bit and S the sum over
+    // neighbour slots of 2*b_j (present) or 1 (absent), S in [0, 2z].
+    uint32_t thr[2][17];   // accept iff draw < thr (32-bit draw)
+    uint32_t always[2];    // bit S set: class always accepted
+    // Heisenberg parameters, fp32.
+    float jn;              // -J
+    float kk;              // K
+    float hm;              // |h|
+    float beta;            // (float)(1/T)
+    float ax, ay, az;      // anisotropy axis
+    float fx, fy, fz;      // field orientation
+    float pad1, pad2, pad3;
+};
+
+constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass
+constexpr uint32_t kIsingSeg = 16;     // positions per Ising work item
+constexpr uint32_t kHeisSeg = 4;       // positions per Heisenberg work item
+constexpr uint32_t kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map hardware block
+// b to a logical block so that each XCD walks one contiguous slab of the work
+// (its L2 then holds the rows its neighbours need).  grid must be a multiple
+// of 8.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid) {
+    const uint32_t per = grid / kNumXcd;
+    return (b % kNumXcd) * per + b / kNumXcd;
+}
+
+// Own Cody-Waite exp for x <= 0 in fp32 (Cody and Waite, "Software Manual for
+// the Elementary Functions", 1980): x = n*ln2 + r, |r| <= ln2/2, e^r by its
+// degree-7 Taylor polynomial (coefficients 1/k!), times 2^n.  ln2 is split
+// into a 16-bit head (0x3F317200) and the fp32-rounded remainder so n*head
+// is exact.  Every operation is a single IEEE fp32 op or an fma, so host and
+// device results are bit-identical.  Returns 0 below -86, where e^x < 2^-124
+// (the caller compares a uniform in [0,1) against it); above, the result is
+// a normal number, so denormal modes never matter.
+VG_HD float exp_nonpos(float x) {
+    if (x < -86.0f) return 0.0f;
+    const float log2e = (float)1.4426950408889634;
+    const float ln2_hi = 0.693145751953125f;
+    const float ln2_lo = (float)(0.6931471805599453 - 0.693145751953125);
+    const float n = rintf(x * log2e);
+    float r = fmaf(n, -ln2_hi, x);
+    r = fmaf(n, -ln2_lo, r);
+    float p = (float)(1.0 / 5040.0);
+    p = fmaf(p, r, (float)(1.0 / 720.0));
+    p = fmaf(p, r, (float)(1.0 / 120.0));
+    p = fmaf(p, r, (float)(1.0 / 24.0));
+    p = fmaf(p, r, (float)(1.0 / 6.0));
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    const int e = (int)n + 127;          // n in [-124, 0]: normal scale factor
+    union {
+        uint32_t u;
+        float f;
+    } s;
+    s.u = (uint32_t)e << 23;
+    return p * s.f;
+}
+
+// 24-bit uniform in [0,1) and in [-1,1) from a 32-bit draw (exact in fp32).
+VG_HD float unit24(uint32_t w) { return (float)(w >> 8) * 5.9604644775390625e-08f; }
+VG_HD float sym24(uint32_t w) { return (float)(w >> 8) * 1.1920928955078125e-07f - 1.0f; }
+
+}  // namespace vg
