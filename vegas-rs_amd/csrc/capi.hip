@@ -195,6 +195,29 @@ void ising_thresholds(const ChainHost& ch, ChainDev& d) {
                 d.thr[b][S] = (uint32_t)th;
         }
     }
+    // Byte tables of the packed decision: entry c of table (g, b) is class
+    // S = 2c + g (c up neighbours among 6 - g present slots).
+    std::memset(d.t8, 0, sizeof d.t8);
+    std::memset(d.a8, 0, sizeof d.a8);
+    for (int g = 0; g < 5; ++g)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 8; ++c) {
+                const int S = 2 * c + g;
+                if (S > 2 * z) continue;
+                const uint32_t al = (d.always[b] >> S) & 1u ? 0xFFu : 0u;
+                const uint32_t hi = al ? 0xFFu : d.thr[b][S] >> 24;
+                d.t8[g][b][c >> 2] |= hi << (8 * (c & 3));
+                d.a8[g][b][c >> 2] |= al << (8 * (c & 3));
+            }
+}
+
+// m, s with n / d == (n * m) >> s for every n < 2^31: s = 31 + ceil(log2 d),
+// m = ceil(2^s / d) < 2^32; the error n*(m - 2^s/d)/2^s < 1/d never carries.
+void magic_div(uint32_t d, uint32_t& m, uint32_t& s) {
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    s = 31 + l;
+    m = (uint32_t)(((1ull << s) + d - 1) / d);
 }
 
 void heis_params(const ChainHost& ch, ChainDev& d) {
@@ -220,7 +243,9 @@ void refresh_params(vg_batch* B, uint32_t i) {
         heis_params(ch, d);
 }
 
-uint32_t quantities(int group) { return group == kIsingSc ? 2u : 5u; }
+// Ising passes write {mag, bond} per wave (4 waves per workgroup); Heisenberg
+// passes write {bond, mx, my, mz, axis2} per workgroup.
+uint32_t quantities(int group) { return group == kIsingSc ? 8u : 5u; }
 uint32_t quantity_bytes(int group) { return group == kIsingSc ? 4u : 8u; }
 
 int upload_descriptors(vg_batch* B, int g) {
@@ -561,16 +586,28 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             const uint32_t align = g == kIsingSc ? 64u : 16u;  // 64-byte aligned rows
             d.pitch = (uint32_t)round_up(d.w, align);
             d.segs = (d.w + seg - 1) / seg;
-            const uint64_t items = (uint64_t)d.rows * d.segs;
-            if (items > 0xFFFFFFFFull - vg::kBlock) {
+            d.hy = (d.ly + 1) / 2;
+            d.nzc = (d.lz + vg::kIsingRows - 1) / vg::kIsingRows;
+            const uint64_t items = g == kIsingSc ? (uint64_t)d.nzc * 2 * d.hy * d.segs
+                                                 : (uint64_t)d.rows * d.segs;
+            if (items > 0x7FFFFFFFull - vg::kBlock || (uint64_t)d.rows > 0x7FFFFFFFull) {
                 free_batch(B);
                 return fail(VG_ERR_INVALID_ARGUMENT, "chain %u too large", G.members[k]);
             }
             d.items = (uint32_t)items;
+            magic_div(d.segs, d.segs_m, d.segs_s);
+            magic_div(d.ly, d.ly_m, d.ly_s);
+            magic_div(d.hy, d.hy_m, d.hy_s);
             d.nblocks = (uint32_t)((items + vg::kBlock - 1) / vg::kBlock);
             d.block0 = (uint32_t)block_chain.size();
             for (uint32_t bidx = 0; bidx < d.nblocks; ++bidx) block_chain.push_back((uint32_t)k);
             d.plane = (uint64_t)d.rows * d.pitch;
+            if (g == kIsingSc && d.plane >= (1ull << 32)) {
+                free_batch(B);
+                return fail(VG_ERR_UNSUPPORTED_LATTICE,
+                            "chain %u: an Ising colour plane of %llu bytes exceeds 4 GiB",
+                            G.members[k], (unsigned long long)d.plane);
+            }
             const uint64_t planes_per_colour = g == kIsingSc ? 1 : 3;
             const uint64_t elem_align = g == kIsingSc ? 256 : 64;  // 256-byte chain alignment
             offset = round_up(offset, elem_align);

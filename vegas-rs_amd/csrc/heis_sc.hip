@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
                     const float x1 = sym24(w1), x2 = sym24(w2);
                     const float qq = fa(fm(x1, x1), fm(x2, x2));
                     if (qq < 1.0f) {
-                        const float root = __fsqrt_rn(fs(1.0f, qq));
+                        const float root = sqrt_rn(fs(1.0f, qq));
                         px = fm(fm(2.0f, x1), root);
                         py = fm(fm(2.0f, x2), root);
                         pz = fs(1.0f, fm(2.0f, qq));

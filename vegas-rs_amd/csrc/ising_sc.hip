@@ -6,30 +6,38 @@
 // the Exchange + Zeeman local energy of src/energy.rs:147-151, 194-206:
 // instead of N attempts at uniformly random sites, one sweep updates every
 // colour-0 site, then every colour-1 site.  A site's flip is accepted when
-// dE <= 0, or else when its 32-bit draw is below floor-free threshold
-// ceil(exp(-dE/T) * 2^32) -- the reference's `u < exp(-dE/T)` rule
-// (integrator.rs:127-134) with u quantised to 2^-32.
+// dE <= 0, or else when its 32-bit draw k satisfies k < ceil(exp(-dE/T)*2^32)
+// -- the reference's `u < exp(-dE/T)` rule (integrator.rs:127-134) with u
+// quantised to 2^-32.
 //
-// Work item: 16 consecutive positions of one row of the colour plane (one
-// aligned 16-byte load per row touched).  Neighbour counts are summed four
-// sites per 32-bit register.  The acceptance draw of each site is split: its
-// high byte comes from one Philox call shared by the item's 16 sites; only
-// when that byte equals the high byte of the threshold (probability 2^-8)
-// is the low 24 bits' Philox call made.  The result equals comparing the full
-// 32-bit draw.
+// Work item: one 16-position segment of a row, marched through kIsingRows
+// consecutive z planes.  Along z the row below is the previous row's
+// neighbour above, so each plane costs four 16-byte loads (own, y-1, y+1,
+// z+1) instead of six.  Items of one wave share the parity of y, so the
+// direction of the x-neighbour byte shift (which alternates with y+z) is
+// uniform across the wave.
+//
+// Per site everything is done four sites per 32-bit register: neighbour
+// counts by byte-wise adds, the acceptance class lookup by v_perm_b32 on
+// 8-entry byte tables, and the comparison of the draw's high byte with the
+// threshold's high byte by borrow-free byte arithmetic.  The high bytes of a
+// segment's 16 draws come from one Philox call.  When a draw's high byte
+// equals the threshold's (probability 2^-8) the low 24 bits decide; those
+// ties are rare per row but common per wave, so a row with ties only leaves a
+// record (slot by ballot) in the wave's LDS queue; once the wave has finished
+// its planes, one lane per record resolves its ties, writes the flipped bytes
+// and corrects the wave's sums.  The result equals comparing the full 32-bit
+// draw.  No workgroup barrier is needed after the tables are loaded.
 #include "vg_device.hpp"
 
 namespace vg {
 
-__device__ __forceinline__ uint32_t sel4(uint32_t k, uint32_t a, uint32_t b, uint32_t c,
-                                         uint32_t d) {
-    return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
-}
-
-__device__ __forceinline__ uint32_t set_byte(uint32_t word, uint32_t byte, uint32_t value) {
-    const uint32_t sh = 8u * byte;
-    return (word & ~(0xFFu << sh)) | (value << sh);
-}
+constexpr uint32_t kWaves = kBlock / 64;
+constexpr uint32_t kTieRows = 64;     // tie records per wave (one per row with ties)
+constexpr uint32_t kTieWords = 10;
+constexpr uint32_t kH = 0x80808080u;  // byte MSBs
+constexpr uint32_t kL = 0x7F7F7F7Fu;
+constexpr uint32_t kOne = 0x01010101u;
 
 __device__ __forceinline__ int32_t wave_sum(int32_t v) {
 #pragma unroll
@@ -37,12 +45,41 @@ __device__ __forceinline__ int32_t wave_sum(int32_t v) {
     return v;
 }
 
-// Loads one neighbour row segment of the other colour plane (or marks it
-// absent on an open boundary).
-__device__ __forceinline__ uint4 load_seg(const uint8_t* plane, int64_t row, uint32_t pitch,
-                                          uint32_t i0, bool present) {
+__device__ __forceinline__ uint4 ld16(const uint8_t* base, uint64_t off, bool present) {
     if (!present) return make_uint4(0, 0, 0, 0);
-    return *reinterpret_cast<const uint4*>(plane + row * (int64_t)pitch + i0);
+    return *reinterpret_cast<const uint4*>(base + off);
+}
+
+__device__ __forceinline__ uint32_t pick4(uint32_t k, uint32_t a, uint32_t b, uint32_t c,
+                                          uint32_t d) {
+    return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
+}
+
+// Byte-wise unsigned r < t, as MSB-per-byte flags.
+__device__ __forceinline__ uint32_t bytes_lt(uint32_t r, uint32_t t) {
+    const uint32_t d = (r | kH) - (t & kL);
+    return ((~r & t) | (~(r ^ t) & ~d)) & kH;
+}
+
+// Byte-wise r == t, as MSB-per-byte flags.
+__device__ __forceinline__ uint32_t bytes_eq(uint32_t r, uint32_t t) {
+    const uint32_t x = r ^ t;
+    return ~(((x & kL) + kL) | x) & kH;
+}
+
+// Four bits -> one byte each (0 or 1).
+__device__ __forceinline__ uint32_t nibble_bytes(uint32_t v) {
+    return ((v & 0xFu) * 0x00204081u) & kOne;
+}
+
+// Low 24 bits of a tied draw decide: accept iff they are below the low 24
+// bits of the threshold; an always-accepted class needs no draw.
+__device__ __forceinline__ bool tie_accept(const uint32_t* lut, uint32_t always_b, uint32_t bj,
+                                           uint32_t S, uint32_t pos, uint32_t row, uint64_t t,
+                                           uint32_t colour, uint32_t k0, uint32_t k1) {
+    if ((always_b >> S) & 1u) return true;
+    const U4 rb = philox(U4{pos >> 2, row, (uint32_t)t, ctr_w(t, kIsingB, colour)}, k0, k1);
+    return (u4_word(rb, pos & 3u) >> 8) < (lut[bj * 17u + S] & 0xFFFFFFu);
 }
 
 __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict__ chains,
@@ -52,202 +89,293 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                      uint32_t colour, uint32_t s,
                                                      uint32_t total_blocks, uint32_t grid) {
     __shared__ uint32_t lut[34];
-    __shared__ int32_t red[2][kBlock / 64];
+    __shared__ uint32_t tab[20];  // t8, [g][b][2]
+    // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4]}
+    __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
 
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
-    if (threadIdx.x < 34) lut[threadIdx.x] = (&C.thr[0][0])[threadIdx.x];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid < 34) lut[tid] = (&C.thr[0][0])[tid];
+    if (tid >= 64 && tid < 84) tab[tid - 64] = (&C.t8[0][0][0])[tid - 64];
     __syncthreads();
 
-    const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
-    int32_t m_part = 0, b_part = 0;
+    const uint64_t t = C.sweep0 + s;
+    const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
+    const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
+    const uint32_t always0 = C.always[0], always1 = C.always[1];
+    // A colour plane is below 4 GiB (checked on the host): 32-bit offsets.
+    uint8_t* const own_plane = arena + C.off[colour];
+    const uint8_t* const oth = arena + C.off[colour ^ 1u];
+    const uint32_t zstep = ly * pitch;  // bytes between planes z and z+1
+    const uint32_t local = (b - C.block0) * kBlock + tid;
+    uint32_t (*const q_wave)[kTieWords] = tie_q[wave];
+    uint32_t nrec = 0;  // records queued by this wave (wave-uniform)
 
+    int32_t m_part = 0, b_part = 0;
+    uint32_t seg = 0, y = 0, zc = 0;
     if (local < C.items) {
-        const uint32_t row = local / C.segs;
-        const uint32_t seg = local - row * C.segs;
-        const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
-        const uint32_t y = row % ly, z = row / ly;
-        const uint32_t q = (colour + y + z) & 1u;  // own sites sit at x = 2i + q
+        const uint32_t r = fast_div(local, C.segs_m, C.segs_s);
+        seg = local - r * C.segs;
+        const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
+        y = 2u * (r - r2 * C.hy) + (r2 & 1u);
+        zc = r2 >> 1;
+    }
+    if (local < C.items && y < ly) {
+        const uint32_t z0 = zc * kIsingRows;
+        const uint32_t nk = min(kIsingRows, lz - z0);
         const uint32_t i0 = seg * kIsingSeg;
         const uint32_t nvalid = min(kIsingSeg, w - i0);
+        const uint32_t vbits = nvalid >= 16 ? 0xFFFFu : ((1u << nvalid) - 1u);
+        uint32_t validH[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) validH[k] = nibble_bytes(vbits >> (4 * k)) << 7;
+        const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
+        const bool ym_ok = y > 0 || py;
+        const bool yp_ok = y + 1 < ly || py;
+        // offsets of rows y-1 and y+1 relative to row y (wrapping when periodic)
+        const int32_t dym = y > 0 ? -(int32_t)pitch : (int32_t)((ly - 1) * pitch);
+        const int32_t dyp = y + 1 < ly ? (int32_t)pitch : -(int32_t)((ly - 1) * pitch);
+        const uint32_t gy = (ym_ok ? 0u : 1u) + (yp_ok ? 0u : 1u);
+        // x neighbour outside the 16 positions: i0-1 (q = 0) or i0+16 (q = 1)
+        const bool xl_ok = i0 > 0 || px;
+        const uint32_t xl_pos = i0 > 0 ? i0 - 1 : w - 1;
+        const uint32_t pw = w - i0;  // positions left in the row, 1..16 in the last segment
+        const bool xr_in = i0 + kIsingSeg < w;
+        uint32_t off = (y + ly * z0) * pitch + i0;  // row (y, z0), this segment
+        // the row window along z: row (y, z-1) and row (y, z)
+        const bool zm0_ok = z0 > 0 || pz;
+        uint4 below = ld16(oth, z0 > 0 ? off - zstep : off + (lz - 1) * zstep, zm0_ok);
+        uint4 here = ld16(oth, off, true);
+        bool below_ok = zm0_ok;
 
-        uint8_t* own_plane = arena + C.off[colour];
-        const uint8_t* oth = arena + C.off[colour ^ 1u];
-        uint4* own_ptr = reinterpret_cast<uint4*>(own_plane + (uint64_t)row * pitch + i0);
-        const uint4 o4 = *own_ptr;
-        uint4 m4 = load_seg(oth, row, pitch, i0, true);
+#pragma unroll
+        for (uint32_t k = 0; k < kIsingRows; ++k) {
+            if (k >= nk) break;
+            const uint32_t z = z0 + k;
+            const uint32_t row = y + ly * z;
+            const bool zp_ok = z + 1 < lz || pz;
+            uint4* own_ptr = reinterpret_cast<uint4*>(own_plane + off);
+            const uint4 o4 = *own_ptr;
+            const uint4 ym4 = ld16(oth, off + dym, ym_ok);
+            const uint4 yp4 = ld16(oth, off + dyp, yp_ok);
+            const uint4 above = ld16(oth, z + 1 < lz ? off + zstep : off - (lz - 1) * zstep, zp_ok);
+            const uint32_t g = gy + (below_ok ? 0u : 1u) + (zp_ok ? 0u : 1u);
 
-        // Neighbour rows along y and z (same positions i in the other plane).
-        uint32_t gu = 0;  // absent neighbour slots common to all 16 sites
-        const bool ym_ok = (y > 0) || (pbc & 2u);
-        const bool yp_ok = (y + 1 < ly) || (pbc & 2u);
-        const bool zm_ok = (z > 0) || (pbc & 4u);
-        const bool zp_ok = (z + 1 < lz) || (pbc & 4u);
-        const int64_t r_ym = y > 0 ? (int64_t)row - 1 : (int64_t)row + (ly - 1);
-        const int64_t r_yp = y + 1 < ly ? (int64_t)row + 1 : (int64_t)row - (ly - 1);
-        const int64_t r_zm = z > 0 ? (int64_t)row - ly : (int64_t)row + (int64_t)(lz - 1) * ly;
-        const int64_t r_zp =
-            z + 1 < lz ? (int64_t)row + ly : (int64_t)row - (int64_t)(lz - 1) * ly;
-        const uint4 ym4 = load_seg(oth, r_ym, pitch, i0, ym_ok);
-        const uint4 yp4 = load_seg(oth, r_yp, pitch, i0, yp_ok);
-        const uint4 zm4 = load_seg(oth, r_zm, pitch, i0, zm_ok);
-        const uint4 zp4 = load_seg(oth, r_zp, pitch, i0, zp_ok);
-        gu = (ym_ok ? 0u : 1u) + (yp_ok ? 0u : 1u) + (zm_ok ? 0u : 1u) + (zp_ok ? 0u : 1u);
-
-        // Along x the two neighbours of position i are i-1,i (q = 0) or i,i+1
-        // (q = 1) in the other plane's same row.
-        const uint8_t* orow = oth + (uint64_t)row * pitch;
-        const bool px = pbc & 1u;
-        uint32_t m[4] = {m4.x, m4.y, m4.z, m4.w};
-        uint32_t xs[4];
-        uint32_t gx_pos = 0xFFFFFFFFu;  // position whose x neighbour is absent
-        if (q == 0) {
-            uint32_t extra = 0;
-            if (i0 > 0)
-                extra = orow[i0 - 1];
-            else if (px)
-                extra = orow[w - 1];
-            else
-                gx_pos = 0;
-            xs[0] = __builtin_amdgcn_alignbyte(m[0], extra << 24, 3);
-            xs[1] = __builtin_amdgcn_alignbyte(m[1], m[0], 3);
-            xs[2] = __builtin_amdgcn_alignbyte(m[2], m[1], 3);
-            xs[3] = __builtin_amdgcn_alignbyte(m[3], m[2], 3);
-        } else {
-            uint32_t extra = 0;
-            if (i0 + kIsingSeg < w) {
-                extra = orow[i0 + kIsingSeg];
+            // x neighbours of position i: i-1,i (q = 0) or i,i+1 (q = 1) in
+            // the other plane's same row.
+            const uint32_t q = (colour + y + z) & 1u;
+            const uint8_t* orow = oth + (off - i0);
+            uint32_t m[4] = {here.x, here.y, here.z, here.w};
+            uint32_t xs[4];
+            uint32_t gx = 0xFFFFFFFFu;  // position whose x neighbour is absent
+            if (q == 0) {
+                const uint32_t extra = xl_ok ? (uint32_t)orow[xl_pos] : 0u;
+                if (!xl_ok) gx = 0;
+                xs[0] = __builtin_amdgcn_alignbyte(m[0], extra << 24, 3);
+                xs[1] = __builtin_amdgcn_alignbyte(m[1], m[0], 3);
+                xs[2] = __builtin_amdgcn_alignbyte(m[2], m[1], 3);
+                xs[3] = __builtin_amdgcn_alignbyte(m[3], m[2], 3);
             } else {
-                // The last position w-1 needs position w, i.e. position 0
-                // across the periodic boundary, or nothing when x is open.
-                const uint32_t wrap = px ? (uint32_t)orow[0] : 0u;
-                const uint32_t pw = w - i0;  // 1..16
-                if (!px) gx_pos = pw - 1;
-                if (pw == kIsingSeg) {
-                    extra = wrap;
+                uint32_t extra = 0;
+                if (xr_in) {
+                    extra = orow[i0 + kIsingSeg];
                 } else {
+                    // Position w-1 needs position w: position 0 across the
+                    // periodic boundary, or nothing when x is open.
+                    const uint32_t wrap = px ? (uint32_t)orow[0] : 0u;
+                    if (!px) gx = pw - 1;
+                    if (pw == kIsingSeg) {
+                        extra = wrap;
+                    } else {
 #pragma unroll
-                    for (uint32_t k = 0; k < 4; ++k)
-                        if ((pw >> 2) == k) m[k] = set_byte(m[k], pw & 3u, wrap);
+                        for (uint32_t kk = 0; kk < 4; ++kk)
+                            if ((pw >> 2) == kk) {
+                                const uint32_t sh = 8u * (pw & 3u);
+                                m[kk] = (m[kk] & ~(0xFFu << sh)) | (wrap << sh);
+                            }
+                    }
+                }
+                xs[0] = __builtin_amdgcn_alignbyte(m[1], m[0], 1);
+                xs[1] = __builtin_amdgcn_alignbyte(m[2], m[1], 1);
+                xs[2] = __builtin_amdgcn_alignbyte(m[3], m[2], 1);
+                xs[3] = __builtin_amdgcn_alignbyte(extra, m[3], 1);
+            }
+            const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
+            const uint32_t dn[4] = {below.x, below.y, below.z, below.w};
+            const uint32_t up4[4] = {above.x, above.y, above.z, above.w};
+            const uint32_t ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
+            const uint32_t ypv[4] = {yp4.x, yp4.y, yp4.z, yp4.w};
+            const uint32_t t0lo = tab[g * 4 + 0], t0hi = tab[g * 4 + 1];
+            const uint32_t t1lo = tab[g * 4 + 2], t1hi = tab[g * 4 + 3];
+
+            const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, colour)}, k0, k1);
+            const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
+            uint32_t live[4] = {validH[0], validH[1], validH[2], validH[3]};
+            if (C.proposal) {
+                const U4 rp = philox(
+                    U4{i0 >> 7, row, (uint32_t)t, ctr_w(t, kIsingProposal, colour)}, k0, k1);
+                const uint32_t bits = (u4_word(rp, (i0 >> 5) & 3u) >> (i0 & 31u)) & 0xFFFFu;
+                // Spin::rand proposes Up when its draw is below 1/2
+                // (state.rs:76-84): proposal bit 0 -> Up.  A proposal equal
+                // to the current spin changes nothing.
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    live[kk] &= (~(nibble_bytes(bits >> (4 * kk)) ^ o[kk]) & kOne) << 7;
+            }
+            uint32_t cnt[4], accH[4], tieH[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                cnt[kk] = m[kk] + xs[kk] + ymv[kk] + ypv[kk] + dn[kk] + up4[kk];  // 0..6
+                const uint32_t bm = __builtin_amdgcn_perm(0u, 0xFF00u, o[kk]);   // 0xFF where Up
+                const uint32_t th0 = __builtin_amdgcn_perm(t0hi, t0lo, cnt[kk]);
+                const uint32_t th1 = __builtin_amdgcn_perm(t1hi, t1lo, cnt[kk]);
+                const uint32_t th = (th1 & bm) | (th0 & ~bm);
+                accH[kk] = bytes_lt(rw[kk], th) & live[kk];
+                tieH[kk] = bytes_eq(rw[kk], th) & live[kk];
+            }
+            // The one position with an absent x neighbour (open x boundary)
+            // has S = 2c + g + 1, which the tables above do not cover.
+            if (gx < nvalid) {
+                const uint32_t kk = gx >> 2, sh = 8u * (gx & 3u);
+                const uint32_t c = (pick4(kk, cnt[0], cnt[1], cnt[2], cnt[3]) >> sh) & 0xFFu;
+                const uint32_t bj = (pick4(kk, o[0], o[1], o[2], o[3]) >> sh) & 1u;
+                const uint32_t S = 2u * c + g + 1u;
+                const uint32_t alw = ((bj ? always1 : always0) >> S) & 1u;
+                const uint32_t th_hi = alw ? 0xFFu : lut[bj * 17u + S] >> 24;
+                const uint32_t r8 = (pick4(kk, rw[0], rw[1], rw[2], rw[3]) >> sh) & 0xFFu;
+                const uint32_t lv = (pick4(kk, live[0], live[1], live[2], live[3]) >> sh) & 0x80u;
+                const uint32_t a = (lv && r8 < th_hi) ? 0x80u : 0u;
+                const uint32_t ti = (lv && r8 == th_hi) ? 0x80u : 0u;
+#pragma unroll
+                for (uint32_t q2 = 0; q2 < 4; ++q2)
+                    if (q2 == kk) {
+                        accH[q2] = (accH[q2] & ~(0x80u << sh)) | (a << sh);
+                        tieH[q2] = (tieH[q2] & ~(0x80u << sh)) | (ti << sh);
+                    }
+            }
+            uint32_t n[4], any = 0;
+            int32_t upc = 0;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const uint32_t f = accH[kk] >> 7;
+                any |= f;
+                n[kk] = o[kk] ^ f;
+                upc += __popc(n[kk]);
+            }
+            if (any) *own_ptr = make_uint4(n[0], n[1], n[2], n[3]);
+            m_part += 2 * upc - (int32_t)nvalid;
+            if (colour == 1) {
+                // sum_j s_j (S_j - 6), s = 2n - 1, S = 2c + g (+1 at an open x end)
+                uint32_t nc = 0, cs = 0;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const uint32_t cv = cnt[kk] & ((validH[kk] >> 7) * 0xFFu);
+                    nc = __builtin_amdgcn_udot4(n[kk], cv, nc, false);
+                    cs = __builtin_amdgcn_udot4(cv, kOne, cs, false);
+                }
+                const int32_t g6 = (int32_t)g - 6;
+                b_part += 4 * (int32_t)nc + 2 * g6 * upc - 2 * (int32_t)cs - g6 * (int32_t)nvalid;
+                if (gx < nvalid) {
+                    const uint32_t kk = gx >> 2, sh = 8u * (gx & 3u);
+                    const int32_t nj = (int32_t)((pick4(kk, n[0], n[1], n[2], n[3]) >> sh) & 1u);
+                    b_part += 2 * nj - 1;
                 }
             }
-            xs[0] = __builtin_amdgcn_alignbyte(m[1], m[0], 1);
-            xs[1] = __builtin_amdgcn_alignbyte(m[2], m[1], 1);
-            xs[2] = __builtin_amdgcn_alignbyte(m[3], m[2], 1);
-            xs[3] = __builtin_amdgcn_alignbyte(extra, m[3], 1);
+            // Rows with ties leave one record each in the wave's queue (their
+            // tied bytes were written unchanged above); slots by ballot.
+            // tie flags -> 16-bit mask (byte MSBs gathered by a multiply)
+            const uint32_t tmask = ((((tieH[0] >> 7) & kOne) * 0x10204080u) >> 28) |
+                                   (((((tieH[1] >> 7) & kOne) * 0x10204080u) >> 28) << 4) |
+                                   (((((tieH[2] >> 7) & kOne) * 0x10204080u) >> 28) << 8) |
+                                   (((((tieH[3] >> 7) & kOne) * 0x10204080u) >> 28) << 12);
+            const uint64_t has = __ballot(tmask != 0);
+            if (has) {
+                if (tmask) {
+                    const uint32_t slot =
+                        nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
+                    if (slot < kTieRows) {
+                        uint32_t* rec = q_wave[slot];
+                        rec[0] = lane | (k << 6) | (g << 9) | (min(gx, 31u) << 12);
+                        rec[1] = tmask;
+                        *reinterpret_cast<uint4*>(rec + 2) = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
+                        *reinterpret_cast<uint4*>(rec + 6) = make_uint4(o[0], o[1], o[2], o[3]);
+                    } else {
+                        // Queue full (practically never): resolve in this lane.
+                        uint32_t tm = tmask;
+                        while (tm) {
+                            const uint32_t j = __builtin_ctz(tm);
+                            tm &= tm - 1u;
+                            const uint32_t kk = j >> 2, sh = 8u * (j & 3u);
+                            const uint32_t c = (pick4(kk, cnt[0], cnt[1], cnt[2], cnt[3]) >> sh) & 0xFFu;
+                            const uint32_t bj = (pick4(kk, o[0], o[1], o[2], o[3]) >> sh) & 1u;
+                            const uint32_t S = 2u * c + g + (j == gx ? 1u : 0u);
+                            if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t,
+                                           colour, k0, k1)) {
+                                own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                const int32_t ds = 2 - 4 * (int32_t)bj;
+                                m_part += ds;
+                                if (colour == 1) b_part += ds * ((int32_t)S - 6);
+                            }
+                        }
+                    }
+                }
+                nrec += __builtin_popcountll(has);
+            }
+            below = here;
+            here = above;
+            below_ok = true;
+            off += zstep;
         }
-        const uint32_t ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
-        const uint32_t ypv[4] = {yp4.x, yp4.y, yp4.z, yp4.w};
-        const uint32_t zmv[4] = {zm4.x, zm4.y, zm4.z, zm4.w};
-        const uint32_t zpv[4] = {zp4.x, zp4.y, zp4.z, zp4.w};
-        const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
-        // S = 2 * (up neighbours) + (absent neighbour slots), per byte <= 12.
-        uint32_t S[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            // m[k] differs from the loaded row only at a padding position.
-            const uint32_t cnt = m[k] + xs[k] + ymv[k] + ypv[k] + zmv[k] + zpv[k];
-            S[k] = (cnt << 1) + gu * 0x01010101u;
-            if ((gx_pos >> 2) == (uint32_t)k) S[k] += 1u << (8u * (gx_pos & 3u));
-        }
+    }
 
-        const uint64_t t = C.sweep0 + s;
-        const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
-        const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, colour)}, k0, k1);
-        const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
-        uint32_t cand = 0xFFFFu;  // flip proposal: every site is a candidate
-        if (C.proposal) {
-            const U4 rp = philox(U4{i0 >> 7, row, (uint32_t)t, ctr_w(t, kIsingProposal, colour)},
-                                 k0, k1);
-            const uint32_t word = u4_word(rp, (i0 >> 5) & 3u);
-            const uint32_t bits = (word >> (i0 & 31u)) & 0xFFFFu;
-            // Spin::rand proposes Up when its draw is below 1/2 (state.rs:76-84):
-            // bit 0 -> Up.  A proposal equal to the current spin changes nothing.
-            uint32_t ob = 0;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) ob |= ((o[j >> 2] >> (8 * (j & 3))) & 1u) << j;
-            cand = ~(bits ^ ob) & 0xFFFFu;
-        }
-        const uint32_t always0 = C.always[0], always1 = C.always[1];
-        uint32_t flip = 0, tie = 0;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int k = j >> 2, sh = 8 * (j & 3);
-            const uint32_t sj = (S[k] >> sh) & 0xFFu;
-            const uint32_t bj = (o[k] >> sh) & 1u;
-            const uint32_t alw = ((bj ? always1 : always0) >> sj) & 1u;
-            const uint32_t th_hi = lut[bj * 17u + sj] >> 24;
-            const uint32_t r8 = (rw[k] >> sh) & 0xFFu;
-            flip |= (alw | (uint32_t)(r8 < th_hi)) << j;
-            tie |= ((alw ^ 1u) & (uint32_t)(r8 == th_hi)) << j;
-        }
-        const uint32_t vmask = (nvalid >= 16) ? 0xFFFFu : ((1u << nvalid) - 1u);
-        flip &= cand & vmask;
-        tie &= cand & vmask;
-        while (tie) {
-            const uint32_t j = __builtin_ctz(tie);
-            const uint32_t g = j >> 2;  // group of 4 positions sharing one call
-            const U4 rb = philox(U4{(i0 >> 2) + g, row, (uint32_t)t, ctr_w(t, kIsingB, colour)},
-                                 k0, k1);
-            const uint32_t sg = sel4(g, S[0], S[1], S[2], S[3]);
-            const uint32_t og = sel4(g, o[0], o[1], o[2], o[3]);
-#pragma unroll
-            for (uint32_t jj = 0; jj < 4; ++jj) {
-                const uint32_t pos = 4 * g + jj;
-                if ((tie >> pos) & 1u) {
-                    const uint32_t sj = (sg >> (8 * jj)) & 0xFFu;
-                    const uint32_t bj = (og >> (8 * jj)) & 1u;
-                    const uint32_t th = lut[bj * 17u + sj];
-                    const uint32_t lo24 = u4_word(rb, jj) >> 8;
-                    flip |= (uint32_t)(lo24 < (th & 0xFFFFFFu)) << pos;
+    // Resolve this wave's queued ties: one lane per record draws the low 24
+    // bits of each tied site, writes the flipped byte and corrects the sums.
+    // The wave's own row stores must land first.  nrec is wave-uniform (it
+    // only grows by ballot counts), so every lane agrees on the loop bound.
+    const uint32_t nq = min(nrec, kTieRows);
+    if (nq) {
+        __builtin_amdgcn_s_waitcnt(0);
+        for (uint32_t e = lane; e < nq; e += 64) {
+            const uint32_t* rec = q_wave[e];
+            const uint32_t meta = rec[0];
+            const uint32_t owner = (wave << 6) | (meta & 63u), k = (meta >> 6) & 7u;
+            const uint32_t g = (meta >> 9) & 7u, gx = (meta >> 12) & 31u;
+            const uint32_t l = (b - C.block0) * kBlock + owner;
+            const uint32_t r = fast_div(l, C.segs_m, C.segs_s);
+            const uint32_t sg = l - r * C.segs;
+            const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
+            const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
+            const uint32_t zz = (r2 >> 1) * kIsingRows + k;
+            const uint32_t row = yy + ly * zz;
+            uint32_t tm = rec[1];
+            while (tm) {
+                const uint32_t j = __builtin_ctz(tm);
+                tm &= tm - 1u;
+                const uint32_t sh = 8u * (j & 3u);
+                const uint32_t c = (rec[2 + (j >> 2)] >> sh) & 0xFFu;
+                const uint32_t bj = (rec[6 + (j >> 2)] >> sh) & 1u;
+                const uint32_t S = 2u * c + g + (j == gx ? 1u : 0u);
+                const uint32_t pos = sg * kIsingSeg + j;
+                if (tie_accept(lut, bj ? always1 : always0, bj, S, pos, row, t, colour, k0, k1)) {
+                    own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    const int32_t ds = 2 - 4 * (int32_t)bj;
+                    m_part += ds;
+                    if (colour == 1) b_part += ds * ((int32_t)S - 6);
                 }
             }
-            tie &= ~(0xFu << (4 * g));
         }
-
-        uint32_t n[4];
-        int32_t up = 0, bond = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            uint32_t fm = 0;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) fm |= ((flip >> (4 * k + jj)) & 1u) << (8 * jj);
-            n[k] = o[k] ^ fm;
-            up += __popc(n[k]);
-        }
-        if (flip) *own_ptr = make_uint4(n[0], n[1], n[2], n[3]);
-        m_part = 2 * up - (int32_t)nvalid;
-        if (colour == 1) {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int k = j >> 2, sh = 8 * (j & 3);
-                const int32_t sj = (int32_t)((S[k] >> sh) & 0xFFu) - 6;
-                const int32_t nb = (int32_t)((n[k] >> sh) & 1u);
-                bond += ((uint32_t)j < nvalid) ? (nb ? sj : -sj) : 0;
-            }
-        }
-        b_part = bond;
     }
 
     m_part = wave_sum(m_part);
     b_part = wave_sum(b_part);
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     if (lane == 0) {
-        red[0][wave] = m_part;
-        red[1][wave] = b_part;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int32_t mm = 0, bb = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kBlock / 64; ++k) {
-            mm += red[0][k];
-            bb += red[1][k];
-        }
-        int32_t* p = partials + ((uint64_t)(s * 2u + colour) * total_blocks + b) * 2u;
-        p[0] = mm;
-        p[1] = bb;
+        int32_t* p = partials + ((uint64_t)(s * 2u + colour) * total_blocks + b) * (2u * kWaves) +
+                     2u * wave;
+        p[0] = m_part;
+        p[1] = b_part;
     }
 }
 
@@ -263,10 +391,15 @@ __global__ __launch_bounds__(256) void ising_reduce(const ChainDev* __restrict__
     int64_t mm = 0, bb = 0;
     for (uint32_t k = threadIdx.x; k < C.nblocks; k += kBlock) {
         const uint32_t blk = C.block0 + k;
-        const int32_t* p0 = partials + ((uint64_t)(s * 2u + 0u) * total_blocks + blk) * 2u;
-        const int32_t* p1 = partials + ((uint64_t)(s * 2u + 1u) * total_blocks + blk) * 2u;
-        mm += (int64_t)p0[0] + (int64_t)p1[0];
-        bb += (int64_t)p1[1];
+        const int32_t* p0 =
+            partials + ((uint64_t)(s * 2u + 0u) * total_blocks + blk) * (2u * kWaves);
+        const int32_t* p1 =
+            partials + ((uint64_t)(s * 2u + 1u) * total_blocks + blk) * (2u * kWaves);
+#pragma unroll
+        for (uint32_t v = 0; v < kWaves; ++v) {
+            mm += (int64_t)p0[2 * v] + (int64_t)p1[2 * v];
+            bb += (int64_t)p1[2 * v + 1];
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {

@@ -109,7 +109,7 @@ __global__ void heis_init(const ChainDev* __restrict__ chain, float* __restrict_
             const float x2 = sym24(u4_word(r, 2 * t + 1));
             const float q = __fadd_rn(__fmul_rn(x1, x1), __fmul_rn(x2, x2));
             if (!done && q < 1.0f) {
-                const float root = __fsqrt_rn(__fsub_rn(1.0f, q));
+                const float root = sqrt_rn(__fsub_rn(1.0f, q));
                 sx = __fmul_rn(__fmul_rn(2.0f, x1), root);
                 sy = __fmul_rn(__fmul_rn(2.0f, x2), root);
                 sz = __fsub_rn(1.0f, __fmul_rn(2.0f, q));

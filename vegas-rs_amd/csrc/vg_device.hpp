@@ -100,10 +100,24 @@ struct ChainDev {
     uint32_t slot;         // chain index in its batch (record row)
     uint32_t z;            // coordination number: 6 (sc) or 8 (bcc)
     uint32_t pad0;
+    // n / d == (n * m) >> s for n < 2^31 (segs, ly and hy divisors)
+    uint32_t segs_m, segs_s, ly_m, ly_s;
+    // Ising work order: rows of one y parity, kIsingRows z planes per lane
+    uint32_t hy;           // (ly + 1) / 2, rows of one parity class per plane
+    uint32_t hy_m, hy_s;
+    uint32_t nzc;          // z chunks of kIsingRows planes
     // Ising acceptance: class (b, S) with b the current bit and S the sum over
     // neighbour slots of 2*b_j (present) or 1 (absent), S in [0, 2z].
     uint32_t thr[2][17];   // accept iff draw < thr (32-bit draw)
     uint32_t always[2];    // bit S set: class always accepted
+    // The same classes as byte tables for the packed sc decision, indexed by
+    // g (absent y/z neighbour slots, 0..4), b, and the up-neighbour count c
+    // (S = 2c + g): t8 = high byte of thr, or 0xFF when the class is always
+    // accepted (a draw byte of 0xFF then reads as a tie, which the tie
+    // resolver accepts without drawing); a8 = 0xFF when always accepted.
+    // Each 8-entry table is two words, entries 0-3 in the first.
+    uint32_t t8[5][2][2];
+    uint32_t a8[5][2][2];
     // Heisenberg parameters, fp32.
     float jn;              // -J
     float kk;              // K
@@ -115,9 +129,16 @@ struct ChainDev {
 };
 
 constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass
-constexpr uint32_t kIsingSeg = 16;     // positions per Ising work item
+constexpr uint32_t kIsingSeg = 16;     // positions per Ising row segment
+constexpr uint32_t kIsingRows = 4;     // z planes per Ising work item
 constexpr uint32_t kHeisSeg = 4;       // positions per Heisenberg work item
 constexpr uint32_t kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Exact unsigned division by a per-chain constant d for n < 2^31:
+// m = ceil(2^s / d) with s = 31 + ceil(log2 d) (computed on the host).
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s) {
+    return (uint32_t)(((uint64_t)n * m) >> s);
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs; map hardware block
 // b to a logical block so that each XCD walks one contiguous slab of the work
@@ -159,6 +180,24 @@ VG_HD float exp_nonpos(float x) {
     } s;
     s.u = (uint32_t)e << 23;
     return p * s.f;
+}
+
+// Correctly rounded fp32 square root for normal x > 0.  The hardware
+// v_sqrt_f32 result s is within 1 ulp; with s- and s+ its neighbours, the
+// exact residuals x - s-*s and x - s+*s (one fma each) decide whether the
+// true root lies below the midpoint of s- and s, or above that of s and s+
+// (a root of a float is never exactly a midpoint).  HIP's sqrtf and
+// __fsqrt_rn are 1-ulp on gfx950, so they are not used where the CPU oracle
+// must reproduce the bits.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    const uint32_t u = __float_as_uint(s);
+    const float sm = __uint_as_float(u - 1u), sp = __uint_as_float(u + 1u);
+    if (__builtin_fmaf(-sm, s, x) <= 0.0f)
+        s = sm;
+    else if (__builtin_fmaf(-sp, s, x) > 0.0f)
+        s = sp;
+    return s;
 }
 
 // 24-bit uniform in [0,1) and in [-1,1) from a 32-bit draw (exact in fp32).
