@@ -189,6 +189,14 @@ int vg_batch_run(vg_batch* batch, uint64_t sweeps, double* energy, double* magne
 int vg_batch_run_sums(vg_batch* batch, uint64_t sweeps, vg_sweep_sums* sums);
 /* Execution options: 1 = capture sweep blocks into a hipGraph and replay. */
 int vg_batch_set_graph(vg_batch* batch, int enabled);
+/* Tile of the Ising pass: z planes marched by one work item (1, 2, 4 or 8;
+ * default 4).  Results do not depend on it. */
+int vg_batch_set_tile(vg_batch* batch, uint32_t tile);
+int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile);
+/* Times `trial_sweeps` sweeps with every tile and keeps the fastest.  The
+ * chains' spins and sweep counters are restored after every trial, so tuning
+ * does not change any trajectory. */
+int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen_tile);
 /* Last run's device time in milliseconds (hipEvent pair around the sweeps). */
 int vg_batch_last_run_ms(const vg_batch* batch, double* ms);
 /* Per-launch timing of the colour-pass kernels: when enabled (plain launches

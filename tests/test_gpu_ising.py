@@ -179,3 +179,40 @@ def test_errors_mirror_reference_kinds(vg, ctx):
     bad = vg.chain(L, hamiltonian=vg.Hamiltonian(0, 0, 0.0, 0.0, (0.0, 0.0, 1.0), 0.0))
     with pytest.raises(vg.VegasError):
         vg.Batch(ctx, [bad])
+
+
+@pytest.mark.parametrize("dims,pbc", [((64, 64, 64), (1, 1, 1)), ((34, 6, 13), (1, 1, 0)),
+                                      ((10, 10, 1), (1, 1, 0))])
+def test_tiles_give_identical_trajectories(vg, orc, ctx, dims, pbc):
+    """The z tile (planes per work item) changes only the work layout: every
+    tile reproduces the oracle bit for bit."""
+    d, OL, OM = make_pair(vg, orc, dims, pbc, 3.0, 0.1, (0, 0, 1), "flip",
+                          ("exchange", "zeeman"), seed=5)
+    spins = orc.ising_init(OL, 5)
+    want = orc.ising_sweeps(OL, OM, 5, 0, 4, spins)
+    for tile in (1, 2, 4, 8):
+        with vg.Batch(ctx, [d]) as b:
+            b.set_tile(tile)
+            assert b.tile() == tile
+            b.randomize()
+            b.set_thermostat(0, 3.0, 0.1)
+            assert np.array_equal(b.run_sums(4)[0], want), tile
+            assert np.array_equal(b.get_ising(0), spins), tile
+
+
+def test_autotune_leaves_trajectory_unchanged(vg, orc, ctx):
+    descs = [make_pair(vg, orc, (32, 32, 32), seed=s)[0] for s in (1, 2)]
+    runs = []
+    for tune in (False, True):
+        with vg.Batch(ctx, descs) as b:
+            b.randomize()
+            for i in range(2):
+                b.set_thermostat(i, 4.0)
+            b.run(3)
+            if tune:
+                assert b.autotune(4) in (1, 2, 4, 8)
+            runs.append((b.run_sums(5), [b.get_ising(i) for i in range(2)],
+                         [b.sweep_count(i) for i in range(2)]))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert all(np.array_equal(a, c) for a, c in zip(runs[0][1], runs[1][1]))
+    assert runs[0][2] == runs[1][2] == [8, 8]

@@ -88,6 +88,9 @@ _SIGS = {
     "vg_batch_set_graph": [_vp, C.c_int],
     "vg_batch_last_run_ms": [_vp, _P(C.c_double)],
     "vg_batch_set_timing": [_vp, C.c_int],
+    "vg_batch_set_tile": [_vp, C.c_uint32],
+    "vg_batch_get_tile": [_vp, _P(C.c_uint32)],
+    "vg_batch_autotune": [_vp, C.c_uint32, _P(C.c_uint32)],
     "vg_batch_pass_stats": [_vp, C.c_int32, _P(C.c_double), _P(C.c_uint64)],
 }
 for _name, _args in _SIGS.items():
@@ -262,6 +265,20 @@ class Batch:
 
     def set_graph(self, enabled: bool) -> None:
         _check(_lib.vg_batch_set_graph(self.h, 1 if enabled else 0))
+
+    def set_tile(self, tile: int) -> None:
+        _check(_lib.vg_batch_set_tile(self.h, tile))
+
+    def tile(self) -> int:
+        v = C.c_uint32()
+        _check(_lib.vg_batch_get_tile(self.h, C.byref(v)))
+        return v.value
+
+    def autotune(self, trial_sweeps: int = 20) -> int:
+        """Picks the fastest tile by timing; spins and counters are restored."""
+        v = C.c_uint32()
+        _check(_lib.vg_batch_autotune(self.h, trial_sweeps, C.byref(v)))
+        return v.value
 
     def set_timing(self, enabled: bool) -> None:
         _check(_lib.vg_batch_set_timing(self.h, 1 if enabled else 0))

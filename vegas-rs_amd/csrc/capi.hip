@@ -17,6 +17,7 @@
 #include "vg_device.hpp"
 
 namespace vg {
+template <uint32_t K>
 __global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
                               uint32_t, uint32_t, uint32_t);
 __global__ void ising_reduce(const ChainDev*, const int32_t*, uint32_t, uint32_t, int64_t*);
@@ -72,6 +73,8 @@ struct Group {
     ChainDev* d_chains = nullptr;
     ChainDev* h_staging = nullptr;  // pinned copy used for the upload
     uint32_t* d_block_chain = nullptr;
+    uint32_t block_chain_cap = 0;
+    uint32_t tile = 4;  // Ising: z planes per work item (template parameter of the pass)
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
     void* arena = nullptr;
@@ -257,6 +260,48 @@ int upload_descriptors(vg_batch* B, int g) {
     return VG_OK;
 }
 
+constexpr uint32_t kDefaultTile = 4;
+
+// Work decomposition of a group for a tile (Ising: z planes per work item):
+// items and blocks per chain, the block -> chain map, the grid.  Replays of
+// captured graphs bake the grid in, so they are dropped.
+int layout_work(vg_batch* B, int g, uint32_t tile) {
+    Group& G = B->groups[g];
+    std::vector<uint32_t> block_chain;
+    for (size_t k = 0; k < G.members.size(); ++k) {
+        ChainDev& d = G.host[k];
+        uint64_t items;
+        if (g == kIsingSc) {
+            d.nzc = (d.lz + tile - 1) / tile;
+            items = (uint64_t)d.nzc * 2 * d.hy * d.segs;
+        } else {
+            items = (uint64_t)d.rows * d.segs;
+        }
+        if (items > 0x7FFFFFFFull - vg::kBlock)
+            return fail(VG_ERR_INVALID_ARGUMENT, "chain %u too large", G.members[k]);
+        d.items = (uint32_t)items;
+        d.nblocks = (uint32_t)((items + vg::kBlock - 1) / vg::kBlock);
+        d.block0 = (uint32_t)block_chain.size();
+        for (uint32_t bidx = 0; bidx < d.nblocks; ++bidx) block_chain.push_back((uint32_t)k);
+    }
+    if (block_chain.size() > G.block_chain_cap) {
+        if (G.d_block_chain) quiet(hipFree(G.d_block_chain));
+        G.d_block_chain = nullptr;
+        G.block_chain_cap = 0;
+        VG_HIP(hipMalloc(&G.d_block_chain, block_chain.size() * sizeof(uint32_t)));
+        G.block_chain_cap = (uint32_t)block_chain.size();
+    }
+    VG_HIP(hipMemcpy(G.d_block_chain, block_chain.data(), block_chain.size() * sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
+    G.tile = tile;
+    G.total_blocks = (uint32_t)block_chain.size();
+    G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
+    G.partial_bytes_per_sweep = 2ull * G.total_blocks * quantities(g) * quantity_bytes(g);
+    for (auto& kv : B->graphs) quiet(hipGraphExecDestroy(kv.second));
+    B->graphs.clear();
+    return VG_OK;
+}
+
 int free_batch(vg_batch* B) {
     if (!B) return VG_OK;
     for (auto& kv : B->graphs) quiet(hipGraphExecDestroy(kv.second));
@@ -274,6 +319,30 @@ int free_batch(vg_batch* B) {
     if (B->ev1) quiet(hipEventDestroy(B->ev1));
     delete B;
     return VG_OK;
+}
+
+void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
+    const dim3 grid(G.grid), block(vg::kBlock);
+    uint8_t* arena = (uint8_t*)G.arena;
+    int32_t* part = (int32_t*)G.partials;
+    switch (G.tile) {
+        case 1:
+            hipLaunchKernelGGL(vg::ising_sc_pass<1>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+        case 2:
+            hipLaunchKernelGGL(vg::ising_sc_pass<2>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+        case 8:
+            hipLaunchKernelGGL(vg::ising_sc_pass<8>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+        default:
+            hipLaunchKernelGGL(vg::ising_sc_pass<4>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+    }
 }
 
 // Enqueues the sweeps of one block (nsw sweeps, both colours of every group,
@@ -295,9 +364,7 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                     VG_HIP(hipEventRecord(B->pass_events[k], st));
                 }
                 if (g == kIsingSc)
-                    hipLaunchKernelGGL(vg::ising_sc_pass, dim3(G.grid), dim3(vg::kBlock), 0, st,
-                                       G.d_chains, G.d_block_chain, (uint8_t*)G.arena,
-                                       (int32_t*)G.partials, c, s, G.total_blocks, G.grid);
+                    launch_ising_pass(G, st, c, s);
                 else
                     hipLaunchKernelGGL(vg::heis_sc_pass, dim3(G.grid), dim3(vg::kBlock), 0, st,
                                        G.d_chains, G.d_block_chain, (float*)G.arena,
@@ -565,7 +632,6 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         if (G.members.empty()) continue;
         G.host.resize(G.members.size());
         uint64_t offset = 0;  // in elements (bytes for Ising, floats for Heisenberg)
-        std::vector<uint32_t> block_chain;
         for (size_t k = 0; k < G.members.size(); ++k) {
             ChainHost& ch = B->chains[G.members[k]];
             ChainDev& d = G.host[k];
@@ -587,20 +653,14 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             d.pitch = (uint32_t)round_up(d.w, align);
             d.segs = (d.w + seg - 1) / seg;
             d.hy = (d.ly + 1) / 2;
-            d.nzc = (d.lz + vg::kIsingRows - 1) / vg::kIsingRows;
-            const uint64_t items = g == kIsingSc ? (uint64_t)d.nzc * 2 * d.hy * d.segs
-                                                 : (uint64_t)d.rows * d.segs;
-            if (items > 0x7FFFFFFFull - vg::kBlock || (uint64_t)d.rows > 0x7FFFFFFFull) {
+            if ((uint64_t)d.rows * d.segs > 0x7FFFFFFFull - vg::kBlock ||
+                (uint64_t)d.rows > 0x7FFFFFFFull) {
                 free_batch(B);
                 return fail(VG_ERR_INVALID_ARGUMENT, "chain %u too large", G.members[k]);
             }
-            d.items = (uint32_t)items;
             magic_div(d.segs, d.segs_m, d.segs_s);
             magic_div(d.ly, d.ly_m, d.ly_s);
             magic_div(d.hy, d.hy_m, d.hy_s);
-            d.nblocks = (uint32_t)((items + vg::kBlock - 1) / vg::kBlock);
-            d.block0 = (uint32_t)block_chain.size();
-            for (uint32_t bidx = 0; bidx < d.nblocks; ++bidx) block_chain.push_back((uint32_t)k);
             d.plane = (uint64_t)d.rows * d.pitch;
             if (g == kIsingSc && d.plane >= (1ull << 32)) {
                 free_batch(B);
@@ -618,25 +678,24 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             offset += planes_per_colour * d.plane;
             refresh_params(B, G.members[k]);
         }
-        G.total_blocks = (uint32_t)block_chain.size();
-        G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
         const uint64_t elem = g == kIsingSc ? 1 : 4;
         G.arena_bytes = std::max<uint64_t>(offset * elem, 256);
         hipError_t e = hipMalloc(&G.arena, G.arena_bytes);
         if (e == hipSuccess) e = hipMemset(G.arena, 0, G.arena_bytes);
         if (e == hipSuccess) e = hipMalloc(&G.d_chains, G.host.size() * sizeof(ChainDev));
         if (e == hipSuccess) e = hipHostMalloc(&G.h_staging, G.host.size() * sizeof(ChainDev));
-        if (e == hipSuccess) e = hipMalloc(&G.d_block_chain, block_chain.size() * sizeof(uint32_t));
-        if (e == hipSuccess)
-            e = hipMemcpy(G.d_block_chain, block_chain.data(), block_chain.size() * sizeof(uint32_t),
-                          hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             free_batch(B);
             return fail(e == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE,
                         "allocating the chain arena (%llu bytes): %s",
                         (unsigned long long)G.arena_bytes, hipGetErrorString(e));
         }
-        G.partial_bytes_per_sweep = 2ull * G.total_blocks * quantities(g) * quantity_bytes(g);
+        // Work layout with the finest tile sizes the partial buffers.
+        int rc = layout_work(B, g, 1);
+        if (rc) {
+            free_batch(B);
+            return rc;
+        }
     }
     // Sweep-block length: keep the partial buffers within 64 MiB per group.
     uint64_t max_per_sweep = 1;
@@ -650,6 +709,13 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         if (e != hipSuccess) {
             free_batch(B);
             return fail(VG_ERR_OUT_OF_MEMORY, "allocating partial sums: %s", hipGetErrorString(e));
+        }
+        if (g == kIsingSc) {
+            int rc = layout_work(B, g, kDefaultTile);
+            if (rc) {
+                free_batch(B);
+                return rc;
+            }
         }
     }
     const uint64_t rec_bytes = (uint64_t)n_chains * B->max_block_sweeps * 5u * sizeof(int64_t);
@@ -897,6 +963,71 @@ int vg_batch_pass_stats(const vg_batch* batch, int32_t spin, double* total_ms, u
     *total_ms = batch->pass_ms[g];
     *launches = batch->pass_launches[g];
     return VG_OK;
+}
+
+int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
+    if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    if (tile != 1 && tile != 2 && tile != 4 && tile != 8)
+        return fail(VG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8 (got %u)", tile);
+    if (batch->groups[kIsingSc].members.empty()) return VG_OK;
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    return layout_work(batch, kIsingSc, tile);
+}
+
+int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile) {
+    if (!batch || !tile) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    *tile = batch->groups[kIsingSc].tile;
+    return VG_OK;
+}
+
+int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) {
+    if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    Group& G = batch->groups[kIsingSc];
+    if (G.members.empty() || trial_sweeps == 0) {
+        if (chosen) *chosen = G.tile;
+        return VG_OK;
+    }
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    hipStream_t st = batch->ctx->stream;
+    // The trials advance the chains; the arena and the sweep counters are
+    // restored after each, so tuning leaves every trajectory unchanged.
+    void* backup = nullptr;
+    VG_HIP(hipMalloc(&backup, G.arena_bytes));
+    std::vector<uint64_t> counters(batch->chains.size());
+    for (size_t i = 0; i < batch->chains.size(); ++i) counters[i] = batch->chains[i].sweeps;
+    auto restore = [&]() -> int {
+        VG_HIP(hipMemcpyAsync(G.arena, backup, G.arena_bytes, hipMemcpyDeviceToDevice, st));
+        VG_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < batch->chains.size(); ++i) batch->chains[i].sweeps = counters[i];
+        return VG_OK;
+    };
+    int rc = VG_OK;
+    hipError_t e = hipMemcpyAsync(backup, G.arena, G.arena_bytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = fail(VG_ERR_DEVICE, "autotune backup: %s", hipGetErrorString(e));
+    const uint32_t start_tile = G.tile;
+    uint32_t best = start_tile;
+    double best_ms = 1e300;
+    const uint32_t tiles[4] = {1, 2, 4, 8};
+    for (uint32_t ti = 0; ti < 4 && rc == VG_OK; ++ti) {
+        rc = layout_work(batch, kIsingSc, tiles[ti]);
+        if (rc == VG_OK) rc = run_impl(batch, 2, nullptr, nullptr, nullptr);
+        if (rc == VG_OK) rc = restore();
+        if (rc == VG_OK) rc = run_impl(batch, trial_sweeps, nullptr, nullptr, nullptr);
+        if (rc == VG_OK && batch->last_ms < best_ms) {
+            best_ms = batch->last_ms;
+            best = tiles[ti];
+        }
+        if (rc == VG_OK) rc = restore();
+    }
+    quiet(hipFree(backup));
+    if (rc != VG_OK) {
+        layout_work(batch, kIsingSc, start_tile);
+        return rc;
+    }
+    rc = layout_work(batch, kIsingSc, best);
+    if (chosen) *chosen = best;
+    return rc;
 }
 
 int vg_batch_last_run_ms(const vg_batch* batch, double* ms) {

@@ -10,8 +10,8 @@
 // -- the reference's `u < exp(-dE/T)` rule (integrator.rs:127-134) with u
 // quantised to 2^-32.
 //
-// Work item: one 16-position segment of a row, marched through kIsingRows
-// consecutive z planes.  Along z the row below is the previous row's
+// Work item: one 16-position segment of a row, marched through K (the tile,
+// a template parameter picked by the host) consecutive z planes.  Along z the row below is the previous row's
 // neighbour above, so each plane costs four 16-byte loads (own, y-1, y+1,
 // z+1) instead of six.  Items of one wave share the parity of y, so the
 // direction of the x-neighbour byte shift (which alternates with y+z) is
@@ -82,6 +82,7 @@ __device__ __forceinline__ bool tie_accept(const uint32_t* lut, uint32_t always_
     return (u4_word(rb, pos & 3u) >> 8) < (lut[bj * 17u + S] & 0xFFFFFFu);
 }
 
+template <uint32_t K>
 __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict__ chains,
                                                      const uint32_t* __restrict__ block_chain,
                                                      uint8_t* __restrict__ arena,
@@ -123,8 +124,8 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         zc = r2 >> 1;
     }
     if (local < C.items && y < ly) {
-        const uint32_t z0 = zc * kIsingRows;
-        const uint32_t nk = min(kIsingRows, lz - z0);
+        const uint32_t z0 = zc * K;
+        const uint32_t nk = min(K, lz - z0);
         const uint32_t i0 = seg * kIsingSeg;
         const uint32_t nvalid = min(kIsingSeg, w - i0);
         const uint32_t vbits = nvalid >= 16 ? 0xFFFFu : ((1u << nvalid) - 1u);
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         bool below_ok = zm0_ok;
 
 #pragma unroll
-        for (uint32_t k = 0; k < kIsingRows; ++k) {
+        for (uint32_t k = 0; k < K; ++k) {
             if (k >= nk) break;
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t sg = l - r * C.segs;
             const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
             const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
-            const uint32_t zz = (r2 >> 1) * kIsingRows + k;
+            const uint32_t zz = (r2 >> 1) * K + k;
             const uint32_t row = yy + ly * zz;
             uint32_t tm = rec[1];
             while (tm) {
@@ -378,6 +379,15 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         p[1] = b_part;
     }
 }
+
+template __global__ void ising_sc_pass<1>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                          uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<2>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                          uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<4>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                          uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<8>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                          uint32_t, uint32_t, uint32_t, uint32_t);
 
 // Sums the pass partials of each (chain, sweep) into its record:
 // record = {bond, 0, 0, mag, 0} (int64).  grid = (chains, sweeps).

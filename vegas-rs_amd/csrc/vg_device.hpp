@@ -102,10 +102,10 @@ struct ChainDev {
     uint32_t pad0;
     // n / d == (n * m) >> s for n < 2^31 (segs, ly and hy divisors)
     uint32_t segs_m, segs_s, ly_m, ly_s;
-    // Ising work order: rows of one y parity, kIsingRows z planes per lane
+    // Ising work order: rows of one y parity, a tile of z planes per lane
     uint32_t hy;           // (ly + 1) / 2, rows of one parity class per plane
     uint32_t hy_m, hy_s;
-    uint32_t nzc;          // z chunks of kIsingRows planes
+    uint32_t nzc;          // z chunks of one tile of planes
     // Ising acceptance: class (b, S) with b the current bit and S the sum over
     // neighbour slots of 2*b_j (present) or 1 (absent), S in [0, 2z].
     uint32_t thr[2][17];   // accept iff draw < thr (32-bit draw)
@@ -130,7 +130,6 @@ struct ChainDev {
 
 constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass
 constexpr uint32_t kIsingSeg = 16;     // positions per Ising row segment
-constexpr uint32_t kIsingRows = 4;     // z planes per Ising work item
 constexpr uint32_t kHeisSeg = 4;       // positions per Heisenberg work item
 constexpr uint32_t kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 
