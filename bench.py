@@ -87,11 +87,24 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--side", type=int, default=256, help="lattice side (256 for the headline)")
-    ap.add_argument("--graph", action="store_true", help="replay sweep blocks from a hipGraph")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="plain launches instead of replaying sweep blocks from a hipGraph")
     ap.add_argument("--cpu-sweeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-sweeps", type=int, default=100)
+    ap.add_argument("--tile", type=int, default=0, help="Ising tile; 0 = autotune")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        import torch
+        a = torch.ones(args.copy_bytes // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every chain "
               f"fits one MI355X and multi-GPU runs are not part of this build.", file=sys.stderr)
@@ -107,10 +120,12 @@ def main():
     attempts_per_step = n_sites * len(temps)
     ctx = vg.Context(0)
     batch = vg.Batch(ctx, descs)
-    batch.set_graph(args.graph)
     batch.randomize()
     for i, T in enumerate(temps):
         batch.set_thermostat(i, T)
+    tile = batch.autotune(10) if args.tile == 0 else args.tile
+    batch.set_tile(tile)
+    batch.set_graph(args.graph)
     log(f"bench: {len(temps)} chains sc {side}^3 int8, warmup {args.warmup}")
     if args.warmup:
         batch.run(args.warmup)
@@ -122,14 +137,19 @@ def main():
     value = attempts_per_step * args.steps / wall
     log(f"bench: {value:.4e} attempts/s (wall {wall:.2f} s, device {device_ms / 1e3:.2f} s)")
 
-    # Per-launch kernel time of the colour pass (HIP events on the engine's stream).
+    # Mean launch time of the colour pass from the HIP events the engine
+    # records on its stream around every timed sweep block: the blocks hold
+    # 2 pass launches per sweep plus one small reduction, so this slightly
+    # overstates the pass time.
+    launches = 2 * args.steps
+    avg_ms = device_ms / launches
+    # Cross-check: per-launch events around each pass of a short plain run.
     batch.set_graph(False)
     batch.set_timing(True)
     tsw = max(1, min(args.timing_sweeps, args.steps))
     batch.run(tsw, observe=False)
-    pass_ms, launches = batch.pass_stats(vg.ISING)
+    ev_ms, ev_launches = batch.pass_stats(vg.ISING)
     batch.set_timing(False)
-    avg_ms = pass_ms / launches
     attempts_per_launch = attempts_per_step // 2
     alg_bytes = 2 * attempts_per_launch  # read + write of each site once per sweep
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
@@ -165,6 +185,7 @@ def main():
             "chains": len(temps),
             "attempts_per_step": attempts_per_step,
             "graph": bool(args.graph),
+            "tile": tile,
         },
         "roofline": {
             "bound": "hbm",
@@ -177,6 +198,7 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
             "mean_launch_us": avg_ms * 1e3,
             "launches_timed": launches,
+            "mean_launch_us_per_launch_events": ev_ms / ev_launches * 1e3,
         },
         "cpu_baseline": cpu,
         "observables_last_sweep": {

@@ -662,7 +662,7 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             magic_div(d.ly, d.ly_m, d.ly_s);
             magic_div(d.hy, d.hy_m, d.hy_s);
             d.plane = (uint64_t)d.rows * d.pitch;
-            if (g == kIsingSc && d.plane >= (1ull << 32)) {
+            if (g == kIsingSc && d.plane > 0xFFFFF000ull) {
                 free_batch(B);
                 return fail(VG_ERR_UNSUPPORTED_LATTICE,
                             "chain %u: an Ising colour plane of %llu bytes exceeds 4 GiB",

@@ -45,9 +45,23 @@ __device__ __forceinline__ int32_t wave_sum(int32_t v) {
     return v;
 }
 
-__device__ __forceinline__ uint4 ld16(const uint8_t* base, uint64_t off, bool present) {
-    if (!present) return make_uint4(0, 0, 0, 0);
-    return *reinterpret_cast<const uint4*>(base + off);
+// Colour planes are read and written through raw buffer resources: 32-bit
+// offsets (a plane is below 4 GiB), and an offset past the plane (kNoRow)
+// reads as zeros, which is exactly an absent neighbour row on an open
+// boundary.  Dword 3 = 0x00020000 selects 32-bit data format on gfx9.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kNoRow = 0xFFFFFF00u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const uint8_t* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(p), 0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ u32x4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+}
+
+__device__ __forceinline__ uint32_t bload8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
 }
 
 __device__ __forceinline__ uint32_t pick4(uint32_t k, uint32_t a, uint32_t b, uint32_t c,
@@ -109,6 +123,8 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     // A colour plane is below 4 GiB (checked on the host): 32-bit offsets.
     uint8_t* const own_plane = arena + C.off[colour];
     const uint8_t* const oth = arena + C.off[colour ^ 1u];
+    const __amdgpu_buffer_rsrc_t rw_own = plane_rsrc(own_plane, (uint32_t)C.plane);
+    const __amdgpu_buffer_rsrc_t ro = plane_rsrc(oth, (uint32_t)C.plane);
     const uint32_t zstep = ly * pitch;  // bytes between planes z and z+1
     const uint32_t local = (b - C.block0) * kBlock + tid;
     uint32_t (*const q_wave)[kTieWords] = tie_q[wave];
@@ -147,8 +163,8 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         uint32_t off = (y + ly * z0) * pitch + i0;  // row (y, z0), this segment
         // the row window along z: row (y, z-1) and row (y, z)
         const bool zm0_ok = z0 > 0 || pz;
-        uint4 below = ld16(oth, z0 > 0 ? off - zstep : off + (lz - 1) * zstep, zm0_ok);
-        uint4 here = ld16(oth, off, true);
+        u32x4 below = bload16(ro, !zm0_ok ? kNoRow : (z0 > 0 ? off - zstep : off + (lz - 1) * zstep));
+        u32x4 here = bload16(ro, off);
         bool below_ok = zm0_ok;
 
 #pragma unroll
@@ -157,45 +173,43 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
             const bool zp_ok = z + 1 < lz || pz;
-            uint4* own_ptr = reinterpret_cast<uint4*>(own_plane + off);
-            const uint4 o4 = *own_ptr;
-            const uint4 ym4 = ld16(oth, off + dym, ym_ok);
-            const uint4 yp4 = ld16(oth, off + dyp, yp_ok);
-            const uint4 above = ld16(oth, z + 1 < lz ? off + zstep : off - (lz - 1) * zstep, zp_ok);
+            const u32x4 o4 = bload16(rw_own, off);
+            const u32x4 ym4 = bload16(ro, ym_ok ? off + dym : kNoRow);
+            const u32x4 yp4 = bload16(ro, yp_ok ? off + dyp : kNoRow);
+            const u32x4 above =
+                bload16(ro, !zp_ok ? kNoRow : (z + 1 < lz ? off + zstep : off - (lz - 1) * zstep));
             const uint32_t g = gy + (below_ok ? 0u : 1u) + (zp_ok ? 0u : 1u);
 
             // x neighbours of position i: i-1,i (q = 0) or i,i+1 (q = 1) in
-            // the other plane's same row.
+            // the other plane's same row.  The byte outside the segment is
+            // position i0-1 (q = 0), or i0+16, or position 0 across the
+            // periodic end (q = 1); kNoRow reads zero when x is open.
             const uint32_t q = (colour + y + z) & 1u;
-            const uint8_t* orow = oth + (off - i0);
+            const uint32_t orow = off - i0;
             uint32_t m[4] = {here.x, here.y, here.z, here.w};
             uint32_t xs[4];
             uint32_t gx = 0xFFFFFFFFu;  // position whose x neighbour is absent
             if (q == 0) {
-                const uint32_t extra = xl_ok ? (uint32_t)orow[xl_pos] : 0u;
+                const uint32_t extra = bload8(ro, xl_ok ? orow + xl_pos : kNoRow);
                 if (!xl_ok) gx = 0;
                 xs[0] = __builtin_amdgcn_alignbyte(m[0], extra << 24, 3);
                 xs[1] = __builtin_amdgcn_alignbyte(m[1], m[0], 3);
                 xs[2] = __builtin_amdgcn_alignbyte(m[2], m[1], 3);
                 xs[3] = __builtin_amdgcn_alignbyte(m[3], m[2], 3);
             } else {
-                uint32_t extra = 0;
-                if (xr_in) {
-                    extra = orow[i0 + kIsingSeg];
-                } else {
+                uint32_t extra = bload8(ro, xr_in ? off + kIsingSeg : (px ? orow : kNoRow));
+                if (!xr_in) {
                     // Position w-1 needs position w: position 0 across the
                     // periodic boundary, or nothing when x is open.
-                    const uint32_t wrap = px ? (uint32_t)orow[0] : 0u;
                     if (!px) gx = pw - 1;
-                    if (pw == kIsingSeg) {
-                        extra = wrap;
-                    } else {
+                    if (pw != kIsingSeg) {
 #pragma unroll
                         for (uint32_t kk = 0; kk < 4; ++kk)
                             if ((pw >> 2) == kk) {
                                 const uint32_t sh = 8u * (pw & 3u);
-                                m[kk] = (m[kk] & ~(0xFFu << sh)) | (wrap << sh);
+                                m[kk] = (m[kk] & ~(0xFFu << sh)) | (extra << sh);
                             }
+                        extra = 0;
                     }
                 }
                 xs[0] = __builtin_amdgcn_alignbyte(m[1], m[0], 1);
@@ -265,7 +279,14 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                 n[kk] = o[kk] ^ f;
                 upc += __popc(n[kk]);
             }
-            if (any) *own_ptr = make_uint4(n[0], n[1], n[2], n[3]);
+            if (any) {
+                u32x4 nv;
+                nv.x = n[0];
+                nv.y = n[1];
+                nv.z = n[2];
+                nv.w = n[3];
+                __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+            }
             m_part += 2 * upc - (int32_t)nvalid;
             if (colour == 1) {
                 // sum_j s_j (S_j - 6), s = 2n - 1, S = 2c + g (+1 at an open x end)
