@@ -1,0 +1,406 @@
+// Tests of the C++ host library (vegas-rs_amd/host).  `--cpu` runs the tests
+// that need no device; `--gpu` the ones that drive the engine.  The CPU
+// tests restate the reference's own unit tests (state.rs:330-382,
+// energy.rs:300-375) with two-sided expectations and its program schedules.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "oracle.h"
+#include "vegas/vegas.hpp"
+
+using namespace vegas;
+
+static int failures = 0;
+static int checks = 0;
+
+#define EXPECT(cond)                                                               \
+    do {                                                                           \
+        ++checks;                                                                  \
+        if (!(cond)) {                                                             \
+            ++failures;                                                            \
+            std::fprintf(stderr, "  FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+        }                                                                          \
+    } while (0)
+
+static bool close(double a, double b, double tol = 1e-12) { return std::fabs(a - b) < tol; }
+
+template <class F>
+static bool throws_kind(F f, ErrorKind k) {
+    try {
+        f();
+    } catch (const Error& e) {
+        return e.kind() == k;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- CPU ----
+static void ising_spin_multiplies_correctly() {
+    std::mt19937_64 rng(1);
+    const IsingSpin up = IsingSpin::up(), down = IsingSpin::down(), r = IsingSpin::rand(rng);
+    EXPECT(close(up.dot(up), 1.0));
+    EXPECT(close(up.dot(down), -1.0));
+    EXPECT(close(down.dot(up), -1.0));
+    EXPECT(close(down.dot(down), 1.0));
+    EXPECT(close(r.dot(r), 1.0));
+    EXPECT(close(r.dot(up) + r.dot(down), 0.0));
+}
+
+static void ising_magnetization_can_be_added_with_spin() {
+    const auto st = State<IsingSpin>::up_with_size(10);
+    const auto m = st.magnetization();
+    EXPECT(m.magnitude() == 10.0);
+    EXPECT(m.orientation() == IsingSpin::up());
+}
+
+static void heisenberg_spin_multiplies_correctly() {
+    std::mt19937_64 rng(2);
+    const HeisenbergSpin up = HeisenbergSpin::up(), down = HeisenbergSpin::down();
+    const HeisenbergSpin r = HeisenbergSpin::rand(rng);
+    EXPECT(close(up.dot(up), 1.0));
+    EXPECT(close(up.dot(down), -1.0));
+    EXPECT(close(r.dot(r), 1.0, 1e-15));
+}
+
+static void heisenberg_spins_are_random_and_unit() {
+    std::mt19937_64 rng(3);
+    EXPECT(HeisenbergSpin::rand(rng) != HeisenbergSpin::rand(rng));
+    for (int i = 0; i < 100; ++i) {
+        const HeisenbergSpin s = HeisenbergSpin::rand(rng);
+        EXPECT(close(s.dot(s), 1.0, 1e-15));
+    }
+}
+
+static void lengths_of_states() { EXPECT(State<HeisenbergSpin>::up_with_size(10).len() == 10); }
+
+static void field_and_thermostat() {
+    // Field::magnitude is |magnitude| (state.rs:219-221); T is clamped.
+    EXPECT(Field<IsingSpin>(IsingSpin::up(), -0.7).magnitude() == 0.7);
+    EXPECT(Thermostat<IsingSpin>(0.0, Field<IsingSpin>::zero()).temperature() == DBL_EPSILON);
+    EXPECT(Thermostat<IsingSpin>(-3.0, Field<IsingSpin>::zero()).temperature() == DBL_EPSILON);
+    const auto t = Thermostat<IsingSpin>(2.0, Field<IsingSpin>(IsingSpin::down(), 0.5));
+    EXPECT(t.with_temperature(1e-300).temperature() == DBL_EPSILON);
+    EXPECT(t.with_temperature(3.0).field().magnitude() == 0.5);
+    EXPECT(t.with_field(Field<IsingSpin>::zero()).temperature() == 2.0);
+    EXPECT(Thermostat<IsingSpin>::near_zero().temperature() == DBL_EPSILON);
+}
+
+static void energy_unit_tests() {
+    const auto ups = State<HeisenbergSpin>::up_with_size(10);
+    const auto downs = State<HeisenbergSpin>::down_with_size(10);
+    const auto nz = Thermostat<HeisenbergSpin>::near_zero();
+    EXPECT(close(Gauge(10.0).total_energy(nz, ups), 100.0));
+    const UniaxialAnisotropy<HeisenbergSpin> an(HeisenbergSpin::up(), 1.0);
+    EXPECT(close(an.total_energy(nz, ups), 10.0));   // K omitted in the total
+    EXPECT(close(an.total_energy(nz, downs), 10.0));
+    EXPECT(close(an.energy(nz, ups, 3), 1.0));
+    const Zeeman<HeisenbergSpin> z;
+    for (double mag : {1.0, 2.0}) {
+        const Thermostat<HeisenbergSpin> th(0.0, Field<HeisenbergSpin>(HeisenbergSpin::up(), mag));
+        EXPECT(close(z.total_energy(th, ups), -10.0 * mag));
+        EXPECT(close(z.total_energy(th, downs), 10.0 * mag));
+        EXPECT(close(z.energy(th, ups, 0), mag));  // per-site sign is + (energy.rs:147-151)
+    }
+    const auto comp = Compound<Gauge, UniaxialAnisotropy<HeisenbergSpin>>(Gauge(10.0), an);
+    EXPECT(close(comp.total_energy(nz, ups), 110.0));
+    const auto mac = hamiltonian(UniaxialAnisotropy<HeisenbergSpin>(HeisenbergSpin::up(), 1.0),
+                                 Gauge(1.0));
+    EXPECT(close(mac.total_energy(nz, ups), 20.0));
+    const vg_hamiltonian d = describe(hamiltonian(Exchange::from_lattice(1.5, Lattice::sc().expand_all(4)),
+                                                  Zeeman<IsingSpin>(), Gauge(0.25)));
+    EXPECT(d.terms == (VG_TERM_EXCHANGE | VG_TERM_ZEEMAN | VG_TERM_GAUGE));
+    EXPECT(d.compound == 1 && d.exchange == 1.5 && d.gauge == 0.25);
+}
+
+static void exchange_ground_states() {
+    // CLI Hamiltonian on the 10x10x1 sample (z open): Compound double counts.
+    const Lattice l = Lattice::sc().expand(10, 10, 1).drop_z();
+    const auto h = hamiltonian(Exchange::from_lattice(1.0, l), Zeeman<IsingSpin>());
+    const auto st = State<IsingSpin>::up_with_size(100);
+    const auto th = Thermostat<IsingSpin>(1.0, Field<IsingSpin>::zero());
+    EXPECT(close(h.total_energy(th, st) / 100.0, -4.0));
+    const auto ex = Exchange::from_lattice(1.0, Lattice::sc().expand_all(6));
+    EXPECT(close(ex.total_energy(th, State<IsingSpin>::down_with_size(216)) / 216.0, -3.0));
+    EXPECT(Lattice::bcc().expand_all(3).num_sites() == 54);
+    EXPECT(throws_kind([] { Lattice::fcc().expand_all(2).descriptor(); },
+                       ErrorKind::UnsupportedLattice));
+}
+
+static void accumulator_moments() {
+    Accumulator a;
+    for (double v : {1.0, 2.0, 3.0}) a.collect(v);
+    EXPECT(close(a.mean(), 2.0));
+    EXPECT(close(a.variance(), 2.0 / 3.0));
+    EXPECT(close(a.binder_cumulant(), 0.5));
+}
+
+static void program_schedules() {
+    // f64 accumulation, tested after stepping (program.rs:202-211, 302-332).
+    const auto c1 = CoolDown().set_max_temperature(4.0).set_min_temperature(0.1).schedule();
+    EXPECT(c1.size() == 39);
+    EXPECT(close(c1.back().temperature, 0.2, 1e-12));
+    EXPECT(CoolDown().set_max_temperature(3.0).set_min_temperature(0.1).schedule().size() == 29);
+    EXPECT(CoolDown().schedule().size() == 30);
+    EXPECT(CoolDown().set_max_temperature(5.0).set_cool_rate(0.05).schedule().size() == 100);
+    EXPECT(CoolDown().set_max_temperature(2.5).set_cool_rate(0.05).schedule().size() == 50);
+    const auto h = HysteresisLoop().schedule();
+    EXPECT(h.size() == 55);
+    double hi = -1e9, lo = 1e9;
+    for (const Stage& s : h) {
+        hi = std::max(hi, s.field);
+        lo = std::min(lo, s.field);
+    }
+    EXPECT(close(hi, 1.1, 1e-9) && close(lo, -1.1, 1e-9));
+    EXPECT(HysteresisLoop(500, 0, 1.0, 0.5, 0.05).schedule().size() == 55);
+    EXPECT(Relax().schedule().size() == 1 && Relax().schedule()[0].relax == 1000);
+}
+
+static void program_validation() {
+    EXPECT(throws_kind([] { Relax(0, 1.0).validate(); }, ErrorKind::NoSteps));
+    EXPECT(throws_kind([] { Relax(1, 0.0).validate(); }, ErrorKind::ZeroTemperature));
+    EXPECT(throws_kind([] { CoolDown(1.0, 2.0).validate(); }, ErrorKind::TemperatureMaxLessThanMin));
+    EXPECT(throws_kind([] { CoolDown(1.0, 0.5, 0.1, 1, 0).validate(); }, ErrorKind::NoSteps));
+    EXPECT(throws_kind([] { CoolDown(1.0, 0.0).validate(); }, ErrorKind::ZeroTemperature));
+    EXPECT(throws_kind([] { CoolDown(1.0, 0.5, 0.0).validate(); }, ErrorKind::ZeroCoolRate));
+    EXPECT(throws_kind([] { HysteresisLoop(0).validate(); }, ErrorKind::NoSteps));
+    EXPECT(throws_kind([] { HysteresisLoop(1, 1, 0.0).validate(); }, ErrorKind::ZeroTemperature));
+    EXPECT(throws_kind([] { HysteresisLoop(1, 1, 1.0, 0.0).validate(); }, ErrorKind::ZeroField));
+    EXPECT(throws_kind([] { HysteresisLoop(1, 1, 1.0, 1.0, 0.0).validate(); },
+                       ErrorKind::ZeroFieldStep));
+    // the order of the checks: max < min is reported before no steps
+    EXPECT(throws_kind([] { CoolDown(1.0, 2.0, 0.0, 0, 0).validate(); },
+                       ErrorKind::TemperatureMaxLessThanMin));
+}
+
+static void stat_line_format() {
+    EXPECT(format16(3.0) == "3.0000000000000000");
+    EXPECT(format16(-0.1) == "-0.1000000000000000");
+    EXPECT(format16(std::nan("")) == "NaN");
+    EXPECT(format16(1.0 / 3.0) == "0.3333333333333333");
+}
+
+// ---------------------------------------------------------------- GPU ----
+template <class H, class S>
+class Recorder : public Instrument<H, S> {
+  public:
+    std::vector<std::string> events;
+    std::vector<SweepRecord> records;
+    void on_relax_start(const Thermostat<S>& th, const H&, size_t n) override {
+        char b[96];
+        std::snprintf(b, sizeof b, "relax_start T=%g n=%zu", th.temperature(), n);
+        events.push_back(b);
+    }
+    void on_relax_end() override { events.push_back("relax_end"); }
+    void on_measure_start(const Thermostat<S>&, const H&, size_t) override {
+        events.push_back("measure_start");
+    }
+    void on_measure_end() override { events.push_back("measure_end"); }
+    void after_step(const SweepRecord& r) override { records.push_back(r); }
+};
+
+static void machine_hooks_and_parity() {
+    // The Machine's sweeps are the engine's chain seeded by the rng's first
+    // draw: energy, |M| and the final spins equal the oracle's for that seed.
+    using H = Compound<Exchange, Zeeman<IsingSpin>>;
+    const Lattice l = Lattice::sc().expand(10, 10, 1).drop_z();
+    H h = hamiltonian(Exchange::from_lattice(1.0, l), Zeeman<IsingSpin>());
+    std::mt19937_64 rng(42);
+    std::mt19937_64 rng_copy = rng;
+    const uint64_t seed = rng_copy();
+    Machine<H, MetropolisFlipIntegrator, IsingSpin>::Instruments ins;
+    auto* rec = new Recorder<H, IsingSpin>();
+    ins.emplace_back(rec);
+    Machine<H, MetropolisFlipIntegrator, IsingSpin> m(
+        Thermostat<IsingSpin>(2.8, Field<IsingSpin>::zero()), h, MetropolisFlipIntegrator(),
+        std::move(ins), State<IsingSpin>::up_with_size(100));
+    const CoolDown cool(2.0, 1.9, 0.1, 5, 10);  // f64 steps: 2.0 and 1.9
+    const auto stages = cool.schedule();
+    EXPECT(stages.size() == 2);
+    Relax(30, 2.2).run(rng, m);
+    cool.run(rng, m);
+    EXPECT(rec->events.size() == 2 + 4 * stages.size());
+    EXPECT(rec->events[0] == "relax_start T=2.2 n=100");
+    EXPECT(rec->events[1] == "relax_end" && rec->events[3] == "relax_end");
+    EXPECT(rec->events[4] == "measure_start" && rec->events[5] == "measure_end");
+    EXPECT(rec->records.size() == 30 + 15 * stages.size());
+    // the same trajectory from the oracle
+    or_lattice ol{0, 10, 10, 1, 3};
+    or_model om{};
+    om.terms = 3;
+    om.compound = 1;
+    om.exchange = 1.0;
+    om.field_dir[2] = 1.0;
+    std::vector<int8_t> s(100, 1);
+    std::vector<double> temps{2.2};
+    std::vector<size_t> lens{30};
+    for (const Stage& st : stages) {
+        temps.push_back(st.temperature);
+        lens.push_back(st.relax + st.measure);
+    }
+    size_t at = 0;
+    bool same = true;
+    for (size_t k = 0; k < temps.size(); ++k) {
+        om.temperature = temps[k];
+        std::vector<int64_t> sums(5 * lens[k]);
+        or_ising_sweeps(&ol, &om, seed, at, lens[k], s.data(), sums.data());
+        for (size_t j = 0; j < lens[k]; ++j) {
+            same &= rec->records[at + j].energy == -2.0 * (double)sums[5 * j];
+            same &= rec->records[at + j].magnetization == std::fabs((double)sums[5 * j + 3]);
+        }
+        at += lens[k];
+    }
+    EXPECT(same);
+    const auto now = m.state();
+    bool same_state = true;
+    for (size_t i = 0; i < 100; ++i) same_state &= (int8_t)now.at(i).value() == s[i];
+    EXPECT(same_state);
+}
+
+static void stat_sensor_lines() {
+    using H = Compound<Exchange, Zeeman<IsingSpin>>;
+    const Lattice l = Lattice::sc().expand(10, 10, 1).drop_z();
+    std::ostringstream out;
+    Machine<H, MetropolisFlipIntegrator, IsingSpin>::Instruments ins;
+    ins.emplace_back(new StatSensor<H, IsingSpin>(out));
+    auto* obs = new ObservableSensor<H, IsingSpin>();
+    ins.emplace_back(obs);
+    std::mt19937_64 rng(7);
+    Machine<H, MetropolisFlipIntegrator, IsingSpin> m(
+        Thermostat<IsingSpin>(2.8, Field<IsingSpin>::zero()),
+        hamiltonian(Exchange::from_lattice(1.0, l), Zeeman<IsingSpin>()),
+        MetropolisFlipIntegrator(), std::move(ins), State<IsingSpin>::rand_with_size(rng, 100));
+    CoolDown(4.0, 3.5, 0.1, 20, 200).run(rng, m);
+    std::string text = out.str();
+    int lines = 0;
+    for (char c : text) lines += c == '\n';
+    EXPECT(lines == 5);  // 4.0 ... 3.6: the f64 steps pass 3.5 at 3.4999999999999996
+    EXPECT(text.rfind("4.0000000000000000 0.0000000000000000 ", 0) == 0);
+    EXPECT(obs->rows().size() == 5 * (20 + 200));
+    EXPECT(obs->rows()[0].relax && obs->rows()[20].stage == 1 && !obs->rows()[20].relax);
+    // mean E of the first measure phase from the rows equals the line's third column
+    Accumulator a;
+    for (size_t k = 20; k < 220; ++k) a.collect(obs->rows()[k].energy);
+    const std::string first = text.substr(0, text.find('\n'));
+    std::istringstream f(first);
+    double T, h, e;
+    f >> T >> h >> e;
+    EXPECT(close(e, a.mean(), 1e-9));
+}
+
+static void state_sensor_snapshots() {
+    using H = Exchange;
+    const Lattice l = Lattice::sc().expand_all(4);
+    Machine<H, MetropolisFlipIntegrator, IsingSpin>::Instruments ins;
+    auto* ss = new StateSensor<H, IsingSpin>(4);
+    ins.emplace_back(ss);
+    std::mt19937_64 rng(9);
+    Machine<H, MetropolisFlipIntegrator, IsingSpin> m(
+        Thermostat<IsingSpin>(3.0, Field<IsingSpin>::zero()), Exchange::from_lattice(1.0, l),
+        MetropolisFlipIntegrator(), std::move(ins), State<IsingSpin>::up_with_size(64));
+    Relax(10, 3.0).run(rng, m);
+    EXPECT(ss->snapshots().size() == 3);  // steps 0, 4, 8
+    EXPECT(ss->snapshots()[1].step == 4 && ss->snapshots()[2].step == 8);
+    // snapshot at step 8 = state after 9 sweeps: equal to a fresh run of 9
+    std::mt19937_64 rng2(9);
+    Machine<H, MetropolisFlipIntegrator, IsingSpin> m2(
+        Thermostat<IsingSpin>(3.0, Field<IsingSpin>::zero()), Exchange::from_lattice(1.0, l),
+        MetropolisFlipIntegrator(), {}, State<IsingSpin>::up_with_size(64));
+    Relax(9, 3.0).run(rng2, m2);
+    const auto st9 = m2.state();
+    bool same = true;
+    for (size_t i = 0; i < 64; ++i) same &= st9.at(i) == ss->snapshots()[2].state.at(i);
+    EXPECT(same);
+}
+
+static void integrator_step_adapter() {
+    const Lattice l = Lattice::sc().expand_all(8);
+    const auto h = Exchange::from_lattice(1.0, l);
+    std::mt19937_64 rng(11), rng_copy(11);
+    const uint64_t seed = rng_copy();
+    auto st = State<IsingSpin>::up_with_size(512);
+    const Thermostat<IsingSpin> th(5.0, Field<IsingSpin>::zero());
+    const auto next = step(MetropolisFlipIntegrator(), rng, th, h, st);
+    or_lattice ol{0, 8, 8, 8, 7};
+    or_model om{};
+    om.terms = 1;
+    om.exchange = 1.0;
+    om.temperature = 5.0;
+    om.field_dir[2] = 1.0;
+    std::vector<int8_t> s(512, 1);
+    or_ising_sweeps(&ol, &om, seed, 0, 1, s.data(), nullptr);
+    bool same = true;
+    for (size_t i = 0; i < 512; ++i) same &= (int8_t)next.at(i).value() == s[i];
+    EXPECT(same);
+    EXPECT(throws_kind(
+        [&] {
+            std::mt19937_64 r(1);
+            step(WolffIntegrator(), r, th, h, st);
+        },
+        ErrorKind::NotImplemented));
+}
+
+static void heisenberg_machine_runs() {
+    using H = Compound<Compound<Exchange, UniaxialAnisotropy<HeisenbergSpin>>, Zeeman<HeisenbergSpin>>;
+    const Lattice l = Lattice::sc().expand_all(8);
+    H h = hamiltonian(Exchange::from_lattice(1.0, l),
+                      UniaxialAnisotropy<HeisenbergSpin>(HeisenbergSpin::up(), 0.1),
+                      Zeeman<HeisenbergSpin>());
+    std::mt19937_64 rng(5);
+    Machine<H, MetropolisIntegrator, HeisenbergSpin>::Instruments ins;
+    auto* rec = new Recorder<H, HeisenbergSpin>();
+    ins.emplace_back(rec);
+    Machine<H, MetropolisIntegrator, HeisenbergSpin> m(
+        Thermostat<HeisenbergSpin>(2.8, Field<HeisenbergSpin>::zero()), h, MetropolisIntegrator(),
+        std::move(ins), State<HeisenbergSpin>::rand_with_size(rng, 512));
+    HysteresisLoop(5, 2, 1.0, 0.2, 0.1).run(rng, m);
+    EXPECT(rec->records.size() == HysteresisLoop(5, 2, 1.0, 0.2, 0.1).schedule().size() * 7);
+    // E of the final state recomputed on the host with the reference formulas
+    const auto st = m.state();
+    const double host_e = h.total_energy(m.thermostat(), st);
+    EXPECT(std::fabs(host_e - rec->records.back().energy) <= 1e-6 * std::fabs(host_e) + 1e-9);
+    EXPECT(std::fabs(st.magnetization().magnitude() - rec->records.back().magnetization) < 1e-5);
+}
+
+int main(int argc, char** argv) {
+    const bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
+    std::vector<std::pair<const char*, std::function<void()>>> tests;
+    if (!gpu) {
+        tests = {{"ising_spin_multiplies_correctly", ising_spin_multiplies_correctly},
+                 {"ising_magnetization_can_be_added_with_spin", ising_magnetization_can_be_added_with_spin},
+                 {"heisenberg_spin_multiplies_correctly", heisenberg_spin_multiplies_correctly},
+                 {"heisenberg_spins_are_random_and_unit", heisenberg_spins_are_random_and_unit},
+                 {"lengths_of_states", lengths_of_states},
+                 {"field_and_thermostat", field_and_thermostat},
+                 {"energy_unit_tests", energy_unit_tests},
+                 {"exchange_ground_states", exchange_ground_states},
+                 {"accumulator_moments", accumulator_moments},
+                 {"program_schedules", program_schedules},
+                 {"program_validation", program_validation},
+                 {"stat_line_format", stat_line_format}};
+    } else {
+        tests = {{"machine_hooks_and_parity", machine_hooks_and_parity},
+                 {"stat_sensor_lines", stat_sensor_lines},
+                 {"state_sensor_snapshots", state_sensor_snapshots},
+                 {"integrator_step_adapter", integrator_step_adapter},
+                 {"heisenberg_machine_runs", heisenberg_machine_runs}};
+    }
+    for (auto& t : tests) {
+        const int before = failures;
+        try {
+            t.second();
+        } catch (const std::exception& e) {
+            ++failures;
+            std::fprintf(stderr, "  FAIL %s: exception %s\n", t.first, e.what());
+        }
+        std::printf("%s %s\n", failures == before ? "ok  " : "FAIL", t.first);
+        std::fflush(stdout);
+    }
+    std::printf("%d checks, %d failures\n", checks, failures);
+    return failures ? 1 : 0;
+}

@@ -1,0 +1,17 @@
+// Host library of the MI355X Metropolis engine: the reference's Rust trait
+// surface (state.rs, thermostat.rs, energy.rs, integrator.rs, machine.rs,
+// instrument.rs, accumulator.rs, program.rs) in C++ over the C ABI of
+// libvegas_hip.so.
+#pragma once
+
+#include "vegas_hip.h"
+
+#include "accumulator.hpp"
+#include "energy.hpp"
+#include "error.hpp"
+#include "instrument.hpp"
+#include "lattice.hpp"
+#include "machine.hpp"
+#include "program.hpp"
+#include "state.hpp"
+#include "thermostat.hpp"
