@@ -1,0 +1,131 @@
+"""bcc chains (both sublattices as the two colours) vs the CPU checkerboard
+oracle: Ising bit-exact after every sweep block, Heisenberg fp32 bit-exact,
+over periodic/open boundaries, odd and degenerate sizes, and mixed batches."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(vg, orc, dims, pbc, spin, proposal, terms, K=0.0, seed=3):
+    L = vg.lattice(vg.BCC, *dims, pbc=tuple(bool(p) for p in pbc))
+    kw = {"zeeman": "zeeman" in terms}
+    if "exchange" in terms:
+        kw["exchange"] = 1.0
+    if "anisotropy" in terms:
+        kw["anisotropy"] = K
+    d = vg.chain(L, vg.ISING if spin == "ising" else vg.HEISENBERG,
+                 vg.FLIP if proposal == "flip" else vg.RANDOM, vg.hamiltonian(**kw), seed)
+    OL = orc.lattice("bcc", *dims, pbc=pbc)
+    return d, OL
+
+
+ISING = [
+    # dims, pbc, T, h, hdir_z, proposal, terms
+    ((16, 16, 16), (1, 1, 1), 6.35, 0.0, 1, "flip", ("exchange",)),
+    ((5, 3, 7), (1, 1, 1), 5.0, 0.2, 1, "flip", ("exchange", "zeeman")),
+    ((17, 4, 3), (0, 1, 0), 4.0, 0.0, 1, "flip", ("exchange",)),
+    ((40, 2, 2), (1, 0, 1), 3.0, 0.1, -1, "flip", ("exchange", "zeeman")),
+    ((1, 5, 5), (1, 1, 1), 6.0, 0.0, 1, "flip", ("exchange",)),
+    ((33, 1, 1), (1, 1, 1), 2.5, 0.0, 1, "flip", ("exchange",)),
+    ((20, 6, 4), (0, 0, 0), 5.0, 0.3, 1, "random", ("exchange", "zeeman")),
+    ((32, 8, 8), (1, 1, 1), 1.0, 0.0, 1, "flip", ("exchange",)),
+    ((48, 5, 3), (1, 1, 1), 30.0, 0.5, 1, "random", ("exchange", "zeeman")),
+]
+
+
+@pytest.mark.parametrize("dims,pbc,T,h,hz,proposal,terms", ISING)
+def test_bcc_ising_bit_exact(vg, orc, ctx, dims, pbc, T, h, hz, proposal, terms):
+    d, OL = _pair(vg, orc, dims, pbc, "ising", proposal, terms, seed=21)
+    OM = orc.model(spin="ising", proposal=proposal, terms=terms, temperature=T, field=h,
+                   field_dir=(0.0, 0.0, float(hz)))
+    spins = orc.ising_init(OL, 21)
+    with vg.Batch(ctx, [d]) as b:
+        b.randomize()
+        assert np.array_equal(b.get_ising(0), spins)
+        b.set_thermostat(0, T, h, (0.0, 0.0, float(hz)))
+        for k in range(3):
+            got = b.run_sums(1)[0]
+            want = orc.ising_sweeps(OL, OM, 21, k, 1, spins)
+            assert np.array_equal(got, want), k
+            assert np.array_equal(b.get_ising(0), spins), k
+        got = b.run_sums(9)[0]
+        want = orc.ising_sweeps(OL, OM, 21, 3, 9, spins)
+        assert np.array_equal(got, want)
+        assert np.array_equal(b.get_ising(0), spins)
+
+
+HEIS = [
+    # dims, pbc, T, h, proposal, terms, K
+    ((8, 8, 8), (1, 1, 1), 2.0, 0.05, "random", ("exchange", "zeeman", "anisotropy"), 0.1),
+    ((5, 3, 7), (1, 1, 1), 1.0, 0.0, "flip", ("exchange",), 0.0),
+    ((9, 4, 3), (0, 1, 1), 1.4, 0.2, "random", ("exchange", "zeeman"), 0.0),
+    ((1, 3, 3), (1, 1, 1), 0.7, 0.0, "random", ("exchange", "anisotropy"), 0.5),
+    ((6, 1, 2), (1, 1, 0), 3.0, 0.1, "random", ("exchange", "zeeman", "anisotropy"), 0.1),
+    ((13, 2, 2), (0, 0, 0), 0.5, 0.3, "random", ("exchange", "zeeman"), 0.0),
+]
+
+
+@pytest.mark.parametrize("dims,pbc,T,h,proposal,terms,K", HEIS)
+def test_bcc_heisenberg_bit_exact(vg, orc, ctx, dims, pbc, T, h, proposal, terms, K):
+    d, OL = _pair(vg, orc, dims, pbc, "heisenberg", proposal, terms, K, seed=8)
+    OM = orc.model(spin="heisenberg", proposal=proposal, terms=terms, anisotropy=K,
+                   temperature=T, field=h)
+    xyz = orc.heis_init(OL, 8)
+    with vg.Batch(ctx, [d]) as b:
+        b.randomize()
+        assert np.array_equal(b.get_heisenberg(0), xyz.astype(np.float64))
+        b.set_thermostat(0, T, h)
+        got = b.run_sums(5)[0]
+        want = orc.heis_sweeps(OL, OM, 8, 0, 5, xyz)
+        assert np.array_equal(got, want)
+        assert np.array_equal(b.get_heisenberg(0), xyz.astype(np.float64))
+
+
+def test_mixed_sc_bcc_ising_heisenberg_batch(vg, orc, ctx):
+    """sc and bcc, Ising and Heisenberg chains in one batch (four kernel
+    groups) evolve exactly as each alone."""
+    specs = [("sc", "ising", (16, 8, 8), 4.0, 1), ("bcc", "ising", (8, 8, 8), 6.0, 2),
+             ("sc", "heisenberg", (8, 8, 8), 1.4, 3), ("bcc", "heisenberg", (6, 6, 6), 2.0, 4),
+             ("bcc", "ising", (12, 4, 4), 5.0, 5)]
+    descs, refs = [], []
+    for kind, spin, dims, T, seed in specs:
+        L = vg.lattice(vg.SC if kind == "sc" else vg.BCC, *dims)
+        H = vg.hamiltonian(exchange=1.0, zeeman=True)
+        descs.append(vg.chain(L, vg.ISING if spin == "ising" else vg.HEISENBERG,
+                              vg.FLIP if spin == "ising" else vg.RANDOM, H, seed))
+        OL = orc.lattice(kind, *dims)
+        OM = orc.model(spin=spin, proposal="flip" if spin == "ising" else "random",
+                       terms=("exchange", "zeeman"), temperature=T, field=0.1)
+        refs.append((spin, OL, OM, seed, T))
+    with vg.Batch(ctx, descs) as b:
+        b.randomize()
+        for i, (*_, T) in enumerate(refs):
+            b.set_thermostat(i, T, 0.1)
+        got = b.run_sums(6)
+        for i, (spin, OL, OM, seed, _) in enumerate(refs):
+            if spin == "ising":
+                st = orc.ising_init(OL, seed)
+                want = orc.ising_sweeps(OL, OM, seed, 0, 6, st)
+                assert np.array_equal(b.get_ising(i), st), i
+            else:
+                st = orc.heis_init(OL, seed)
+                want = orc.heis_sweeps(OL, OM, seed, 0, 6, st)
+                assert np.array_equal(b.get_heisenberg(i), st.astype(np.float64)), i
+            assert np.array_equal(got[i], want), i
+
+
+def test_bcc_observables_follow_reference_total_energy(vg, orc, ctx):
+    """E and |M| per sweep equal the reference's total_energy and
+    magnetization of the downloaded bcc state (Compound double counts)."""
+    d, OL = _pair(vg, orc, (6, 4, 3), (1, 1, 0), "ising", "flip", ("exchange", "zeeman"), seed=5)
+    OM = orc.model(terms=("exchange", "zeeman"), temperature=4.0, field=0.4)
+    with vg.Batch(ctx, [d]) as b:
+        b.randomize()
+        b.set_thermostat(0, 4.0, 0.4)
+        e, m = b.run(4)
+        s = orc.Sequential(OL, OM, 1)
+        s.set_thermostat(4.0, 0.4)
+        s.set_state(b.get_ising(0))
+        assert e[0, -1] == pytest.approx(s.total_energy(), abs=1e-9)
+        assert m[0, -1] == s.magnetization()

@@ -34,7 +34,14 @@ CASES = {
                       ("exchange", "zeeman", "anisotropy"), 0.1, 64, 4000, 30000),
     "heis_sc8_T2.5": ("heisenberg", (8, 8, 8), (1, 1, 1), 2.5, 0.05, "random",
                       ("exchange", "zeeman", "anisotropy"), 0.1, 64, 4000, 30000),
+    "ising_bcc5_T8": ("ising", (5, 5, 5), (1, 1, 1), 8.0, 0.1, "flip",
+                      ("exchange", "zeeman"), 0.0, 64, 6000, 60000),
+    "ising_bcc6x4x3_open_T5": ("ising", (6, 4, 3), (1, 0, 0), 5.0, 0.0, "flip",
+                               ("exchange", "zeeman"), 0.0, 64, 6000, 60000),
+    "heis_bcc5_T3": ("heisenberg", (5, 5, 5), (1, 1, 1), 3.0, 0.05, "random",
+                     ("exchange", "zeeman", "anisotropy"), 0.1, 64, 4000, 30000),
 }
+BCC = {"ising_bcc5_T8", "ising_bcc6x4x3_open_T5", "heis_bcc5_T3"}
 
 
 def _observables(e, m, n, T):
@@ -45,18 +52,18 @@ def _observables(e, m, n, T):
 ORDERED_START = {"ising_sc8_T4_h0.05_random"}
 
 
-def _engine(vg, ctx, spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps, down):
-    L = vg.lattice(vg.SC, *dims, pbc=tuple(bool(p) for p in pbc))
+def _engine(vg, ctx, kind, spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps, down):
+    L = vg.lattice(vg.BCC if kind == "bcc" else vg.SC, *dims, pbc=tuple(bool(p) for p in pbc))
     kw = {"zeeman": "zeeman" in terms}
     if "exchange" in terms:
         kw["exchange"] = 1.0
     if "anisotropy" in terms:
         kw["anisotropy"] = K
     H = vg.hamiltonian(**kw)
-    kind = vg.ISING if spin == "ising" else vg.HEISENBERG
-    descs = [vg.chain(L, kind, vg.FLIP if proposal == "flip" else vg.RANDOM, H, s)
+    sk = vg.ISING if spin == "ising" else vg.HEISENBERG
+    descs = [vg.chain(L, sk, vg.FLIP if proposal == "flip" else vg.RANDOM, H, s)
              for s in range(chains)]
-    n = int(np.prod(dims))
+    n = int(np.prod(dims)) * (2 if kind == "bcc" else 1)
     with vg.Batch(ctx, descs) as b:
         b.randomize()
         if down:
@@ -70,8 +77,9 @@ def _engine(vg, ctx, spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps, 
     return [(float(o.mean()), float(o.std(ddof=1) / np.sqrt(chains))) for o in obs]
 
 
-def _sequential(orc, spin, dims, pbc, T, h, proposal, terms, K, sweeps, down, seed=12345):
-    OL = orc.lattice("sc", *dims, pbc=pbc)
+def _sequential(orc, kind, spin, dims, pbc, T, h, proposal, terms, K, sweeps, down,
+                seed=12345):
+    OL = orc.lattice(kind, *dims, pbc=pbc)
     OM = orc.model(spin=spin, proposal=proposal, terms=terms, anisotropy=K, temperature=T,
                    field=h)
     s = orc.Sequential(OL, OM, seed)
@@ -85,7 +93,7 @@ def _sequential(orc, spin, dims, pbc, T, h, proposal, terms, K, sweeps, down, se
     s.sweeps(sweeps // 5, observe=False)
     e, m = s.sweeps(sweeps)
     s.close()
-    n = int(np.prod(dims))
+    n = orc.num_sites(OL)
     return [binned(e / n), binned(m / n),
             binned_fn(lambda x: x.var() / (n * T * T), [e]),
             binned_fn(lambda x: x.var() / (n * T), [m])]
@@ -95,8 +103,10 @@ def _sequential(orc, spin, dims, pbc, T, h, proposal, terms, K, sweeps, down, se
 def test_checkerboard_agrees_with_sequential_metropolis(vg, orc, ctx, name):
     spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps, seq_sweeps = CASES[name]
     down = name in ORDERED_START
-    gpu = _engine(vg, ctx, spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps, down)
-    seq = _sequential(orc, spin, dims, pbc, T, h, proposal, terms, K, seq_sweeps, down)
+    kind = "bcc" if name in BCC else "sc"
+    gpu = _engine(vg, ctx, kind, spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps,
+                  down)
+    seq = _sequential(orc, kind, spin, dims, pbc, T, h, proposal, terms, K, seq_sweeps, down)
     labels = ["E/N", "|M|/N", "C", "chi"]
     rows = []
     for lab, (a, ea), (b, eb) in zip(labels, gpu, seq):

@@ -248,6 +248,27 @@ def test_checkerboard_ising_matches_exact_enumeration(orc):
         assert abs(mm - em) < 4 * sm, (T, h, mm, em, sm)
 
 
+@pytest.mark.parametrize("dims,pbc,T,h", [((2, 2, 2), (1, 1, 1), 5.0, 0.3),
+                                          ((3, 2, 1), (1, 0, 0), 2.0, 0.0)])
+def test_checkerboard_bcc_ising_matches_exact_enumeration(orc, dims, pbc, T, h):
+    """bcc (sublattices as colours): the checkerboard oracle's long-run
+    averages equal the exact Boltzmann ones, with open boundaries too."""
+    L = orc.lattice("bcc", *dims, pbc=pbc)
+    n = orc.num_sites(L)
+    neigh = [[j for j in orc.neighbours(L, i) if j >= 0] for i in range(n)]
+    M = orc.model(terms=("exchange", "zeeman"), temperature=T, field=h)
+    spins = orc.ising_init(L, 4)
+    orc.ising_sweeps(L, M, 13, 0, 2000, spins)
+    sums = orc.ising_sweeps(L, M, 13, 2000, 100000, spins)
+    bond, mag = sums[:, 0].astype(float), sums[:, 3].astype(float)
+    e = -2.0 * bond + abs(h) * mag
+    ee, em, _, _ = ising_exact(neigh, n, 1.0, h, 1.0, T)
+    me, se = binned(e)
+    mm, sm = binned(np.abs(mag))
+    assert abs(me - ee) < 4 * se, (me, ee, se)
+    assert abs(mm - em) < 4 * sm, (mm, em, sm)
+
+
 def test_sequential_matches_exact_enumeration(orc):
     L = orc.lattice("sc", 4, 4, 1, pbc=(1, 1, 0))
     n = 16

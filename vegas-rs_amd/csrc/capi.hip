@@ -26,6 +26,10 @@ __global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t
 __global__ void ising_gather(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
 __global__ void heis_sc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                              uint32_t, uint32_t, uint32_t);
+__global__ void ising_bcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
+                               uint32_t, uint32_t, uint32_t);
+__global__ void heis_bcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
+                              uint32_t, uint32_t, uint32_t);
 __global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t, int64_t*);
 __global__ void heis_clear(const ChainDev*, float*);
 __global__ void heis_init(const ChainDev*, float*);
@@ -65,7 +69,15 @@ uint64_t round_up(uint64_t v, uint64_t m) { return (v + m - 1) / m * m; }
 inline void quiet(hipError_t) {}
 
 // Kernel groups: chains of one group share an arena and one pass kernel.
-enum GroupKind { kIsingSc = 0, kHeisSc = 1, kNumGroups = 2 };
+enum GroupKind { kIsingSc = 0, kHeisSc = 1, kIsingBcc = 2, kHeisBcc = 3, kNumGroups = 4 };
+
+bool is_ising(int g) { return g == kIsingSc || g == kIsingBcc; }
+
+int group_of(const vg_chain_desc& d) {
+    const bool bcc = d.lattice.kind == VG_LATTICE_BCC;
+    if (d.spin == VG_SPIN_HEISENBERG) return bcc ? kHeisBcc : kHeisSc;
+    return bcc ? kIsingBcc : kIsingSc;
+}
 
 struct Group {
     std::vector<uint32_t> members;  // batch chain indices
@@ -116,8 +128,8 @@ struct vg_batch {
     bool timing = false;
     std::vector<hipEvent_t> pass_events;  // start/stop pairs
     uint32_t pass_events_used = 0;
-    double pass_ms[kNumGroups] = {0.0, 0.0};
-    uint64_t pass_launches[kNumGroups] = {0, 0};
+    double pass_ms[kNumGroups] = {};
+    uint64_t pass_launches[kNumGroups] = {};
 };
 
 namespace {
@@ -142,9 +154,7 @@ int validate_desc(const vg_chain_desc& d) {
                 return fail(VG_ERR_UNSUPPORTED_LATTICE,
                             "periodic sc axis %c has odd length %u: not bipartite", "xyz"[a],
                             dims[a]);
-    } else {
-        return fail(VG_ERR_NOT_IMPLEMENTED, "bcc lattices are not implemented yet");
-    }
+    }  // bcc: the two sublattices are the colours for any sizes and boundaries
     const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
     if ((uint64_t)L.ly * L.lz > 0xFFFFFFFFull || cells > (1ull << 34))
         return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
@@ -240,7 +250,7 @@ void heis_params(const ChainHost& ch, ChainDev& d) {
 void refresh_params(vg_batch* B, uint32_t i) {
     ChainHost& ch = B->chains[i];
     ChainDev& d = B->groups[ch.group].host[ch.index];
-    if (ch.group == kIsingSc)
+    if (is_ising(ch.group))
         ising_thresholds(ch, d);
     else
         heis_params(ch, d);
@@ -248,8 +258,8 @@ void refresh_params(vg_batch* B, uint32_t i) {
 
 // Ising passes write {mag, bond} per wave (4 waves per workgroup); Heisenberg
 // passes write {bond, mx, my, mz, axis2} per workgroup.
-uint32_t quantities(int group) { return group == kIsingSc ? 8u : 5u; }
-uint32_t quantity_bytes(int group) { return group == kIsingSc ? 4u : 8u; }
+uint32_t quantities(int group) { return is_ising(group) ? 8u : 5u; }
+uint32_t quantity_bytes(int group) { return is_ising(group) ? 4u : 8u; }
 
 int upload_descriptors(vg_batch* B, int g) {
     Group& G = B->groups[g];
@@ -262,7 +272,7 @@ int upload_descriptors(vg_batch* B, int g) {
 
 constexpr uint32_t kDefaultTile = 4;
 
-// Work decomposition of a group for a tile (Ising: z planes per work item):
+// Work decomposition of a group for a tile (Ising sc: z planes per work item):
 // items and blocks per chain, the block -> chain map, the grid.  Replays of
 // captured graphs bake the grid in, so they are dropped.
 int layout_work(vg_batch* B, int g, uint32_t tile) {
@@ -363,12 +373,21 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                     if (k + 2 > B->pass_events.size()) return fail(VG_ERR_DEVICE, "timing events exhausted");
                     VG_HIP(hipEventRecord(B->pass_events[k], st));
                 }
+                const dim3 grid(G.grid), block(vg::kBlock);
                 if (g == kIsingSc)
                     launch_ising_pass(G, st, c, s);
+                else if (g == kIsingBcc)
+                    hipLaunchKernelGGL(vg::ising_bcc_pass, grid, block, 0, st, G.d_chains,
+                                       G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials,
+                                       c, s, G.total_blocks, G.grid);
+                else if (g == kHeisSc)
+                    hipLaunchKernelGGL(vg::heis_sc_pass, grid, block, 0, st, G.d_chains,
+                                       G.d_block_chain, (float*)G.arena, (int64_t*)G.partials,
+                                       c, s, G.total_blocks, G.grid);
                 else
-                    hipLaunchKernelGGL(vg::heis_sc_pass, dim3(G.grid), dim3(vg::kBlock), 0, st,
-                                       G.d_chains, G.d_block_chain, (float*)G.arena,
-                                       (int64_t*)G.partials, c, s, G.total_blocks, G.grid);
+                    hipLaunchKernelGGL(vg::heis_bcc_pass, grid, block, 0, st, G.d_chains,
+                                       G.d_block_chain, (float*)G.arena, (int64_t*)G.partials,
+                                       c, s, G.total_blocks, G.grid);
                 if (timed) {
                     VG_HIP(hipEventRecord(B->pass_events[B->pass_events_used + 1], st));
                     B->pass_events_used += 2;
@@ -380,7 +399,7 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
         Group& G = B->groups[g];
         if (G.members.empty()) continue;
         dim3 grid((uint32_t)G.members.size(), nsw);
-        if (g == kIsingSc)
+        if (is_ising(g))
             hipLaunchKernelGGL(vg::ising_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
                                (const int32_t*)G.partials, G.total_blocks, nsw, B->d_records);
         else
@@ -621,7 +640,7 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         const vg_lattice& L = ch.desc.lattice;
         const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
         ch.n_sites = L.kind == VG_LATTICE_SC ? cells : 2 * cells;
-        ch.group = ch.desc.spin == VG_SPIN_HEISENBERG ? kHeisSc : kIsingSc;
+        ch.group = group_of(ch.desc);
         Group& G = B->groups[ch.group];
         ch.index = (uint32_t)G.members.size();
         G.members.push_back(i);
@@ -648,8 +667,8 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             d.w = L.kind == VG_LATTICE_SC ? L.lx / 2 : L.lx;
             d.proposal = (uint32_t)ch.desc.proposal;
             d.slot = G.members[k];
-            const uint32_t seg = g == kIsingSc ? vg::kIsingSeg : vg::kHeisSeg;
-            const uint32_t align = g == kIsingSc ? 64u : 16u;  // 64-byte aligned rows
+            const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
+            const uint32_t align = is_ising(g) ? 64u : 16u;  // 64-byte aligned rows
             d.pitch = (uint32_t)round_up(d.w, align);
             d.segs = (d.w + seg - 1) / seg;
             d.hy = (d.ly + 1) / 2;
@@ -662,14 +681,14 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             magic_div(d.ly, d.ly_m, d.ly_s);
             magic_div(d.hy, d.hy_m, d.hy_s);
             d.plane = (uint64_t)d.rows * d.pitch;
-            if (g == kIsingSc && d.plane > 0xFFFFF000ull) {
+            if (is_ising(g) && d.plane > 0xFFFFF000ull) {
                 free_batch(B);
                 return fail(VG_ERR_UNSUPPORTED_LATTICE,
                             "chain %u: an Ising colour plane of %llu bytes exceeds 4 GiB",
                             G.members[k], (unsigned long long)d.plane);
             }
-            const uint64_t planes_per_colour = g == kIsingSc ? 1 : 3;
-            const uint64_t elem_align = g == kIsingSc ? 256 : 64;  // 256-byte chain alignment
+            const uint64_t planes_per_colour = is_ising(g) ? 1 : 3;
+            const uint64_t elem_align = is_ising(g) ? 256 : 64;  // 256-byte chain alignment
             offset = round_up(offset, elem_align);
             d.off[0] = offset;
             offset += planes_per_colour * d.plane;
@@ -678,7 +697,7 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             offset += planes_per_colour * d.plane;
             refresh_params(B, G.members[k]);
         }
-        const uint64_t elem = g == kIsingSc ? 1 : 4;
+        const uint64_t elem = is_ising(g) ? 1 : 4;
         G.arena_bytes = std::max<uint64_t>(offset * elem, 256);
         hipError_t e = hipMalloc(&G.arena, G.arena_bytes);
         if (e == hipSuccess) e = hipMemset(G.arena, 0, G.arena_bytes);
@@ -795,7 +814,7 @@ int vg_batch_randomize(vg_batch* batch) {
         Group& G = batch->groups[ch.group];
         const ChainDev& d = G.host[ch.index];
         const ChainDev* dp = G.d_chains + ch.index;
-        if (ch.group == kIsingSc) {
+        if (is_ising(ch.group)) {
             const uint64_t n = 2ull * d.plane;
             hipLaunchKernelGGL(vg::ising_init, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
                                dp, (uint8_t*)G.arena);
@@ -816,7 +835,7 @@ int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
     if (rc) return rc;
     if (!spins) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
     ChainHost& ch = batch->chains[chain];
-    if (ch.group != kIsingSc) return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Ising", chain);
+    if (!is_ising(ch.group)) return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Ising", chain);
     for (uint64_t n = 0; n < ch.n_sites; ++n)
         if (spins[n] != 1 && spins[n] != -1)
             return fail(VG_ERR_INVALID_ARGUMENT, "spin %llu is %d, not +1 or -1",
@@ -846,7 +865,7 @@ int vg_batch_get_ising(vg_batch* batch, uint32_t chain, int8_t* spins) {
     if (rc) return rc;
     if (!spins) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
     ChainHost& ch = batch->chains[chain];
-    if (ch.group != kIsingSc) return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Ising", chain);
+    if (!is_ising(ch.group)) return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Ising", chain);
     Group& G = batch->groups[ch.group];
     hipStream_t st = batch->ctx->stream;
     VG_HIP(hipSetDevice(batch->ctx->device));
@@ -869,7 +888,7 @@ int vg_batch_set_heisenberg(vg_batch* batch, uint32_t chain, const double* xyz) 
     if (rc) return rc;
     if (!xyz) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
     ChainHost& ch = batch->chains[chain];
-    if (ch.group != kHeisSc)
+    if (is_ising(ch.group))
         return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Heisenberg", chain);
     Group& G = batch->groups[ch.group];
     hipStream_t st = batch->ctx->stream;
@@ -896,7 +915,7 @@ int vg_batch_get_heisenberg(vg_batch* batch, uint32_t chain, double* xyz) {
     if (rc) return rc;
     if (!xyz) return fail(VG_ERR_INVALID_ARGUMENT, "null spins");
     ChainHost& ch = batch->chains[chain];
-    if (ch.group != kHeisSc)
+    if (is_ising(ch.group))
         return fail(VG_ERR_INVALID_ARGUMENT, "chain %u is not Heisenberg", chain);
     Group& G = batch->groups[ch.group];
     hipStream_t st = batch->ctx->stream;
@@ -959,9 +978,14 @@ int vg_batch_set_timing(vg_batch* batch, int enabled) {
 
 int vg_batch_pass_stats(const vg_batch* batch, int32_t spin, double* total_ms, uint64_t* launches) {
     if (!batch || !total_ms || !launches) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
-    const int g = spin == VG_SPIN_HEISENBERG ? kHeisSc : kIsingSc;
-    *total_ms = batch->pass_ms[g];
-    *launches = batch->pass_launches[g];
+    // every group whose chains have this spin kind
+    *total_ms = 0.0;
+    *launches = 0;
+    for (int g = 0; g < kNumGroups; ++g)
+        if (is_ising(g) == (spin != VG_SPIN_HEISENBERG)) {
+            *total_ms += batch->pass_ms[g];
+            *launches += batch->pass_launches[g];
+        }
     return VG_OK;
 }
 

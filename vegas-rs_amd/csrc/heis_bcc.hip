@@ -1,0 +1,130 @@
+// fp32 Heisenberg checkerboard colour pass on the body-centred cubic
+// lattice, for any number of chains packed in one arena.
+//
+// The per-site update is the one of the sc pass (heis_common.hpp); only the
+// neighbour sum differs.  A corner site of cell (x,y,z) (colour 0) bonds to
+// the body centres of cells (x-dx, y-dy, z-dz), a body centre (colour 1) to
+// the corners of cells (x+dx, y+dy, z+dz), dx,dy,dz in {0,1}; the field h is
+// summed over the eight slots in the order dz, dy, dx (dx fastest) with an
+// absent slot (open boundary) adding +0.0f.
+//
+// Work item: 4 consecutive positions of one row; each of the four neighbour
+// rows is read with one aligned 16-byte load per component, plus the one
+// position beyond the item (x-1 or x+4).
+#include "heis_common.hpp"
+
+namespace vg {
+
+__global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict__ chains,
+                                                     const uint32_t* __restrict__ block_chain,
+                                                     float* __restrict__ arena,
+                                                     int64_t* __restrict__ partials,
+                                                     uint32_t colour, uint32_t s,
+                                                     uint32_t total_blocks, uint32_t grid) {
+    const uint32_t b = xcd_block(blockIdx.x, grid);
+    if (b >= total_blocks) return;  // uniform over the workgroup
+    const ChainDev& C = chains[block_chain[b]];
+    const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
+    int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
+
+    if (local < C.items) {
+        const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
+        const uint64_t plane = C.plane;
+        const uint32_t row = fast_div(local, C.segs_m, C.segs_s);
+        const uint32_t seg = local - row * C.segs;
+        const uint32_t z = fast_div(row, C.ly_m, C.ly_s);
+        const uint32_t y = row - z * ly;
+        const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
+        const uint32_t i0 = seg * kHeisSeg;
+        const uint32_t nvalid = min(kHeisSeg, w - i0);
+        float* own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
+        const float* oth = arena + C.off[colour ^ 1u];
+
+        bool yok, zok;
+        uint32_t y2, z2;
+        if (colour == 0) {
+            yok = y > 0 || py;
+            y2 = y > 0 ? y - 1 : ly - 1;
+            zok = z > 0 || pz;
+            z2 = z > 0 ? z - 1 : lz - 1;
+        } else {
+            yok = y + 1 < ly || py;
+            y2 = y + 1 < ly ? y + 1 : 0;
+            zok = z + 1 < lz || pz;
+            z2 = z + 1 < lz ? z + 1 : 0;
+        }
+        // slot rows in order (dy, dz) = (0,0), (1,0), (0,1), (1,1)
+        const uint32_t rows[4] = {row, y2 + ly * z, y + ly * z2, y2 + ly * z2};
+        const bool rok[4] = {true, yok, zok, yok && zok};
+        // x offset of the second slot of each row: x-1 (colour 0), x+1 (colour 1)
+        bool xe_ok;
+        uint32_t xe_pos;
+        if (colour == 0) {
+            xe_pos = i0 > 0 ? i0 - 1 : w - 1;
+            xe_ok = i0 > 0 || px;
+        } else {
+            xe_pos = i0 + kHeisSeg < w ? i0 + kHeisSeg : 0;
+            xe_ok = i0 + kHeisSeg < w || px;
+        }
+        const bool last = i0 + kHeisSeg >= w;
+
+        float4 so[3], v[4][3];
+        float xe[4][3], wrap0[4][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) so[c] = ld4(own + c * plane, true);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float* rp = oth + (uint64_t)rows[r] * pitch;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v[r][c] = ld4(rp + c * plane + i0, rok[r]);
+                xe[r][c] = rok[r] && xe_ok ? rp[c * plane + xe_pos] : 0.0f;
+                // colour 1: position w-1 bonds to position 0 across the end
+                wrap0[r][c] = colour == 1 && last && px && rok[r] ? rp[c * plane] : 0.0f;
+            }
+        }
+
+        Stream st;
+        st.start(C, i0, row, C.sweep0 + s, colour);
+        bool live = true;
+        float ns[3][4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const float o[3] = {f4get(so[0], j), f4get(so[1], j), f4get(so[2], j)};
+            ns[0][j] = o[0];
+            ns[1][j] = o[1];
+            ns[2][j] = o[2];
+            if (j >= nvalid) continue;
+            const uint32_t pos = i0 + j;
+            float h[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float acc_h = 0.0f;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float a = f4get(v[r][c], j);
+                    float bx;
+                    if (colour == 0)
+                        bx = j == 0 ? xe[r][c] : f4get(v[r][c], j - 1);
+                    else
+                        bx = pos + 1 == w ? wrap0[r][c] : (j == 3 ? xe[r][c] : f4get(v[r][c], j + 1));
+                    acc_h = r == 0 ? fa(a, bx) : fa(fa(acc_h, a), bx);
+                }
+                h[c] = acc_h;
+            }
+            float n[3];
+            heis_site(C, st, live, colour, o, h, n, acc);
+            ns[0][j] = n[0];
+            ns[1][j] = n[1];
+            ns[2][j] = n[2];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            *reinterpret_cast<float4*>(own + c * plane) =
+                make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
+    }
+
+    heis_partials(acc, partials, colour, s, total_blocks, b);
+}
+
+}  // namespace vg

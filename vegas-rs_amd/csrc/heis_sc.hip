@@ -15,49 +15,9 @@
 // two words per Marsaglia try until a try lands inside the unit disc.
 // All arithmetic is single IEEE fp32 operations in a fixed order (no
 // contraction), so the CPU oracle reproduces every bit.
-#include "vg_device.hpp"
+#include "heis_common.hpp"
 
 namespace vg {
-
-__device__ __forceinline__ float fm(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ float fa(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ float fs(float a, float b) { return __fsub_rn(a, b); }
-__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
-    return fa(fa(fm(ax, bx), fm(ay, by)), fm(az, bz));
-}
-__device__ __forceinline__ int64_t fix27(float v) {
-    return (int64_t)__float2int_rn(fm(v, kFixScale));
-}
-
-__device__ __forceinline__ float4 ld4(const float* p, bool present) {
-    if (!present) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return *reinterpret_cast<const float4*>(p);
-}
-
-__device__ __forceinline__ float f4get(const float4& v, uint32_t k) {
-    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
-}
-
-// Sequential word stream of one 4-position group.
-struct Stream {
-    U4 cur;
-    uint32_t call, word;
-    uint32_t cx, cy, cz, sweep_hi_bits, k0, k1;
-    __device__ __forceinline__ void refill() {
-        cur = philox(U4{cx, cy, cz, sweep_hi_bits | ((kHeisStream + call) << 1)}, k0, k1);
-        word = 0;
-    }
-    // Returns false once the stream is exhausted (after kHeisMaxCalls calls).
-    __device__ __forceinline__ bool next(uint32_t& out) {
-        if (word == 4) {
-            if (call + 1 >= kHeisMaxCalls) return false;
-            ++call;
-            refill();
-        }
-        out = u4_word(cur, word++);
-        return true;
-    }
-};
 
 __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__ chains,
                                                     const uint32_t* __restrict__ block_chain,
@@ -65,7 +25,6 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
                                                     int64_t* __restrict__ partials,
                                                     uint32_t colour, uint32_t s,
                                                     uint32_t total_blocks, uint32_t grid) {
-    __shared__ int64_t red[5][kBlock / 64];
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
@@ -123,20 +82,9 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
             for (int c = 0; c < 3; ++c) wrap0[c] = orow[c * plane];
         }
 
-        const uint64_t t = C.sweep0 + s;
         Stream st;
-        st.call = 0;
-        st.cx = i0 >> 2;
-        st.cy = row;
-        st.cz = (uint32_t)t;
-        st.sweep_hi_bits = ((uint32_t)(t >> 32) << 8) | colour;
-        st.k0 = (uint32_t)C.seed;
-        st.k1 = (uint32_t)(C.seed >> 32);
-        st.refill();
+        st.start(C, i0, row, C.sweep0 + s, colour);
         bool live = true;
-
-        const float jn = C.jn, kk = C.kk, hm = C.hm, beta = C.beta;
-        const float ax = C.ax, ay = C.ay, az = C.az, fx = C.fx, fy = C.fy, fz = C.fz;
         float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
@@ -170,57 +118,12 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
                 v = fa(v, f4get(zp[c], j));
                 h[c] = v;
             }
-            // Acceptance uniform, then the proposal.
-            uint32_t wu = 0;
-            float px = -ox, py = -oy, pz = -oz;
-            bool ok = live && st.next(wu);
-            live = ok;
-            const float u = unit24(wu);
-            if (C.proposal) {
-                px = 0.0f;
-                py = 0.0f;
-                pz = 1.0f;
-                while (live) {
-                    uint32_t w1, w2;
-                    if (!st.next(w1) || !st.next(w2)) {
-                        live = false;
-                        break;
-                    }
-                    const float x1 = sym24(w1), x2 = sym24(w2);
-                    const float qq = fa(fm(x1, x1), fm(x2, x2));
-                    if (qq < 1.0f) {
-                        const float root = sqrt_rn(fs(1.0f, qq));
-                        px = fm(fm(2.0f, x1), root);
-                        py = fm(fm(2.0f, x2), root);
-                        pz = fs(1.0f, fm(2.0f, qq));
-                        break;
-                    }
-                }
-            }
-            const float dx = fs(px, ox), dy = fs(py, oy), dz = fs(pz, oz);
-            const float e_ex = fm(jn, dot3(dx, dy, dz, h[0], h[1], h[2]));
-            const float pe = dot3(px, py, pz, ax, ay, az);
-            const float oe = dot3(ox, oy, oz, ax, ay, az);
-            const float e_an = fm(kk, fs(fm(pe, pe), fm(oe, oe)));
-            const float e_ze = fm(hm, dot3(dx, dy, dz, fx, fy, fz));
-            const float dE = fa(fa(e_ex, e_an), e_ze);
-            bool accept = dE < 0.0f;
-            if (!accept) accept = u < exp_nonpos(-fm(dE, beta));
-            float sx = ox, sy = oy, sz = oz;
-            if (accept) {
-                sx = px;
-                sy = py;
-                sz = pz;
-            }
-            ns[0][j] = sx;
-            ns[1][j] = sy;
-            ns[2][j] = sz;
-            acc[1] += fix27(sx);
-            acc[2] += fix27(sy);
-            acc[3] += fix27(sz);
-            const float se = dot3(sx, sy, sz, ax, ay, az);
-            acc[4] += fix27(fm(se, se));
-            if (colour == 1) acc[0] += fix27(dot3(sx, sy, sz, h[0], h[1], h[2]));
+            const float o[3] = {ox, oy, oz};
+            float n[3];
+            heis_site(C, st, live, colour, o, h, n, acc);
+            ns[0][j] = n[0];
+            ns[1][j] = n[1];
+            ns[2][j] = n[2];
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -228,23 +131,7 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
                 make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
     }
 
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-    }
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) red[k][wave] = acc[k];
-    }
-    __syncthreads();
-    if (threadIdx.x < 5) {
-        int64_t v = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kBlock / 64; ++k) v += red[threadIdx.x][k];
-        partials[((uint64_t)(s * 2u + colour) * total_blocks + b) * 5u + threadIdx.x] = v;
-    }
+    heis_partials(acc, partials, colour, s, total_blocks, b);
 }
 
 // record = {bond, mx, my, mz, axis2}: both colours for the on-site sums,

@@ -39,31 +39,6 @@ constexpr uint32_t kH = 0x80808080u;  // byte MSBs
 constexpr uint32_t kL = 0x7F7F7F7Fu;
 constexpr uint32_t kOne = 0x01010101u;
 
-__device__ __forceinline__ int32_t wave_sum(int32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Colour planes are read and written through raw buffer resources: 32-bit
-// offsets (a plane is below 4 GiB), and an offset past the plane (kNoRow)
-// reads as zeros, which is exactly an absent neighbour row on an open
-// boundary.  Dword 3 = 0x00020000 selects 32-bit data format on gfx9.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-constexpr uint32_t kNoRow = 0xFFFFFF00u;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const uint8_t* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(p), 0, bytes, 0x00020000);
-}
-
-__device__ __forceinline__ u32x4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-}
-
-__device__ __forceinline__ uint32_t bload8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
-}
-
 __device__ __forceinline__ uint32_t pick4(uint32_t k, uint32_t a, uint32_t b, uint32_t c,
                                           uint32_t d) {
     return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));

@@ -208,6 +208,32 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return s;
 }
 
+// Sum over the 64 lanes of a wave.
+__device__ __forceinline__ int32_t wave_sum(int32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Ising colour planes are read and written through raw buffer resources:
+// 32-bit offsets (a plane is below 4 GiB), and an offset past the plane
+// (kNoRow) reads as zeros, which is exactly an absent neighbour row on an
+// open boundary.  Dword 3 = 0x00020000 selects 32-bit data format on gfx9.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kNoRow = 0xFFFFFF00u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const uint8_t* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(p), 0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ u32x4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+}
+
+__device__ __forceinline__ uint32_t bload8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+
 // 24-bit uniform in [0,1) and in [-1,1) from a 32-bit draw (exact in fp32).
 VG_HD float unit24(uint32_t w) { return (float)(w >> 8) * 5.9604644775390625e-08f; }
 VG_HD float sym24(uint32_t w) { return (float)(w >> 8) * 1.1920928955078125e-07f - 1.0f; }
