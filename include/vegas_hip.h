@@ -206,6 +206,109 @@ int vg_batch_last_run_ms(const vg_batch* batch, double* ms);
 int vg_batch_set_timing(vg_batch* batch, int enabled);
 int vg_batch_pass_stats(const vg_batch* batch, int32_t spin, double* total_ms, uint64_t* launches);
 
+/* ------------------------------------------------------------------ */
+/* Job server: continuous batching of simulation jobs on one GPU.       */
+/* A job is a chain plus a program -- the stages a Program::run drives   */
+/* (Relax, CoolDown, HysteresisLoop, program.rs:97-336) -- and its      */
+/* result is what StatSensor writes per measured stage                  */
+/* (instrument.rs:93-142).  No counterpart in the reference (which runs */
+/* one Machine at a time); it serves many Machines at once.             */
+/*                                                                      */
+/* Jobs are admitted in arrival order (FIFO) when a slot is free and    */
+/* their spins fit in the server's device pool (admission by HBM        */
+/* footprint); admitted jobs share every sweep block with the jobs      */
+/* already running.  Each job's spins, sweep counter (its Philox        */
+/* counter) and accumulators stay resident until it finishes, and its   */
+/* results are bit-identical whether it runs alone or in any mix.       */
+/* The server runs its own worker thread and HIP stream; the functions  */
+/* below may be called from any thread.                                 */
+/* ------------------------------------------------------------------ */
+typedef struct vg_server vg_server;
+
+typedef struct vg_server_config {
+    uint64_t memory_bytes;   /* device pool holding the admitted jobs' spins */
+    uint32_t max_jobs;       /* jobs on the device at once (slots) */
+    uint32_t quantum_sweeps; /* sweeps per scheduling quantum, 1..256 */
+    uint32_t tile;           /* Ising sc tile: 1, 2, 4 or 8 */
+    int32_t use_graph;       /* 1: replay quanta from hipGraphs while the job set is unchanged */
+} vg_server_config;
+
+/* One stage: set the thermostat, run `relax` sweeps, then `measure`
+ * sweeps whose E and |M| are accumulated.  The thermostat is
+ * Thermostat::new(temperature, Field::new(orientation, field)) with the
+ * reference's semantics (T clamped to DBL_EPSILON, |field| applied). */
+typedef struct vg_stage {
+    double temperature;
+    double field;
+    double field_dir[3];
+    uint64_t relax;
+    uint64_t measure;
+} vg_stage;
+
+typedef struct vg_job_desc {
+    vg_chain_desc chain;     /* initial spins: State::rand_with_size from chain.seed */
+    uint32_t n_stages;
+    const vg_stage* stages;
+} vg_job_desc;
+
+/* StatSensor's line for a stage with measure > 0 (instrument.rs:113-123):
+ * T, |h|, <E>, Var(E)/(N T^2), <|M|>, Var(|M|)/(N T), Binder(|M|). */
+typedef struct vg_stage_result {
+    double temperature;
+    double field;
+    double energy;
+    double specific_heat;
+    double magnetization;
+    double susceptibility;
+    double binder;
+    uint64_t measured;       /* sweeps accumulated (0: relax-only stage) */
+} vg_stage_result;
+
+#define VG_JOB_QUEUED 0
+#define VG_JOB_RUNNING 1
+#define VG_JOB_DONE 2
+#define VG_JOB_FAILED 3
+#define VG_JOB_CANCELLED 4
+
+typedef struct vg_job_status {
+    int32_t state;           /* VG_JOB_* */
+    uint32_t stage;          /* current stage index */
+    uint64_t sweeps_done;
+    uint64_t sweeps_total;
+    double submit_s;         /* seconds since the server started; -1 = not yet */
+    double start_s;          /* admitted */
+    double end_s;            /* finished */
+    double last_energy;      /* E and |M| after the job's latest sweep */
+    double last_magnetization;
+} vg_job_status;
+
+typedef struct vg_server_stats {
+    uint64_t submitted, completed, running, queued;
+    uint64_t attempts;       /* spin-flip attempts executed */
+    uint64_t quanta;         /* sweep blocks launched */
+    double device_ms;        /* device time of the sweep blocks */
+    uint64_t pool_bytes, pool_used, pool_peak;
+    uint32_t peak_jobs;      /* most jobs on the device at once */
+    uint32_t pad;
+} vg_server_stats;
+
+int vg_server_create(vg_ctx* ctx, const vg_server_config* config, vg_server** out);
+/* Cancels queued and running jobs, stops the worker, frees the pool. */
+int vg_server_destroy(vg_server* server);
+/* Queues a job.  VG_ERR_ADMISSION if its spins could never fit the pool;
+ * program errors (no stages, zero steps) as the reference's ProgramError. */
+int vg_server_submit(vg_server* server, const vg_job_desc* job, uint64_t* job_id);
+int vg_server_status(vg_server* server, uint64_t job_id, vg_job_status* status);
+/* Blocks until the job is done, failed or cancelled, or timeout_s passes
+ * (timeout_s < 0: no limit); fills status either way. */
+int vg_server_wait(vg_server* server, uint64_t job_id, double timeout_s, vg_job_status* status);
+/* Per-stage results of a finished job (n = its n_stages). */
+int vg_server_results(vg_server* server, uint64_t job_id, vg_stage_result* results, uint32_t n);
+int vg_server_cancel(vg_server* server, uint64_t job_id);
+/* Forgets a finished job (its status and results). */
+int vg_server_release(vg_server* server, uint64_t job_id);
+int vg_server_get_stats(vg_server* server, vg_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

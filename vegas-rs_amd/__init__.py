@@ -61,6 +61,44 @@ class SweepSums(C.Structure):
     _fields_ = [("bond", C.c_int64), ("mag", C.c_int64 * 3), ("axis2", C.c_int64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_bytes", C.c_uint64), ("max_jobs", C.c_uint32),
+                ("quantum_sweeps", C.c_uint32), ("tile", C.c_uint32), ("use_graph", C.c_int32)]
+
+
+class Stage(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("field", C.c_double),
+                ("field_dir", C.c_double * 3), ("relax", C.c_uint64), ("measure", C.c_uint64)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("chain", ChainDesc), ("n_stages", C.c_uint32), ("stages", C.POINTER(Stage))]
+
+
+class StageResult(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("field", C.c_double), ("energy", C.c_double),
+                ("specific_heat", C.c_double), ("magnetization", C.c_double),
+                ("susceptibility", C.c_double), ("binder", C.c_double),
+                ("measured", C.c_uint64)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("stage", C.c_uint32), ("sweeps_done", C.c_uint64),
+                ("sweeps_total", C.c_uint64), ("submit_s", C.c_double), ("start_s", C.c_double),
+                ("end_s", C.c_double), ("last_energy", C.c_double),
+                ("last_magnetization", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("running", C.c_uint64),
+                ("queued", C.c_uint64), ("attempts", C.c_uint64), ("quanta", C.c_uint64),
+                ("device_ms", C.c_double), ("pool_bytes", C.c_uint64), ("pool_used", C.c_uint64),
+                ("pool_peak", C.c_uint64), ("peak_jobs", C.c_uint32), ("pad", C.c_uint32)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = 0, 1, 2, 3, 4
+
+
 if not LIB_PATH.exists():
     raise ImportError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
 
@@ -92,6 +130,15 @@ _SIGS = {
     "vg_batch_get_tile": [_vp, _P(C.c_uint32)],
     "vg_batch_autotune": [_vp, C.c_uint32, _P(C.c_uint32)],
     "vg_batch_pass_stats": [_vp, C.c_int32, _P(C.c_double), _P(C.c_uint64)],
+    "vg_server_create": [_vp, _P(ServerConfig), _P(_vp)],
+    "vg_server_destroy": [_vp],
+    "vg_server_submit": [_vp, _P(JobDesc), _P(C.c_uint64)],
+    "vg_server_status": [_vp, C.c_uint64, _P(JobStatus)],
+    "vg_server_wait": [_vp, C.c_uint64, C.c_double, _P(JobStatus)],
+    "vg_server_results": [_vp, C.c_uint64, _P(StageResult), C.c_uint32],
+    "vg_server_cancel": [_vp, C.c_uint64],
+    "vg_server_release": [_vp, C.c_uint64],
+    "vg_server_get_stats": [_vp, _P(ServerStats)],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -294,3 +341,135 @@ class Batch:
         v = C.c_double()
         _check(_lib.vg_batch_last_run_ms(self.h, C.byref(v)))
         return v.value
+
+
+# --------------------------------------------------------------- programs --
+# Stage lists of the reference's programs (src/program.rs:97-336) with their
+# defaults, validation order and f64 loops.  A stage is (temperature, field,
+# relax sweeps, measure sweeps); the field acts along +z with magnitude
+# |field| (Field::new(S::up(), field), state.rs:219-221).
+
+def _stage(T, field, relax, measure):
+    return Stage(T, field, (C.c_double * 3)(0.0, 0.0, 1.0), relax, measure)
+
+
+def relax(steps=1000, temperature=3.0, field=0.0):
+    """Relax (program.rs:97-124): one relax phase, nothing measured."""
+    if steps == 0:
+        raise VegasError(2, "there should be at least one step")
+    if temperature < np.finfo(float).eps:
+        raise VegasError(3, "temperature must be greater than zero")
+    return [_stage(temperature, field, steps, 0)]
+
+
+def cool_down(max_temperature=3.0, min_temperature=0.05, cool_rate=0.1, relax=1000,
+              steps=20000, field=0.0):
+    """CoolDown (program.rs:181-212)."""
+    eps = np.finfo(float).eps
+    if max_temperature < min_temperature:
+        raise VegasError(1, "maximum temperature must be greater than minimum temperature")
+    if steps == 0:
+        raise VegasError(2, "there should be at least one step")
+    if min_temperature < eps:
+        raise VegasError(3, "temperature must be greater than zero")
+    if cool_rate < eps:
+        raise VegasError(4, "cooling rate must be greater than zero")
+    out, t = [], max_temperature
+    while True:
+        out.append(_stage(t, field, relax, steps))
+        t -= cool_rate
+        if t < min_temperature:
+            break
+    return out
+
+
+def hysteresis_loop(steps=1000, relax=1000, temperature=3.0, max_field=1.0, field_step=0.1):
+    """HysteresisLoop (program.rs:281-334): 0 -> +max -> -max -> +max."""
+    eps = np.finfo(float).eps
+    if steps == 0:
+        raise VegasError(2, "there should be at least one step")
+    if temperature < eps:
+        raise VegasError(3, "temperature must be greater than zero")
+    if max_field < eps:
+        raise VegasError(5, "maximum field must be greater than zero")
+    if field_step < eps:
+        raise VegasError(6, "field step must be greater than zero")
+    out, h = [], 0.0
+    while True:
+        out.append(_stage(temperature, h, relax, steps))
+        h += field_step
+        if h > max_field:
+            break
+    while True:
+        out.append(_stage(temperature, h, relax, steps))
+        h -= field_step
+        if h < -max_field:
+            break
+    while True:
+        out.append(_stage(temperature, h, relax, steps))
+        h += field_step
+        if h > max_field:
+            break
+    return out
+
+
+class Server:
+    """Job server with continuous batching (vg_server_* of vegas_hip.h)."""
+
+    def __init__(self, ctx: Context, memory_bytes=1 << 30, max_jobs=256, quantum_sweeps=100,
+                 tile=2, use_graph=False):
+        cfg = ServerConfig(memory_bytes, max_jobs, quantum_sweeps, tile, 1 if use_graph else 0)
+        h = _vp()
+        _check(_lib.vg_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.n_stages = {}
+
+    def close(self):
+        if self.h:
+            _lib.vg_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, desc: ChainDesc, stages) -> int:
+        stages = list(stages)
+        arr = (Stage * len(stages))(*stages)
+        job = JobDesc(desc, len(stages), arr)
+        jid = C.c_uint64()
+        _check(_lib.vg_server_submit(self.h, C.byref(job), C.byref(jid)))
+        self.n_stages[jid.value] = len(stages)
+        return jid.value
+
+    def status(self, jid: int) -> JobStatus:
+        st = JobStatus()
+        _check(_lib.vg_server_status(self.h, jid, C.byref(st)))
+        return st
+
+    def wait(self, jid: int, timeout: float = -1.0) -> JobStatus:
+        st = JobStatus()
+        _check(_lib.vg_server_wait(self.h, jid, timeout, C.byref(st)))
+        return st
+
+    def results(self, jid: int) -> np.ndarray:
+        """Per-stage (T, |h|, <E>, C, <|M|>, chi, U4, measured) as float64 rows."""
+        n = self.n_stages[jid]
+        arr = (StageResult * n)()
+        _check(_lib.vg_server_results(self.h, jid, arr, n))
+        return np.array([[r.temperature, r.field, r.energy, r.specific_heat, r.magnetization,
+                          r.susceptibility, r.binder, float(r.measured)] for r in arr])
+
+    def cancel(self, jid: int) -> None:
+        _check(_lib.vg_server_cancel(self.h, jid))
+
+    def release(self, jid: int) -> None:
+        _check(_lib.vg_server_release(self.h, jid))
+        self.n_stages.pop(jid, None)
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(_lib.vg_server_get_stats(self.h, C.byref(st)))
+        return st

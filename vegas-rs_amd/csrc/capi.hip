@@ -1,46 +1,11 @@
 // C ABI of the engine (declared in include/vegas_hip.h): contexts, batches of
 // chains packed in per-kernel arenas, the sweep-block driver (plain launches
 // or a replayed hipGraph), and the host side of the observables.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cfloat>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <string>
-#include <vector>
-
-#include "../../include/vegas_hip.h"
-#include "vg_device.hpp"
-
-namespace vg {
-template <uint32_t K>
-__global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
-                              uint32_t, uint32_t, uint32_t);
-__global__ void ising_reduce(const ChainDev*, const int32_t*, uint32_t, uint32_t, int64_t*);
-__global__ void ising_init(const ChainDev*, uint8_t*);
-__global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t);
-__global__ void ising_gather(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
-__global__ void heis_sc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
-                             uint32_t, uint32_t, uint32_t);
-__global__ void ising_bcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
-                               uint32_t, uint32_t, uint32_t);
-__global__ void heis_bcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
-                              uint32_t, uint32_t, uint32_t);
-__global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t, int64_t*);
-__global__ void heis_clear(const ChainDev*, float*);
-__global__ void heis_init(const ChainDev*, float*);
-__global__ void heis_scatter(const ChainDev*, float*, const double*, uint64_t);
-__global__ void heis_gather(const ChainDev*, const float*, double*, uint64_t);
-__global__ void zero_i64(int64_t*, uint64_t);
-}  // namespace vg
+#include "engine.hpp"
 
 using vg::ChainDev;
 
-namespace {
+namespace vgi {
 
 thread_local std::string g_error;
 
@@ -53,86 +18,6 @@ int fail(int code, const char* fmt, ...) {
     g_error = buf;
     return code;
 }
-
-#define VG_HIP(call)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(e_ == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE, \
-                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,  \
-                        __LINE__);                                                        \
-    } while (0)
-
-uint64_t round_up(uint64_t v, uint64_t m) { return (v + m - 1) / m * m; }
-
-// Best-effort cleanup: errors there cannot be reported to anyone.
-inline void quiet(hipError_t) {}
-
-// Kernel groups: chains of one group share an arena and one pass kernel.
-enum GroupKind { kIsingSc = 0, kHeisSc = 1, kIsingBcc = 2, kHeisBcc = 3, kNumGroups = 4 };
-
-bool is_ising(int g) { return g == kIsingSc || g == kIsingBcc; }
-
-int group_of(const vg_chain_desc& d) {
-    const bool bcc = d.lattice.kind == VG_LATTICE_BCC;
-    if (d.spin == VG_SPIN_HEISENBERG) return bcc ? kHeisBcc : kHeisSc;
-    return bcc ? kIsingBcc : kIsingSc;
-}
-
-struct Group {
-    std::vector<uint32_t> members;  // batch chain indices
-    std::vector<ChainDev> host;     // descriptors, uploaded each sweep block
-    ChainDev* d_chains = nullptr;
-    ChainDev* h_staging = nullptr;  // pinned copy used for the upload
-    uint32_t* d_block_chain = nullptr;
-    uint32_t block_chain_cap = 0;
-    uint32_t tile = 4;  // Ising: z planes per work item (template parameter of the pass)
-    uint32_t total_blocks = 0;
-    uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
-    void* arena = nullptr;
-    uint64_t arena_bytes = 0;
-    void* partials = nullptr;  // [sweep][colour][block][quantity]
-    uint64_t partial_bytes_per_sweep = 0;
-};
-
-struct ChainHost {
-    vg_chain_desc desc;
-    uint64_t n_sites = 0;
-    uint64_t sweeps = 0;
-    double temperature = 2.8;  // Thermostat::new(2.8, Field::zero()) of input.rs:274
-    double field_mag = 0.0;
-    double field_dir[3] = {0.0, 0.0, 1.0};
-    int group = 0;
-    uint32_t index = 0;  // index in its group
-};
-
-}  // namespace
-
-struct vg_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-};
-
-struct vg_batch {
-    vg_ctx* ctx = nullptr;
-    std::vector<ChainHost> chains;
-    Group groups[kNumGroups];
-    uint32_t max_block_sweeps = 256;
-    int64_t* d_records = nullptr;  // [chain][sweep][5]
-    int64_t* h_records = nullptr;  // pinned
-    bool use_graph = false;
-    std::map<uint32_t, hipGraphExec_t> graphs;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0;
-    // Per-launch timing of the colour-pass kernels (plain launches only).
-    bool timing = false;
-    std::vector<hipEvent_t> pass_events;  // start/stop pairs
-    uint32_t pass_events_used = 0;
-    double pass_ms[kNumGroups] = {};
-    uint64_t pass_launches[kNumGroups] = {};
-};
-
-namespace {
 
 int coordination(int kind) { return kind == VG_LATTICE_SC ? 6 : 8; }
 
@@ -258,6 +143,45 @@ void refresh_params(vg_batch* B, uint32_t i) {
 
 // Ising passes write {mag, bond} per wave (4 waves per workgroup); Heisenberg
 // passes write {bond, mx, my, mz, axis2} per workgroup.
+int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
+    std::memset(&d, 0, sizeof d);
+    const vg_lattice& L = ch.desc.lattice;
+    d.seed = ch.desc.seed;
+    d.lx = L.lx;
+    d.ly = L.ly;
+    d.lz = L.lz;
+    d.kind = (uint32_t)L.kind;
+    d.pbc = L.pbc;
+    d.z = (uint32_t)coordination(L.kind);
+    d.rows = L.ly * L.lz;
+    d.w = L.kind == VG_LATTICE_SC ? L.lx / 2 : L.lx;
+    d.proposal = (uint32_t)ch.desc.proposal;
+    d.slot = slot;
+    d.limit = 0xFFFFFFFFu;
+    const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
+    const uint32_t align = is_ising(g) ? 64u : 16u;  // 64-byte aligned rows
+    d.pitch = (uint32_t)round_up(d.w, align);
+    d.segs = (d.w + seg - 1) / seg;
+    d.hy = (d.ly + 1) / 2;
+    if ((uint64_t)d.rows * d.segs > 0x7FFFFFFFull - vg::kBlock || (uint64_t)d.rows > 0x7FFFFFFFull)
+        return fail(VG_ERR_INVALID_ARGUMENT, "chain %u too large", slot);
+    magic_div(d.segs, d.segs_m, d.segs_s);
+    magic_div(d.ly, d.ly_m, d.ly_s);
+    magic_div(d.hy, d.hy_m, d.hy_s);
+    d.plane = (uint64_t)d.rows * d.pitch;
+    if (is_ising(g) && d.plane > 0xFFFFF000ull)
+        return fail(VG_ERR_UNSUPPORTED_LATTICE,
+                    "chain %u: an Ising colour plane of %llu bytes exceeds 4 GiB", slot,
+                    (unsigned long long)d.plane);
+    if (is_ising(g))
+        ising_thresholds(ch, d);
+    else
+        heis_params(ch, d);
+    return VG_OK;
+}
+
+uint64_t colour_elems(int g, const ChainDev& d) { return (is_ising(g) ? 1 : 3) * d.plane; }
+
 uint32_t quantities(int group) { return is_ising(group) ? 8u : 5u; }
 uint32_t quantity_bytes(int group) { return is_ising(group) ? 4u : 8u; }
 
@@ -320,7 +244,7 @@ int free_batch(vg_batch* B) {
         if (G.d_chains) quiet(hipFree(G.d_chains));
         if (G.h_staging) quiet(hipHostFree(G.h_staging));
         if (G.d_block_chain) quiet(hipFree(G.d_block_chain));
-        if (G.arena) quiet(hipFree(G.arena));
+        if (G.arena && !G.shared_arena) quiet(hipFree(G.arena));
         if (G.partials) quiet(hipFree(G.partials));
     }
     if (B->d_records) quiet(hipFree(B->d_records));
@@ -566,7 +490,9 @@ int check_chain(const vg_batch* B, uint32_t chain) {
     return VG_OK;
 }
 
-}  // namespace
+}  // namespace vgi
+
+using namespace vgi;
 
 extern "C" {
 
@@ -654,55 +580,25 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         for (size_t k = 0; k < G.members.size(); ++k) {
             ChainHost& ch = B->chains[G.members[k]];
             ChainDev& d = G.host[k];
-            std::memset(&d, 0, sizeof d);
-            const vg_lattice& L = ch.desc.lattice;
-            d.seed = ch.desc.seed;
-            d.lx = L.lx;
-            d.ly = L.ly;
-            d.lz = L.lz;
-            d.kind = (uint32_t)L.kind;
-            d.pbc = L.pbc;
-            d.z = (uint32_t)coordination(L.kind);
-            d.rows = L.ly * L.lz;
-            d.w = L.kind == VG_LATTICE_SC ? L.lx / 2 : L.lx;
-            d.proposal = (uint32_t)ch.desc.proposal;
-            d.slot = G.members[k];
-            const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
-            const uint32_t align = is_ising(g) ? 64u : 16u;  // 64-byte aligned rows
-            d.pitch = (uint32_t)round_up(d.w, align);
-            d.segs = (d.w + seg - 1) / seg;
-            d.hy = (d.ly + 1) / 2;
-            if ((uint64_t)d.rows * d.segs > 0x7FFFFFFFull - vg::kBlock ||
-                (uint64_t)d.rows > 0x7FFFFFFFull) {
+            int rc = init_chain_dev(ch, g, G.members[k], d);
+            if (rc) {
                 free_batch(B);
-                return fail(VG_ERR_INVALID_ARGUMENT, "chain %u too large", G.members[k]);
+                return rc;
             }
-            magic_div(d.segs, d.segs_m, d.segs_s);
-            magic_div(d.ly, d.ly_m, d.ly_s);
-            magic_div(d.hy, d.hy_m, d.hy_s);
-            d.plane = (uint64_t)d.rows * d.pitch;
-            if (is_ising(g) && d.plane > 0xFFFFF000ull) {
-                free_batch(B);
-                return fail(VG_ERR_UNSUPPORTED_LATTICE,
-                            "chain %u: an Ising colour plane of %llu bytes exceeds 4 GiB",
-                            G.members[k], (unsigned long long)d.plane);
-            }
-            const uint64_t planes_per_colour = is_ising(g) ? 1 : 3;
-            const uint64_t elem_align = is_ising(g) ? 256 : 64;  // 256-byte chain alignment
+            const uint64_t elem_align = 256 / elem_bytes(g);  // 256-byte chain alignment
             offset = round_up(offset, elem_align);
             d.off[0] = offset;
-            offset += planes_per_colour * d.plane;
+            offset += colour_elems(g, d);
             offset = round_up(offset, elem_align);
             d.off[1] = offset;
-            offset += planes_per_colour * d.plane;
-            refresh_params(B, G.members[k]);
+            offset += colour_elems(g, d);
         }
-        const uint64_t elem = is_ising(g) ? 1 : 4;
-        G.arena_bytes = std::max<uint64_t>(offset * elem, 256);
+        G.arena_bytes = std::max<uint64_t>(offset * elem_bytes(g), 256);
         hipError_t e = hipMalloc(&G.arena, G.arena_bytes);
         if (e == hipSuccess) e = hipMemset(G.arena, 0, G.arena_bytes);
         if (e == hipSuccess) e = hipMalloc(&G.d_chains, G.host.size() * sizeof(ChainDev));
         if (e == hipSuccess) e = hipHostMalloc(&G.h_staging, G.host.size() * sizeof(ChainDev));
+        G.chains_cap = (uint32_t)G.host.size();
         if (e != hipSuccess) {
             free_batch(B);
             return fail(e == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE,

@@ -1,0 +1,160 @@
+// Internal interface of the engine shared by the C ABI (capi.hip) and the
+// job server (server.hip): chain and group bookkeeping, descriptor upload,
+// work layout, the sweep-block driver and the host side of the observables.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vegas_hip.h"
+#include "vg_device.hpp"
+
+namespace vg {
+template <uint32_t K>
+__global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
+                              uint32_t, uint32_t, uint32_t);
+__global__ void ising_reduce(const ChainDev*, const int32_t*, uint32_t, uint32_t, int64_t*);
+__global__ void ising_init(const ChainDev*, uint8_t*);
+__global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t);
+__global__ void ising_gather(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
+__global__ void heis_sc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
+                             uint32_t, uint32_t, uint32_t);
+__global__ void ising_bcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
+                               uint32_t, uint32_t, uint32_t);
+__global__ void heis_bcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
+                              uint32_t, uint32_t, uint32_t);
+__global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t, int64_t*);
+__global__ void heis_clear(const ChainDev*, float*);
+__global__ void heis_init(const ChainDev*, float*);
+__global__ void heis_scatter(const ChainDev*, float*, const double*, uint64_t);
+__global__ void heis_gather(const ChainDev*, const float*, double*, uint64_t);
+__global__ void zero_i64(int64_t*, uint64_t);
+}  // namespace vg
+
+using vg::ChainDev;
+
+namespace vgi {
+
+// Thread-local message of the last failure (vg_last_error) and a setter
+// that returns the status code.
+extern thread_local std::string g_error;
+int fail(int code, const char* fmt, ...);
+
+#define VG_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE, \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,  \
+                        __LINE__);                                                        \
+    } while (0)
+
+inline uint64_t round_up(uint64_t v, uint64_t m) { return (v + m - 1) / m * m; }
+
+// Best-effort cleanup: errors there cannot be reported to anyone.
+inline void quiet(hipError_t) {}
+
+// Kernel groups: chains of one group share an arena and one pass kernel.
+enum GroupKind { kIsingSc = 0, kHeisSc = 1, kIsingBcc = 2, kHeisBcc = 3, kNumGroups = 4 };
+
+inline bool is_ising(int g) { return g == kIsingSc || g == kIsingBcc; }
+
+inline int group_of(const vg_chain_desc& d) {
+    const bool bcc = d.lattice.kind == VG_LATTICE_BCC;
+    if (d.spin == VG_SPIN_HEISENBERG) return bcc ? kHeisBcc : kHeisSc;
+    return bcc ? kIsingBcc : kIsingSc;
+}
+
+struct Group {
+    std::vector<uint32_t> members;  // batch chain indices
+    std::vector<ChainDev> host;     // descriptors, uploaded each sweep block
+    ChainDev* d_chains = nullptr;
+    ChainDev* h_staging = nullptr;  // pinned copy used for the upload
+    uint32_t* d_block_chain = nullptr;
+    uint32_t block_chain_cap = 0;
+    uint32_t tile = 4;  // Ising: z planes per work item (template parameter of the pass)
+    uint32_t total_blocks = 0;
+    uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
+    void* arena = nullptr;
+    uint64_t arena_bytes = 0;
+    void* partials = nullptr;  // [sweep][colour][block][quantity]
+    uint64_t partial_bytes_per_sweep = 0;
+    // Capacities of d_chains / h_staging (descriptors) and of partials
+    // (bytes), for batches whose membership changes (the job server).
+    uint32_t chains_cap = 0;
+    uint64_t partial_cap = 0;
+    bool shared_arena = false;  // arena is owned by someone else (the server's pool)
+};
+
+struct ChainHost {
+    vg_chain_desc desc;
+    uint64_t n_sites = 0;
+    uint64_t sweeps = 0;
+    double temperature = 2.8;  // Thermostat::new(2.8, Field::zero()) of input.rs:274
+    double field_mag = 0.0;
+    double field_dir[3] = {0.0, 0.0, 1.0};
+    int group = 0;
+    uint32_t index = 0;  // index in its group
+};
+
+}  // namespace vgi
+
+struct vg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct vg_batch {
+    vg_ctx* ctx = nullptr;
+    std::vector<vgi::ChainHost> chains;
+    vgi::Group groups[vgi::kNumGroups];
+    uint32_t max_block_sweeps = 256;
+    int64_t* d_records = nullptr;  // [chain][sweep][5]
+    int64_t* h_records = nullptr;  // pinned
+    bool use_graph = false;
+    std::map<uint32_t, hipGraphExec_t> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    // Per-launch timing of the colour-pass kernels (plain launches only).
+    bool timing = false;
+    std::vector<hipEvent_t> pass_events;  // start/stop pairs
+    uint32_t pass_events_used = 0;
+    double pass_ms[vgi::kNumGroups] = {};
+    uint64_t pass_launches[vgi::kNumGroups] = {};
+};
+
+namespace vgi {
+
+int validate_desc(const vg_chain_desc& d);
+int coordination(int kind);
+void refresh_params(vg_batch* B, uint32_t i);
+// Acceptance tables (Ising) or fp32 parameters (Heisenberg) of a chain's
+// current thermostat.
+void ising_thresholds(const ChainHost& ch, ChainDev& d);
+void heis_params(const ChainHost& ch, ChainDev& d);
+int free_batch(vg_batch* B);
+uint32_t quantities(int group);
+uint32_t quantity_bytes(int group);
+int upload_descriptors(vg_batch* B, int g);
+int layout_work(vg_batch* B, int g, uint32_t tile);
+int enqueue_block(vg_batch* B, uint32_t nsw);
+int launch_block(vg_batch* B, uint32_t nsw);
+double sums_scale(int spin);
+void observables(const ChainHost& ch, const vg_sweep_sums& r, double* E, double* M);
+// Geometry and parameters of chain `ch` as member `slot` of group g (sizes,
+// magic divisors, acceptance tables); the plane offsets are set by the caller.
+int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d);
+// Elements (bytes for Ising, floats for Heisenberg) of one colour of a chain.
+uint64_t colour_elems(int g, const ChainDev& d);
+// Bytes of one element in group g's arena.
+inline uint64_t elem_bytes(int g) { return is_ising(g) ? 1 : 4; }
+
+}  // namespace vgi

@@ -24,6 +24,7 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
+    if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
     int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
 

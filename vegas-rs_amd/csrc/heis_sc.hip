@@ -28,6 +28,7 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
+    if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
     int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
 
@@ -143,6 +144,7 @@ __global__ __launch_bounds__(256) void heis_reduce(const ChainDev* __restrict__ 
     __shared__ int64_t red[5][kBlock / 64];
     const ChainDev& C = chains[blockIdx.x];
     const uint32_t s = blockIdx.y;
+    if (s >= C.limit) return;  // this chain ran fewer sweeps in the block
     int64_t acc[5] = {0, 0, 0, 0, 0};
     for (uint32_t k = threadIdx.x; k < C.nblocks; k += kBlock) {
         const uint32_t blk = C.block0 + k;

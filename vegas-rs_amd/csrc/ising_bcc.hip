@@ -35,6 +35,7 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
+    if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid < 34) lut[tid] = (&C.thr[0][0])[tid];
     __syncthreads();

@@ -86,6 +86,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
+    if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid < 34) lut[tid] = (&C.thr[0][0])[tid];
     if (tid >= 64 && tid < 84) tab[tid - 64] = (&C.t8[0][0][0])[tid - 64];
@@ -394,6 +395,7 @@ __global__ __launch_bounds__(256) void ising_reduce(const ChainDev* __restrict__
     __shared__ int64_t red[2][kBlock / 64];
     const ChainDev& C = chains[blockIdx.x];
     const uint32_t s = blockIdx.y;
+    if (s >= C.limit) return;  // this chain ran fewer sweeps in the block
     int64_t mm = 0, bb = 0;
     for (uint32_t k = threadIdx.x; k < C.nblocks; k += kBlock) {
         const uint32_t blk = C.block0 + k;

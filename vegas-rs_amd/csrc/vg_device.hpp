@@ -108,7 +108,7 @@ struct ChainDev {
     uint32_t proposal;     // 0 flip, 1 random
     uint32_t slot;         // chain index in its batch (record row)
     uint32_t z;            // coordination number: 6 (sc) or 8 (bcc)
-    uint32_t pad0;
+    uint32_t limit;        // sweeps of the current block this chain runs (the rest skip it)
     // n / d == (n * m) >> s for n < 2^31 (segs, ly and hy divisors)
     uint32_t segs_m, segs_s, ly_m, ly_s;
     // Ising work order: rows of one y parity, a tile of z planes per lane
