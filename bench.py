@@ -148,6 +148,143 @@ def run_batched(vg, ctx, sweeps, baseline_sweeps, n_chains=512):
     }
 
 
+def run_heisenberg(vg, ctx, side, sweeps):
+    """Heisenberg sc side^3 x 3 chains (J = 1, K = 0.1 along z, B = 0.05 along
+    z, uniform-sphere proposals, random start from seed 7) at T = 0.5, 1.4,
+    2.5; attempts/s and the pass kernel's roofline at 24 B per attempt."""
+    temps = [0.5, 1.4, 2.5]
+    H = vg.hamiltonian(exchange=1.0, zeeman=True, anisotropy=0.1)
+    L = vg.lattice(vg.SC, side, side, side)
+    descs = [vg.chain(L, vg.HEISENBERG, vg.RANDOM, H, 7) for _ in temps]
+    n = side ** 3
+    with vg.Batch(ctx, descs) as b:
+        b.randomize()
+        for i, T in enumerate(temps):
+            b.set_thermostat(i, T, 0.05)
+        b.set_graph(True)
+        b.run(5)
+        t0 = time.perf_counter()
+        e, m = b.run(sweeps)
+        wall = time.perf_counter() - t0
+        dev_ms = b.last_run_ms()
+        b.set_graph(False)
+        b.set_timing(True)
+        b.run(20, observe=False)
+        ev_ms, ev_n = b.pass_stats(vg.HEISENBERG)
+    value = n * len(temps) * sweeps / wall
+    per_launch = n * len(temps) // 2
+    avg_ms = dev_ms / (2 * sweeps)
+    achieved = 24 * per_launch / (avg_ms * 1e-3) / 1e9
+    log(f"heisenberg: {value:.4e} attempts/s, pass {avg_ms * 1e3:.1f} us -> {achieved:.0f} GB/s")
+    return {
+        "workload": (f"heis-sc{side}-f32: 3 chains sc {side}^3, J=1, K=0.1 z, B=0.05 z, "
+                     f"uniform-sphere proposals, T={temps}"),
+        "value": value,
+        "unit": "spin-flip attempts/s",
+        "sweeps": sweeps,
+        "ms_per_step": wall * 1e3 / sweeps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "heis_sc_pass",
+                     "algorithmic_bytes_per_launch": 24 * per_launch,
+                     "mean_launch_us": avg_ms * 1e3,
+                     "mean_launch_us_per_launch_events": ev_ms / ev_n * 1e3},
+        "observables_last_sweep": {f"T={T}": {"E/N": float(e[i, -1]) / n,
+                                              "|M|/N": float(m[i, -1]) / n}
+                                   for i, T in enumerate(temps)},
+    }
+
+
+def served_jobs(vg, n_jobs, gen_seed=2025):
+    """The served mix: 60% Ising Relax (sc L^3, L even in [16, 96], T uniform in
+    [2, 6], 2000 sweeps), 30% Heisenberg CoolDown (sc L^3, L even in [16, 64],
+    K = 0.1 along z, T 3.0 -> 0.1 by 0.1 = 29 temperatures, 500 measured
+    sweeps each), 10% Heisenberg HysteresisLoop (sc 64^3, K = 0.1, T = 1.0,
+    field 0 -> 0.5 -> -0.5 -> 0.5 by 0.05 = 55 stages, 1000 measured sweeps
+    each).  Compound(Exchange J=1, Zeeman[, Anisotropy]); seeds = job index.
+    Even L: a periodic sc axis needs even length to be two-colourable."""
+    import numpy as np
+    rng = np.random.default_rng(gen_seed)
+    ising_h = vg.hamiltonian(exchange=1.0, zeeman=True)
+    heis_h = vg.hamiltonian(exchange=1.0, zeeman=True, anisotropy=0.1)
+    cool = vg.cool_down(3.0, 0.1, 0.1, relax=0, steps=500)
+    loop = vg.hysteresis_loop(steps=1000, relax=0, temperature=1.0, max_field=0.5,
+                              field_step=0.05)
+    jobs = []
+    for k in range(n_jobs):
+        u = rng.random()
+        if u < 0.6:
+            L = int(rng.integers(8, 49)) * 2
+            T = float(rng.uniform(2.0, 6.0))
+            d = vg.chain(vg.lattice(vg.SC, L, L, L), vg.ISING, vg.FLIP, ising_h, k)
+            jobs.append(("ising-relax", d, vg.relax(2000, T), L ** 3 * 2000))
+        elif u < 0.9:
+            L = int(rng.integers(8, 33)) * 2
+            d = vg.chain(vg.lattice(vg.SC, L, L, L), vg.HEISENBERG, vg.RANDOM, heis_h, k)
+            jobs.append(("heis-cooldown", d, cool, L ** 3 * 500 * len(cool)))
+        else:
+            d = vg.chain(vg.lattice(vg.SC, 64, 64, 64), vg.HEISENBERG, vg.RANDOM, heis_h, k)
+            jobs.append(("heis-hysteresis", d, loop, 64 ** 3 * 1000 * len(loop)))
+    return jobs
+
+
+def run_served(vg, ctx, n_jobs, rates, gen_seed=2025):
+    """Poisson arrivals at each rate: one thread submits each job at its
+    arrival time; latency = finish - arrival.  Throughput = attempts of all
+    jobs / (last finish - first arrival)."""
+    import numpy as np
+    jobs = served_jobs(vg, n_jobs, gen_seed)
+    out = []
+    for rate in rates:
+        arr_rng = np.random.default_rng(gen_seed + int(rate))
+        arrivals = np.cumsum(arr_rng.exponential(1.0 / rate, size=len(jobs)))
+        log(f"served: {len(jobs)} jobs at {rate}/s")
+        with vg.Server(ctx, memory_bytes=16 << 30, max_jobs=1024, quantum_sweeps=100,
+                       tile=2) as srv:
+            t0 = time.perf_counter()
+            ids, sub_t = [], []
+            for (kind, d, stages, _), ta in zip(jobs, arrivals):
+                delay = t0 + ta - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(d, stages))
+                sub_t.append(time.perf_counter() - t0)
+            ends = []
+            for jid in ids:
+                st = srv.wait(jid, 1200.0)
+                if st.state != vg.JOB_DONE:
+                    raise RuntimeError(f"job {jid} ended in state {st.state}")
+                ends.append(st.end_s)
+            stats = srv.stats()
+            # server clock: seconds since its start, which precedes t0 slightly
+            first = srv.status(ids[0])
+            offset = first.submit_s - sub_t[0]
+            lat = np.array([e - (ta + offset) for e, ta in zip(ends, arrivals)])
+            makespan = max(ends) - (arrivals[0] + offset)
+        attempts = sum(j[3] for j in jobs)
+        kinds = {}
+        for (kind, *_), l in zip(jobs, lat):
+            kinds.setdefault(kind, []).append(l)
+        res = {
+            "rate_per_s": rate,
+            "jobs": len(jobs),
+            "attempts": attempts,
+            "throughput": attempts / makespan,
+            "makespan_s": makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "p50_by_kind_s": {k: float(np.percentile(v, 50)) for k, v in kinds.items()},
+            "p99_by_kind_s": {k: float(np.percentile(v, 99)) for k, v in kinds.items()},
+            "peak_jobs": stats.peak_jobs,
+            "pool_peak_bytes": stats.pool_peak,
+            "device_busy_frac": stats.device_ms / 1e3 / makespan,
+        }
+        log(f"served: {rate}/s -> {res['throughput']:.3e} attempts/s, p50 "
+            f"{res['p50_latency_s']:.3f} s, p99 {res['p99_latency_s']:.3f} s, peak "
+            f"{stats.peak_jobs} jobs")
+        out.append(res)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,8 +297,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-sweeps", type=int, default=100)
     ap.add_argument("--tile", type=int, default=0, help="Ising tile; 0 = autotune")
-    ap.add_argument("--workload", default="all", choices=["all", "headline", "batched"],
-                    help="headline: sc 256^3 x 3 only; batched: the 512-chain batch only")
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "headline", "batched", "served", "heisenberg"],
+                    help="headline: sc 256^3 x 3 only; batched: the 512-chain batch only; "
+                         "served: the Poisson job stream only")
+    ap.add_argument("--serve-jobs", type=int, default=2000)
+    ap.add_argument("--heis-side", type=int, default=128)
+    ap.add_argument("--heis-sweeps", type=int, default=500)
+    ap.add_argument("--serve-rates", default="50,500")
     ap.add_argument("--batched-sweeps", type=int, default=10000)
     ap.add_argument("--baseline-sweeps", type=int, default=100)
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -186,6 +329,16 @@ def main():
         with vg.Context(0) as ctx:
             print(json.dumps(run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps)),
                   flush=True)
+        return
+    if args.workload == "heisenberg":
+        with vg.Context(0) as ctx:
+            print(json.dumps(run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)),
+                  flush=True)
+        return
+    if args.workload == "served":
+        rates = [float(r) for r in args.serve_rates.split(",")]
+        with vg.Context(0) as ctx:
+            print(json.dumps(run_served(vg, ctx, args.serve_jobs, rates)), flush=True)
         return
     temps = [3.5, 4.5115, 6.0]
     side = args.side

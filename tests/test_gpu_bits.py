@@ -1,0 +1,111 @@
+"""Bit-packed Ising chains (1 bit per site): the same trajectories as the
+int8 chains and the CPU oracle, bit for bit, over sizes whose rows end inside
+a 16-position segment, open and periodic axes, both proposals and fields."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # dims, pbc, T, h, proposal, terms
+    ((64, 64, 64), (1, 1, 1), 4.5115, 0.0, "flip", ("exchange",)),
+    ((34, 6, 4), (1, 1, 1), 3.0, 0.2, "flip", ("exchange", "zeeman")),
+    ((10, 10, 1), (1, 1, 0), 2.2, 0.0, "flip", ("exchange", "zeeman")),
+    ((2, 4, 6), (1, 1, 1), 3.0, 0.0, "flip", ("exchange",)),
+    ((40, 6, 4), (0, 1, 1), 3.5, 0.1, "random", ("exchange", "zeeman")),
+    ((66, 3, 5), (0, 0, 0), 2.5, 0.0, "flip", ("exchange",)),
+    ((48, 16, 10), (1, 1, 1), 1.0, 0.0, "flip", ("exchange",)),
+    ((16, 16, 16), (1, 1, 1), 4.0, -0.3, "random", ("exchange", "zeeman")),
+]
+
+
+def _desc(vg, dims, pbc, spin, proposal, terms, seed):
+    L = vg.lattice(vg.SC, *dims, pbc=tuple(bool(p) for p in pbc))
+    kw = {"zeeman": "zeeman" in terms}
+    if "exchange" in terms:
+        kw["exchange"] = 1.0
+    return vg.chain(L, spin, vg.FLIP if proposal == "flip" else vg.RANDOM,
+                    vg.hamiltonian(**kw), seed)
+
+
+@pytest.mark.parametrize("dims,pbc,T,h,proposal,terms", CASES)
+def test_bits_match_oracle_and_int8(vg, orc, ctx, dims, pbc, T, h, proposal, terms):
+    hd = (0.0, 0.0, 1.0 if h >= 0 else -1.0)
+    OL = orc.lattice("sc", *dims, pbc=pbc)
+    OM = orc.model(spin="ising", proposal=proposal, terms=terms, temperature=T, field=abs(h),
+                   field_dir=hd)
+    spins = orc.ising_init(OL, 31)
+    descs = [_desc(vg, dims, pbc, vg.ISING_BITS, proposal, terms, 31),
+             _desc(vg, dims, pbc, vg.ISING, proposal, terms, 31)]
+    with vg.Batch(ctx, descs) as b:
+        b.randomize()
+        assert np.array_equal(b.get_ising(0), spins)
+        for i in range(2):
+            b.set_thermostat(i, T, abs(h), hd)
+        for k in range(2):
+            got = b.run_sums(1)
+            want = orc.ising_sweeps(OL, OM, 31, k, 1, spins)
+            assert np.array_equal(got[0], want) and np.array_equal(got[1], want), k
+            assert np.array_equal(b.get_ising(0), spins), k
+        got = b.run_sums(10)
+        want = orc.ising_sweeps(OL, OM, 31, 2, 10, spins)
+        assert np.array_equal(got[0], want) and np.array_equal(got[1], want)
+        assert np.array_equal(b.get_ising(0), spins)
+        assert np.array_equal(b.get_ising(1), spins)
+
+
+def test_bits_set_get_tiles_graph(vg, ctx):
+    d = _desc(vg, (34, 6, 13), (1, 1, 0), vg.ISING_BITS, "flip", ("exchange", "zeeman"), 5)
+    rng = np.random.default_rng(3)
+    s = rng.choice(np.array([-1, 1], dtype=np.int8), size=34 * 6 * 13)
+    runs = []
+    for tile, graph in [(1, False), (2, True), (4, False), (8, True)]:
+        with vg.Batch(ctx, [d]) as b:
+            b.set_ising(0, s)
+            assert np.array_equal(b.get_ising(0), s)
+            b.set_tile(tile)
+            b.set_graph(graph)
+            b.set_thermostat(0, 3.0, 0.1)
+            runs.append((b.run_sums(7), b.get_ising(0)))
+    for r in runs[1:]:
+        assert np.array_equal(r[0], runs[0][0]) and np.array_equal(r[1], runs[0][1])
+
+
+def test_autotune_restores_every_group(vg, orc, ctx):
+    """Tuning trials run every chain of the batch; all kernel groups' spins
+    and counters come back, so trajectories are unchanged."""
+    H = vg.hamiltonian(exchange=1.0, zeeman=True)
+    Hh = vg.hamiltonian(exchange=1.0, zeeman=True, anisotropy=0.1)
+    descs = [vg.chain(vg.lattice(vg.SC, 32, 32, 32), vg.ISING, vg.FLIP, H, 1),
+             vg.chain(vg.lattice(vg.SC, 32, 32, 32), vg.ISING_BITS, vg.FLIP, H, 2),
+             vg.chain(vg.lattice(vg.SC, 8, 8, 8), vg.HEISENBERG, vg.RANDOM, Hh, 3),
+             vg.chain(vg.lattice(vg.BCC, 6, 6, 6), vg.ISING, vg.FLIP, H, 4)]
+    runs = []
+    for tune in (False, True):
+        with vg.Batch(ctx, descs) as b:
+            b.randomize()
+            for i in range(4):
+                b.set_thermostat(i, 3.0 if i != 2 else 1.2, 0.05)
+            b.run(3)
+            if tune:
+                assert b.autotune(3) in (1, 2, 4, 8)
+            runs.append((b.run_sums(5), b.get_ising(0), b.get_ising(1), b.get_heisenberg(2),
+                         b.get_ising(3), [b.sweep_count(i) for i in range(4)]))
+    for a, c in zip(runs[0], runs[1]):
+        assert np.array_equal(np.asarray(a), np.asarray(c))
+    assert runs[1][-1] == [8, 8, 8, 8]
+
+
+def test_bits_memory_is_an_eighth(vg, ctx):
+    """sc 256^3: the bit-packed chain runs the same sweeps as the int8 one
+    (identical sums) from planes one eighth the size."""
+    L = vg.lattice(vg.SC, 256, 256, 256)
+    H = vg.hamiltonian(exchange=1.0)
+    with vg.Batch(ctx, [vg.chain(L, vg.ISING_BITS, vg.FLIP, H, 7),
+                        vg.chain(L, vg.ISING, vg.FLIP, H, 7)]) as b:
+        b.randomize()
+        for i in range(2):
+            b.set_thermostat(i, 4.5115)
+        s = b.run_sums(3)
+        assert np.array_equal(s[0], s[1])
+        assert np.array_equal(b.get_ising(0), b.get_ising(1))

@@ -43,11 +43,10 @@ int validate_desc(const vg_chain_desc& d) {
     const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
     if ((uint64_t)L.ly * L.lz > 0xFFFFFFFFull || cells > (1ull << 34))
         return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
-    if (d.spin != VG_SPIN_ISING && d.spin != VG_SPIN_HEISENBERG) {
-        if (d.spin == VG_SPIN_ISING_BITS)
-            return fail(VG_ERR_NOT_IMPLEMENTED, "bit-packed Ising chains are not implemented yet");
+    if (d.spin != VG_SPIN_ISING && d.spin != VG_SPIN_HEISENBERG && d.spin != VG_SPIN_ISING_BITS)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown spin kind %d", d.spin);
-    }
+    if (d.spin == VG_SPIN_ISING_BITS && L.kind != VG_LATTICE_SC)
+        return fail(VG_ERR_NOT_IMPLEMENTED, "bit-packed Ising chains run on sc lattices");
     if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown proposal %d", d.proposal);
     const vg_hamiltonian& H = d.hamiltonian;
@@ -160,7 +159,10 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
     d.limit = 0xFFFFFFFFu;
     const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
     const uint32_t align = is_ising(g) ? 64u : 16u;  // 64-byte aligned rows
-    d.pitch = (uint32_t)round_up(d.w, align);
+    if (g == kIsingBits)  // bytes: whole 16-position segments, 16-byte aligned rows
+        d.pitch = (uint32_t)round_up((d.w + 15) / 16 * 2, 16);
+    else
+        d.pitch = (uint32_t)round_up(d.w, align);
     d.segs = (d.w + seg - 1) / seg;
     d.hy = (d.ly + 1) / 2;
     if ((uint64_t)d.rows * d.segs > 0x7FFFFFFFull - vg::kBlock || (uint64_t)d.rows > 0x7FFFFFFFull)
@@ -185,6 +187,16 @@ uint64_t colour_elems(int g, const ChainDev& d) { return (is_ising(g) ? 1 : 3) *
 uint32_t quantities(int group) { return is_ising(group) ? 8u : 5u; }
 uint32_t quantity_bytes(int group) { return is_ising(group) ? 4u : 8u; }
 
+void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev* dp,
+                       hipStream_t st) {
+    const uint64_t n = 2ull * d.plane;  // one thread per plane byte
+    const dim3 grid((uint32_t)((n + 255) / 256));
+    if (g == kIsingBits)
+        hipLaunchKernelGGL(vg::ising_init_bits, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
+    else
+        hipLaunchKernelGGL(vg::ising_init, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
+}
+
 int upload_descriptors(vg_batch* B, int g) {
     Group& G = B->groups[g];
     if (G.members.empty()) return VG_OK;
@@ -205,7 +217,7 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
     for (size_t k = 0; k < G.members.size(); ++k) {
         ChainDev& d = G.host[k];
         uint64_t items;
-        if (g == kIsingSc) {
+        if (is_zmarch(g)) {
             d.nzc = (d.lz + tile - 1) / tile;
             items = (uint64_t)d.nzc * 2 * d.hy * d.segs;
         } else {
@@ -255,25 +267,26 @@ int free_batch(vg_batch* B) {
     return VG_OK;
 }
 
+template <bool BITS>
 void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     const dim3 grid(G.grid), block(vg::kBlock);
     uint8_t* arena = (uint8_t*)G.arena;
     int32_t* part = (int32_t*)G.partials;
     switch (G.tile) {
         case 1:
-            hipLaunchKernelGGL(vg::ising_sc_pass<1>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_sc_pass<1, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
         case 2:
-            hipLaunchKernelGGL(vg::ising_sc_pass<2>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_sc_pass<2, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
         case 8:
-            hipLaunchKernelGGL(vg::ising_sc_pass<8>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_sc_pass<8, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
         default:
-            hipLaunchKernelGGL(vg::ising_sc_pass<4>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_sc_pass<4, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
     }
@@ -299,7 +312,9 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                 }
                 const dim3 grid(G.grid), block(vg::kBlock);
                 if (g == kIsingSc)
-                    launch_ising_pass(G, st, c, s);
+                    launch_ising_pass<false>(G, st, c, s);
+                else if (g == kIsingBits)
+                    launch_ising_pass<true>(G, st, c, s);
                 else if (g == kIsingBcc)
                     hipLaunchKernelGGL(vg::ising_bcc_pass, grid, block, 0, st, G.d_chains,
                                        G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials,
@@ -625,7 +640,7 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             free_batch(B);
             return fail(VG_ERR_OUT_OF_MEMORY, "allocating partial sums: %s", hipGetErrorString(e));
         }
-        if (g == kIsingSc) {
+        if (is_zmarch(g)) {
             int rc = layout_work(B, g, kDefaultTile);
             if (rc) {
                 free_batch(B);
@@ -711,9 +726,7 @@ int vg_batch_randomize(vg_batch* batch) {
         const ChainDev& d = G.host[ch.index];
         const ChainDev* dp = G.d_chains + ch.index;
         if (is_ising(ch.group)) {
-            const uint64_t n = 2ull * d.plane;
-            hipLaunchKernelGGL(vg::ising_init, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
-                               dp, (uint8_t*)G.arena);
+            launch_ising_init(ch.group, G, d, dp, st);
         } else {
             const uint64_t n = 2ull * d.rows * d.w;
             hipLaunchKernelGGL(vg::heis_init, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
@@ -745,9 +758,15 @@ int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
     VG_HIP(hipMalloc(&tmp, ch.n_sites));
     hipError_t e = hipMemcpyAsync(tmp, spins, ch.n_sites, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(vg::ising_scatter, dim3((uint32_t)((ch.n_sites + 255) / 256)),
-                           dim3(256), 0, st, G.d_chains + ch.index, (uint8_t*)G.arena, tmp,
-                           ch.n_sites);
+        if (ch.group == kIsingBits) {
+            const uint64_t nb = 2ull * G.host[ch.index].plane;
+            hipLaunchKernelGGL(vg::ising_scatter_bits, dim3((uint32_t)((nb + 255) / 256)),
+                               dim3(256), 0, st, G.d_chains + ch.index, (uint8_t*)G.arena, tmp);
+        } else {
+            hipLaunchKernelGGL(vg::ising_scatter, dim3((uint32_t)((ch.n_sites + 255) / 256)),
+                               dim3(256), 0, st, G.d_chains + ch.index, (uint8_t*)G.arena, tmp,
+                               ch.n_sites);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -769,8 +788,14 @@ int vg_batch_get_ising(vg_batch* batch, uint32_t chain, int8_t* spins) {
     if (rc) return rc;
     int8_t* tmp = nullptr;
     VG_HIP(hipMalloc(&tmp, ch.n_sites));
-    hipLaunchKernelGGL(vg::ising_gather, dim3((uint32_t)((ch.n_sites + 255) / 256)), dim3(256), 0,
-                       st, G.d_chains + ch.index, (const uint8_t*)G.arena, tmp, ch.n_sites);
+    if (ch.group == kIsingBits)
+        hipLaunchKernelGGL(vg::ising_gather_bits, dim3((uint32_t)((ch.n_sites + 255) / 256)),
+                           dim3(256), 0, st, G.d_chains + ch.index, (const uint8_t*)G.arena, tmp,
+                           ch.n_sites);
+    else
+        hipLaunchKernelGGL(vg::ising_gather, dim3((uint32_t)((ch.n_sites + 255) / 256)),
+                           dim3(256), 0, st, G.d_chains + ch.index, (const uint8_t*)G.arena, tmp,
+                           ch.n_sites);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(spins, tmp, ch.n_sites, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -889,64 +914,96 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
     if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
     if (tile != 1 && tile != 2 && tile != 4 && tile != 8)
         return fail(VG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8 (got %u)", tile);
-    if (batch->groups[kIsingSc].members.empty()) return VG_OK;
     VG_HIP(hipSetDevice(batch->ctx->device));
-    return layout_work(batch, kIsingSc, tile);
+    for (int g : {kIsingSc, kIsingBits}) {
+        if (batch->groups[g].members.empty()) continue;
+        int rc = layout_work(batch, g, tile);
+        if (rc) return rc;
+    }
+    return VG_OK;
 }
 
 int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile) {
     if (!batch || !tile) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
-    *tile = batch->groups[kIsingSc].tile;
+    const int g = batch->groups[kIsingSc].members.empty() ? kIsingBits : kIsingSc;
+    *tile = batch->groups[g].tile;
     return VG_OK;
 }
 
 int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) {
     if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
-    Group& G = batch->groups[kIsingSc];
-    if (G.members.empty() || trial_sweeps == 0) {
-        if (chosen) *chosen = G.tile;
+    uint32_t tile_now = 0;
+    vg_batch_get_tile(batch, &tile_now);
+    const bool any = !batch->groups[kIsingSc].members.empty() ||
+                     !batch->groups[kIsingBits].members.empty();
+    if (!any || trial_sweeps == 0) {
+        if (chosen) *chosen = tile_now;
         return VG_OK;
     }
     VG_HIP(hipSetDevice(batch->ctx->device));
     hipStream_t st = batch->ctx->stream;
-    // The trials advance the chains; the arena and the sweep counters are
-    // restored after each, so tuning leaves every trajectory unchanged.
-    void* backup = nullptr;
-    VG_HIP(hipMalloc(&backup, G.arena_bytes));
+    // The trials advance every chain of the batch; all arenas and sweep
+    // counters are restored after each, so tuning changes no trajectory.
+    std::vector<void*> backup(kNumGroups, nullptr);
+    auto free_backup = [&]() {
+        for (void*& p : backup)
+            if (p) quiet(hipFree(p));
+    };
+    for (int g = 0; g < kNumGroups; ++g) {
+        Group& G = batch->groups[g];
+        if (G.members.empty()) continue;
+        hipError_t e = hipMalloc(&backup[g], G.arena_bytes);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(backup[g], G.arena, G.arena_bytes, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) {
+            free_backup();
+            return fail(VG_ERR_OUT_OF_MEMORY, "autotune backup: %s", hipGetErrorString(e));
+        }
+    }
     std::vector<uint64_t> counters(batch->chains.size());
     for (size_t i = 0; i < batch->chains.size(); ++i) counters[i] = batch->chains[i].sweeps;
     auto restore = [&]() -> int {
-        VG_HIP(hipMemcpyAsync(G.arena, backup, G.arena_bytes, hipMemcpyDeviceToDevice, st));
+        for (int g = 0; g < kNumGroups; ++g) {
+            Group& G = batch->groups[g];
+            if (G.members.empty()) continue;
+            VG_HIP(hipMemcpyAsync(G.arena, backup[g], G.arena_bytes, hipMemcpyDeviceToDevice, st));
+        }
         VG_HIP(hipStreamSynchronize(st));
         for (size_t i = 0; i < batch->chains.size(); ++i) batch->chains[i].sweeps = counters[i];
         return VG_OK;
     };
     int rc = VG_OK;
-    hipError_t e = hipMemcpyAsync(backup, G.arena, G.arena_bytes, hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(VG_ERR_DEVICE, "autotune backup: %s", hipGetErrorString(e));
-    const uint32_t start_tile = G.tile;
-    uint32_t best = start_tile;
-    double best_ms = 1e300;
-    const uint32_t tiles[4] = {1, 2, 4, 8};
-    for (uint32_t ti = 0; ti < 4 && rc == VG_OK; ++ti) {
-        rc = layout_work(batch, kIsingSc, tiles[ti]);
-        if (rc == VG_OK) rc = run_impl(batch, 2, nullptr, nullptr, nullptr);
-        if (rc == VG_OK) rc = restore();
-        if (rc == VG_OK) rc = run_impl(batch, trial_sweeps, nullptr, nullptr, nullptr);
-        if (rc == VG_OK && batch->last_ms < best_ms) {
-            best_ms = batch->last_ms;
-            best = tiles[ti];
+    for (int g : {kIsingSc, kIsingBits}) {
+        Group& G = batch->groups[g];
+        if (G.members.empty() || rc != VG_OK) continue;
+        const uint32_t start_tile = G.tile;
+        uint32_t best = start_tile;
+        double best_ms = 1e300;
+        const uint32_t tiles[4] = {1, 2, 4, 8};
+        for (uint32_t ti = 0; ti < 4 && rc == VG_OK; ++ti) {
+            rc = layout_work(batch, g, tiles[ti]);
+            if (rc == VG_OK) rc = run_impl(batch, 2, nullptr, nullptr, nullptr);
+            if (rc == VG_OK) rc = restore();
+            if (rc == VG_OK) rc = run_impl(batch, trial_sweeps, nullptr, nullptr, nullptr);
+            if (rc == VG_OK && batch->last_ms < best_ms) {
+                best_ms = batch->last_ms;
+                best = tiles[ti];
+            }
+            if (rc == VG_OK) rc = restore();
         }
-        if (rc == VG_OK) rc = restore();
+        if (rc != VG_OK) {
+            layout_work(batch, g, start_tile);
+            break;
+        }
+        rc = layout_work(batch, g, best);
     }
-    quiet(hipFree(backup));
-    if (rc != VG_OK) {
-        layout_work(batch, kIsingSc, start_tile);
-        return rc;
+    free_backup();
+    if (rc == VG_OK && chosen) {
+        const int g = batch->groups[kIsingSc].members.empty() ? kIsingBits : kIsingSc;
+        *chosen = batch->groups[g].tile;
     }
-    rc = layout_work(batch, kIsingSc, best);
-    if (chosen) *chosen = best;
     return rc;
 }
 

@@ -18,9 +18,12 @@
 #include "vg_device.hpp"
 
 namespace vg {
-template <uint32_t K>
+template <uint32_t K, bool BITS>
 __global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
                               uint32_t, uint32_t, uint32_t);
+__global__ void ising_init_bits(const ChainDev*, uint8_t*);
+__global__ void ising_scatter_bits(const ChainDev*, uint8_t*, const int8_t*);
+__global__ void ising_gather_bits(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
 __global__ void ising_reduce(const ChainDev*, const int32_t*, uint32_t, uint32_t, int64_t*);
 __global__ void ising_init(const ChainDev*, uint8_t*);
 __global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t);
@@ -63,13 +66,23 @@ inline uint64_t round_up(uint64_t v, uint64_t m) { return (v + m - 1) / m * m; }
 inline void quiet(hipError_t) {}
 
 // Kernel groups: chains of one group share an arena and one pass kernel.
-enum GroupKind { kIsingSc = 0, kHeisSc = 1, kIsingBcc = 2, kHeisBcc = 3, kNumGroups = 4 };
+enum GroupKind {
+    kIsingSc = 0,
+    kHeisSc = 1,
+    kIsingBcc = 2,
+    kHeisBcc = 3,
+    kIsingBits = 4,  // sc, 1 bit per site
+    kNumGroups = 5
+};
 
-inline bool is_ising(int g) { return g == kIsingSc || g == kIsingBcc; }
+inline bool is_ising(int g) { return g == kIsingSc || g == kIsingBcc || g == kIsingBits; }
+// Groups whose pass marches a tile of z planes per work item.
+inline bool is_zmarch(int g) { return g == kIsingSc || g == kIsingBits; }
 
 inline int group_of(const vg_chain_desc& d) {
     const bool bcc = d.lattice.kind == VG_LATTICE_BCC;
     if (d.spin == VG_SPIN_HEISENBERG) return bcc ? kHeisBcc : kHeisSc;
+    if (d.spin == VG_SPIN_ISING_BITS) return kIsingBits;
     return bcc ? kIsingBcc : kIsingSc;
 }
 
@@ -141,6 +154,9 @@ void refresh_params(vg_batch* B, uint32_t i);
 void ising_thresholds(const ChainHost& ch, ChainDev& d);
 void heis_params(const ChainHost& ch, ChainDev& d);
 int free_batch(vg_batch* B);
+// Random initial state (State::rand_with_size) of an Ising chain of group g.
+void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev* dp,
+                       hipStream_t st);
 uint32_t quantities(int group);
 uint32_t quantity_bytes(int group);
 int upload_descriptors(vg_batch* B, int g);

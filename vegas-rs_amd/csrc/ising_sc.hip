@@ -71,7 +71,43 @@ __device__ __forceinline__ bool tie_accept(const uint32_t* lut, uint32_t always_
     return (u4_word(rb, pos & 3u) >> 8) < (lut[bj * 17u + S] & 0xFFFFFFu);
 }
 
-template <uint32_t K>
+// The two storage forms of a colour plane.  Int8: one byte per position,
+// row segments of 16 positions are 16-byte loads.  Bit-packed: bit (i & 7)
+// of byte i >> 3 of the row, a segment is one 16-bit load expanded to the
+// same 16 bytes in registers, so everything after the loads is shared.
+template <bool BITS>
+__device__ __forceinline__ uint32_t seg_off(uint32_t i) {
+    return BITS ? i >> 3 : i;
+}
+
+template <bool BITS>
+__device__ __forceinline__ u32x4 load_seg(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if (!BITS) return bload16(r, off);
+    const uint32_t v = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    u32x4 e;
+    e.x = nibble_bytes(v);
+    e.y = nibble_bytes(v >> 4);
+    e.z = nibble_bytes(v >> 8);
+    e.w = nibble_bytes(v >> 12);
+    return e;
+}
+
+// Spin bit of position i of the row starting at byte offset roff (kNoRow:
+// absent, reads 0).
+template <bool BITS>
+__device__ __forceinline__ uint32_t load_pos(__amdgpu_buffer_rsrc_t r, uint32_t roff, uint32_t i,
+                                             bool present) {
+    const uint32_t v = bload8(r, present ? roff + seg_off<BITS>(i) : kNoRow);
+    return BITS ? (v >> (i & 7u)) & 1u : v;
+}
+
+// 16 bytes of 0/1 -> 16 bits (byte j -> bit j).
+__device__ __forceinline__ uint32_t pack16(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return (((a & kOne) * 0x10204080u) >> 28) | ((((b & kOne) * 0x10204080u) >> 28) << 4) |
+           ((((c & kOne) * 0x10204080u) >> 28) << 8) | ((((d & kOne) * 0x10204080u) >> 28) << 12);
+}
+
+template <uint32_t K, bool BITS>
 __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict__ chains,
                                                      const uint32_t* __restrict__ block_chain,
                                                      uint8_t* __restrict__ arena,
@@ -136,11 +172,12 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         const uint32_t xl_pos = i0 > 0 ? i0 - 1 : w - 1;
         const uint32_t pw = w - i0;  // positions left in the row, 1..16 in the last segment
         const bool xr_in = i0 + kIsingSeg < w;
-        uint32_t off = (y + ly * z0) * pitch + i0;  // row (y, z0), this segment
+        uint32_t off = (y + ly * z0) * pitch + seg_off<BITS>(i0);  // row (y, z0), this segment
         // the row window along z: row (y, z-1) and row (y, z)
         const bool zm0_ok = z0 > 0 || pz;
-        u32x4 below = bload16(ro, !zm0_ok ? kNoRow : (z0 > 0 ? off - zstep : off + (lz - 1) * zstep));
-        u32x4 here = bload16(ro, off);
+        u32x4 below =
+            load_seg<BITS>(ro, !zm0_ok ? kNoRow : (z0 > 0 ? off - zstep : off + (lz - 1) * zstep));
+        u32x4 here = load_seg<BITS>(ro, off);
         bool below_ok = zm0_ok;
 
 #pragma unroll
@@ -149,11 +186,11 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
             const bool zp_ok = z + 1 < lz || pz;
-            const u32x4 o4 = bload16(rw_own, off);
-            const u32x4 ym4 = bload16(ro, ym_ok ? off + dym : kNoRow);
-            const u32x4 yp4 = bload16(ro, yp_ok ? off + dyp : kNoRow);
-            const u32x4 above =
-                bload16(ro, !zp_ok ? kNoRow : (z + 1 < lz ? off + zstep : off - (lz - 1) * zstep));
+            const u32x4 o4 = load_seg<BITS>(rw_own, off);
+            const u32x4 ym4 = load_seg<BITS>(ro, ym_ok ? off + dym : kNoRow);
+            const u32x4 yp4 = load_seg<BITS>(ro, yp_ok ? off + dyp : kNoRow);
+            const u32x4 above = load_seg<BITS>(
+                ro, !zp_ok ? kNoRow : (z + 1 < lz ? off + zstep : off - (lz - 1) * zstep));
             const uint32_t g = gy + (below_ok ? 0u : 1u) + (zp_ok ? 0u : 1u);
 
             // x neighbours of position i: i-1,i (q = 0) or i,i+1 (q = 1) in
@@ -161,19 +198,19 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             // position i0-1 (q = 0), or i0+16, or position 0 across the
             // periodic end (q = 1); kNoRow reads zero when x is open.
             const uint32_t q = (colour + y + z) & 1u;
-            const uint32_t orow = off - i0;
+            const uint32_t orow = off - seg_off<BITS>(i0);
             uint32_t m[4] = {here.x, here.y, here.z, here.w};
             uint32_t xs[4];
             uint32_t gx = 0xFFFFFFFFu;  // position whose x neighbour is absent
             if (q == 0) {
-                const uint32_t extra = bload8(ro, xl_ok ? orow + xl_pos : kNoRow);
+                const uint32_t extra = load_pos<BITS>(ro, orow, xl_pos, xl_ok);
                 if (!xl_ok) gx = 0;
                 xs[0] = __builtin_amdgcn_alignbyte(m[0], extra << 24, 3);
                 xs[1] = __builtin_amdgcn_alignbyte(m[1], m[0], 3);
                 xs[2] = __builtin_amdgcn_alignbyte(m[2], m[1], 3);
                 xs[3] = __builtin_amdgcn_alignbyte(m[3], m[2], 3);
             } else {
-                uint32_t extra = bload8(ro, xr_in ? off + kIsingSeg : (px ? orow : kNoRow));
+                uint32_t extra = load_pos<BITS>(ro, orow, xr_in ? i0 + kIsingSeg : 0, xr_in || px);
                 if (!xr_in) {
                     // Position w-1 needs position w: position 0 across the
                     // periodic boundary, or nothing when x is open.
@@ -256,12 +293,17 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                 upc += __popc(n[kk]);
             }
             if (any) {
-                u32x4 nv;
-                nv.x = n[0];
-                nv.y = n[1];
-                nv.z = n[2];
-                nv.w = n[3];
-                __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+                if (BITS) {
+                    __builtin_amdgcn_raw_buffer_store_b16(
+                        (unsigned short)pack16(n[0], n[1], n[2], n[3]), rw_own, off, 0, 0);
+                } else {
+                    u32x4 nv;
+                    nv.x = n[0];
+                    nv.y = n[1];
+                    nv.z = n[2];
+                    nv.w = n[3];
+                    __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+                }
             }
             m_part += 2 * upc - (int32_t)nvalid;
             if (colour == 1) {
@@ -299,10 +341,13 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                         rec[0] = lane | (k << 6) | (g << 9) | (min(gx, 31u) << 12);
                         rec[1] = tmask;
                         *reinterpret_cast<uint4*>(rec + 2) = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
-                        *reinterpret_cast<uint4*>(rec + 6) = make_uint4(o[0], o[1], o[2], o[3]);
+                        // own bytes after the untied flips (tied ones are
+                        // unchanged); packed, they are the row's stored bits
+                        *reinterpret_cast<uint4*>(rec + 6) = make_uint4(n[0], n[1], n[2], n[3]);
                     } else {
                         // Queue full (practically never): resolve in this lane.
                         uint32_t tm = tmask;
+                        uint32_t flips = 0;
                         while (tm) {
                             const uint32_t j = __builtin_ctz(tm);
                             tm &= tm - 1u;
@@ -312,12 +357,17 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                             const uint32_t S = 2u * c + g + (j == gx ? 1u : 0u);
                             if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t,
                                            colour, k0, k1)) {
-                                own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                if (!BITS) own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                flips |= 1u << j;
                                 const int32_t ds = 2 - 4 * (int32_t)bj;
                                 m_part += ds;
                                 if (colour == 1) b_part += ds * ((int32_t)S - 6);
                             }
                         }
+                        if (BITS && flips)
+                            __builtin_amdgcn_raw_buffer_store_b16(
+                                (unsigned short)(pack16(n[0], n[1], n[2], n[3]) ^ flips), rw_own,
+                                off, 0, 0);
                     }
                 }
                 nrec += __builtin_popcountll(has);
@@ -349,6 +399,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t zz = (r2 >> 1) * K + k;
             const uint32_t row = yy + ly * zz;
             uint32_t tm = rec[1];
+            uint32_t flips = 0;
             while (tm) {
                 const uint32_t j = __builtin_ctz(tm);
                 tm &= tm - 1u;
@@ -358,12 +409,19 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                 const uint32_t S = 2u * c + g + (j == gx ? 1u : 0u);
                 const uint32_t pos = sg * kIsingSeg + j;
                 if (tie_accept(lut, bj ? always1 : always0, bj, S, pos, row, t, colour, k0, k1)) {
-                    own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    if (!BITS) own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    flips |= 1u << j;
                     const int32_t ds = 2 - 4 * (int32_t)bj;
                     m_part += ds;
                     if (colour == 1) b_part += ds * ((int32_t)S - 6);
                 }
             }
+            // Bit-packed: the owner stored the row's 16 bits with the tied
+            // sites unchanged; this lane rewrites them with the accepted ones.
+            if (BITS && flips)
+                __builtin_amdgcn_raw_buffer_store_b16(
+                    (unsigned short)(pack16(rec[6], rec[7], rec[8], rec[9]) ^ flips), rw_own,
+                    row * pitch + seg_off<BITS>(sg * kIsingSeg), 0, 0);
         }
     }
 
@@ -377,14 +435,30 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     }
 }
 
-template __global__ void ising_sc_pass<1>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                          uint32_t, uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<2>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                          uint32_t, uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<4>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                          uint32_t, uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<8>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                          uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<1, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
+template __global__ void ising_sc_pass<2, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
+template __global__ void ising_sc_pass<4, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
+template __global__ void ising_sc_pass<8, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
+template __global__ void ising_sc_pass<1, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
+template __global__ void ising_sc_pass<2, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
+template __global__ void ising_sc_pass<4, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
+template __global__ void ising_sc_pass<8, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                  int32_t*, uint32_t, uint32_t, uint32_t,
+                                                  uint32_t);
 
 // Sums the pass partials of each (chain, sweep) into its record:
 // record = {bond, 0, 0, mag, 0} (int64).  grid = (chains, sweeps).

@@ -73,6 +73,66 @@ __global__ void ising_gather(const ChainDev* __restrict__ chain, const uint8_t* 
     natural[n] = arena[C.off[p.c] + (uint64_t)p.row * C.pitch + p.i] ? 1 : -1;
 }
 
+// Bit-packed Ising planes (sc): position i of a row is bit (i & 7) of byte
+// i >> 3.  One thread per plane byte; bits past w stay zero.
+__device__ __forceinline__ uint64_t sc_site(const ChainDev& C, uint32_t c, uint32_t row, uint32_t i) {
+    const uint32_t y = row % C.ly, z = row / C.ly;
+    const uint32_t x = 2u * i + ((c + y + z) & 1u);
+    return (uint64_t)x + (uint64_t)C.lx * ((uint64_t)y + (uint64_t)C.ly * z);
+}
+
+// The same draws as ising_init: position i is Up iff word (i & 3) of
+// Philox(key = seed, ctr = (i >> 2, row, 0, w(0, init, c))) is below 2^31.
+__global__ void ising_init_bits(const ChainDev* __restrict__ chain, uint8_t* __restrict__ arena) {
+    const ChainDev& C = *chain;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t per_plane = (uint64_t)C.rows * C.pitch;
+    if (idx >= 2 * per_plane) return;
+    const uint32_t c = idx >= per_plane ? 1u : 0u;
+    const uint64_t e = idx - c * per_plane;
+    const uint32_t row = (uint32_t)(e / C.pitch);
+    const uint32_t i0 = 8u * (uint32_t)(e % C.pitch);
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t i = i0 + 4u * h;
+        if (i >= C.w) break;
+        const U4 r = philox(U4{i >> 2, row, 0u, ctr_w(0, kIsingInit, c)}, (uint32_t)C.seed,
+                            (uint32_t)(C.seed >> 32));
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+            if (i + k < C.w && (u4_word(r, k) >> 31) == 0) v |= 1u << (4u * h + k);
+    }
+    arena[C.off[c] + e] = (uint8_t)v;
+}
+
+__global__ void ising_scatter_bits(const ChainDev* __restrict__ chain, uint8_t* __restrict__ arena,
+                                   const int8_t* __restrict__ natural) {
+    const ChainDev& C = *chain;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t per_plane = (uint64_t)C.rows * C.pitch;
+    if (idx >= 2 * per_plane) return;
+    const uint32_t c = idx >= per_plane ? 1u : 0u;
+    const uint64_t e = idx - c * per_plane;
+    const uint32_t row = (uint32_t)(e / C.pitch);
+    const uint32_t i0 = 8u * (uint32_t)(e % C.pitch);
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < 8 && i0 + k < C.w; ++k)
+        if (natural[sc_site(C, c, row, i0 + k)] > 0) v |= 1u << k;
+    arena[C.off[c] + e] = (uint8_t)v;
+}
+
+__global__ void ising_gather_bits(const ChainDev* __restrict__ chain,
+                                  const uint8_t* __restrict__ arena, int8_t* __restrict__ natural,
+                                  uint64_t n_sites) {
+    const ChainDev& C = *chain;
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_sites) return;
+    const Where p = locate(C, n);
+    const uint8_t v = arena[C.off[p.c] + (uint64_t)p.row * C.pitch + (p.i >> 3)];
+    natural[n] = (v >> (p.i & 7u)) & 1u ? 1 : -1;
+}
+
 // Zero the padding of a Heisenberg chain (positions >= w) and, before any
 // state is set, everything.
 __global__ void heis_clear(const ChainDev* __restrict__ chain, float* __restrict__ arena) {

@@ -222,7 +222,7 @@ int rebuild(vg_server* S) {
             VG_HIP(hipHostMalloc(&G.h_staging, cap * sizeof(ChainDev)));
             G.chains_cap = cap;
         }
-        int rc = layout_work(&B, g, g == kIsingSc ? S->cfg.tile : 1);
+        int rc = layout_work(&B, g, is_zmarch(g) ? S->cfg.tile : 1);
         if (rc) return rc;
         // layout_work recorded each chain's blocks in G.host; keep them
         for (size_t k = 0; k < G.members.size(); ++k) S->running[G.members[k]]->dev = G.host[k];
@@ -280,9 +280,7 @@ int quantum(vg_server* S, uint32_t& nsw_out) {
         Group& G = B.groups[j->ch.group];
         const ChainDev* dp = G.d_chains + j->ch.index;
         if (is_ising(j->ch.group)) {
-            const uint64_t n = 2ull * j->dev.plane;
-            hipLaunchKernelGGL(vg::ising_init, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
-                               dp, (uint8_t*)G.arena);
+            launch_ising_init(j->ch.group, G, j->dev, dp, st);
         } else {
             const uint64_t n6 = 6ull * j->dev.plane, n2 = 2ull * j->dev.rows * j->dev.w;
             hipLaunchKernelGGL(vg::heis_clear, dim3((uint32_t)((n6 + 255) / 256)), dim3(256), 0, st,
@@ -469,8 +467,6 @@ void worker_main(vg_server* S) {
 int validate_job(const vg_job_desc& j) {
     int rc = validate_desc(j.chain);
     if (rc) return rc;
-    if (j.chain.spin == VG_SPIN_ISING_BITS)
-        return fail(VG_ERR_NOT_IMPLEMENTED, "the server runs int8 Ising and Heisenberg chains");
     if (j.n_stages == 0 || !j.stages) return fail(VG_ERR_NO_STEPS, "a job needs at least one stage");
     for (uint32_t k = 0; k < j.n_stages; ++k) {
         const vg_stage& s = j.stages[k];
