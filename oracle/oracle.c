@@ -298,34 +298,63 @@ void or_heis_init(const or_lattice* L, uint64_t seed, float* xyz) {
     }
 }
 
-typedef struct {
-    uint64_t seed;
-    uint32_t x, y, z, whi;
-    uint32_t call, used;
-    uint32_t words[4];
-} stream_t;
-
-static void stream_start(stream_t* st, uint64_t seed, uint32_t x, uint32_t y, uint64_t t, uint32_t colour) {
-    st->seed = seed;
-    st->x = x;
-    st->y = y;
-    st->z = (uint32_t)t;
-    st->whi = (uint32_t)((t >> 32) << 8) | colour;
-    st->call = 0;
-    st->used = 0;
-    draw(seed, x, y, st->z, st->whi | (16u << 1), st->words);
+/* 24-bit value m (0..4) of a Philox call: bits [24m, 24m+24) of the 128-bit
+ * string w0 | w1 << 32 | w2 << 64 | w3 << 96 (bits 120-127 unused). */
+static uint32_t v24(const uint32_t w[4], int m) {
+    const uint64_t lo = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+    const uint64_t hi = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+    uint64_t v;
+    switch (m) {
+        case 0: v = lo; break;
+        case 1: v = lo >> 24; break;
+        case 2: v = (lo >> 48) | (hi << 16); break;
+        case 3: v = hi >> 8; break;
+        default: v = hi >> 32; break;
+    }
+    return (uint32_t)(v & 0xFFFFFFu);
 }
 
-/* next word of the stream; 0 once 112 calls are used up */
-static int stream_next(stream_t* st, uint32_t* w) {
-    if (st->used == 4) {
-        if (st->call + 1 >= 112) return 0;
-        st->call++;
-        draw(st->seed, st->x, st->y, st->z, st->whi | ((16u + st->call) << 1), st->words);
-        st->used = 0;
-    }
-    *w = st->words[st->used++];
+static float f24u(uint32_t v) { return (float)v * 5.9604644775390625e-08f; }
+static float f24s(uint32_t v) { return (float)v * 1.1920928955078125e-07f - 1.0f; }
+
+/* Marsaglia try from two 24-bit values; 0 when the point misses the disc. */
+static int marsaglia24(uint32_t a, uint32_t b, float* out) {
+    float x1 = f24s(a), x2 = f24s(b);
+    float p = x1 * x1;
+    float q2 = x2 * x2;
+    float q = p + q2;
+    if (!(q < 1.0f)) return 0;
+    float root = sqrtf(1.0f - q);
+    out[0] = (2.0f * x1) * root;
+    out[1] = (2.0f * x2) * root;
+    out[2] = 1.0f - 2.0f * q;
     return 1;
+}
+
+/* Fresh uniform-sphere proposal of one site: Philox calls k = 0, 1, ...
+ * with counter (pos, row, sweep, purpose 16 + k); u = value 0 of call 0;
+ * tries (1,2), (3,4) of call 0, (0,1), (2,3) of call 1, then (0,1), (2,3)
+ * of calls 2..15; exhausted -> (0,0,1). */
+static float random_proposal(uint64_t seed, uint32_t pos, uint32_t row, uint64_t t, uint32_t c,
+                             float* p) {
+    uint32_t w0[4], w1[4];
+    draw(seed, pos, row, (uint32_t)t, wword(t, 16, c), w0);
+    draw(seed, pos, row, (uint32_t)t, wword(t, 17, c), w1);
+    const float u = f24u(v24(w0, 0));
+    if (marsaglia24(v24(w0, 1), v24(w0, 2), p)) return u;
+    if (marsaglia24(v24(w0, 3), v24(w0, 4), p)) return u;
+    if (marsaglia24(v24(w1, 0), v24(w1, 1), p)) return u;
+    if (marsaglia24(v24(w1, 2), v24(w1, 3), p)) return u;
+    for (uint32_t k = 2; k < 16; ++k) {
+        uint32_t w[4];
+        draw(seed, pos, row, (uint32_t)t, wword(t, 16 + k, c), w);
+        if (marsaglia24(v24(w, 0), v24(w, 1), p)) return u;
+        if (marsaglia24(v24(w, 2), v24(w, 3), p)) return u;
+    }
+    p[0] = 0.0f;
+    p[1] = 0.0f;
+    p[2] = 1.0f;
+    return u;
 }
 
 void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t sweep0,
@@ -346,9 +375,9 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
         for (uint32_t c = 0; c < 2; ++c) {
             for (uint32_t row = 0; row < rows; ++row) {
                 for (uint32_t g = 0; 4 * g < w; ++g) {
-                    stream_t st;
-                    stream_start(&st, seed, g, row, t, c);
-                    int live = 1;
+                    /* flip proposals: word (pos & 3) of one call per 4 positions */
+                    uint32_t fw[4];
+                    if (M->proposal != 1) draw(seed, g, row, (uint32_t)t, wword(t, 16, c), fw);
                     for (uint32_t pos = 4 * g; pos < 4 * g + 4 && pos < w; ++pos) {
                         const uint64_t site = site_of(L, c, row, pos);
                         float* s = xyz + 3 * site;
@@ -362,23 +391,12 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
                                 acc_h = acc_h + (nb[j] < 0 ? 0.0f : xyz[3 * nb[j] + a]);
                             h[a] = acc_h;
                         }
-                        uint32_t wu = 0;
-                        if (live) live = stream_next(&st, &wu);
-                        const float u = unit24f(wu);
                         float p[3] = {-s[0], -s[1], -s[2]};
-                        if (M->proposal == 1) {
-                            p[0] = 0.0f;
-                            p[1] = 0.0f;
-                            p[2] = 1.0f;
-                            while (live) {
-                                uint32_t w1, w2;
-                                if (!stream_next(&st, &w1) || !stream_next(&st, &w2)) {
-                                    live = 0;
-                                    break;
-                                }
-                                if (marsaglia_f32(w1, w2, p)) break;
-                            }
-                        }
+                        float u;
+                        if (M->proposal == 1)
+                            u = random_proposal(seed, pos, row, t, c, p);
+                        else
+                            u = unit24f(fw[pos & 3]);
                         const float d[3] = {p[0] - s[0], p[1] - s[1], p[2] - s[2]};
                         const float e_ex = jn * dot3f(d, h);
                         const float pe = dot3f(p, axis), oe = dot3f(s, axis);

@@ -10,7 +10,7 @@
  *  1. The checkerboard semantics the GPU engine defines (Philox4x32-10 keyed
  *     by seed/sweep/colour/row/position, the Ising 32-bit acceptance draw
  *     split into a shared high byte and a per-site low 24 bits, the fp32
- *     Heisenberg update with its Marsaglia word stream, the per-sweep
+ *     Heisenberg update with its per-site Marsaglia draws, the per-sweep
  *     integer sums).  The engine must match it bit for bit.
  *
  *  2. The reference's own sequential algorithm (odarbelaeze/vegas-rs 0.9.0):

@@ -85,9 +85,9 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
             }
         }
 
-        Stream st;
-        st.start(C, i0, row, C.sweep0 + s, colour);
-        bool live = true;
+        const uint64_t t = C.sweep0 + s;
+        U4 fw = U4{0u, 0u, 0u, 0u};
+        if (!C.proposal) fw = flip_words(C, i0, row, t, colour);
         float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
                 h[c] = acc_h;
             }
             float n[3];
-            heis_site(C, st, live, colour, o, h, n, acc);
+            heis_site(C, colour, pos, row, t, fw, j, o, h, n, acc);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];

@@ -32,73 +32,97 @@ __device__ __forceinline__ float f4get(const float4& v, uint32_t k) {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
 }
 
-// Sequential word stream of one 4-position group: Philox counter
-// (i/4, row, sweep, purpose 16 + call), consumed in position order.
-struct Stream {
-    U4 cur;
-    uint32_t call, word;
-    uint32_t cx, cy, cz, sweep_hi_bits, k0, k1;
-    __device__ __forceinline__ void start(const ChainDev& C, uint32_t i0, uint32_t row,
-                                          uint64_t t, uint32_t colour) {
-        call = 0;
-        cx = i0 >> 2;
-        cy = row;
-        cz = (uint32_t)t;
-        sweep_hi_bits = ((uint32_t)(t >> 32) << 8) | colour;
-        k0 = (uint32_t)C.seed;
-        k1 = (uint32_t)(C.seed >> 32);
-        refill();
+// 24-bit value m (0..4) of a Philox call: bits [24m, 24m+24) of the 128-bit
+// string x | y << 32 | z << 64 | w << 96 (bits 120-127 unused).
+__device__ __forceinline__ uint32_t v24(const U4& c, int m) {
+    switch (m) {
+        case 0: return c.x & 0xFFFFFFu;
+        case 1: return __builtin_amdgcn_alignbit(c.y, c.x, 24) & 0xFFFFFFu;
+        case 2: return __builtin_amdgcn_alignbit(c.z, c.y, 16) & 0xFFFFFFu;
+        case 3: return __builtin_amdgcn_alignbit(c.w, c.z, 8) & 0xFFFFFFu;
+        default: return c.w & 0xFFFFFFu;
     }
-    __device__ __forceinline__ void refill() {
-        cur = philox(U4{cx, cy, cz, sweep_hi_bits | ((kHeisStream + call) << 1)}, k0, k1);
-        word = 0;
-    }
-    // Returns false once the stream is exhausted (after kHeisMaxCalls calls).
-    __device__ __forceinline__ bool next(uint32_t& out) {
-        if (word == 4) {
-            if (call + 1 >= kHeisMaxCalls) return false;
-            ++call;
-            refill();
-        }
-        out = u4_word(cur, word++);
-        return true;
-    }
-};
+}
+// A 24-bit value as a uniform in [0,1) and in [-1,1) (exact in fp32).
+__device__ __forceinline__ float f24u(uint32_t v) { return fm((float)v, 5.9604644775390625e-08f); }
+__device__ __forceinline__ float f24s(uint32_t v) {
+    return fs(fm((float)v, 1.1920928955078125e-07f), 1.0f);
+}
 
-// One site: old spin o, neighbour sum h.  Draws its acceptance uniform (one
-// word), then for a random proposal two words per Marsaglia try until one
-// lands inside the unit disc; an exhausted stream gives u = 0 and proposal
-// (0,0,1).  Returns the new spin in n and adds the site's fixed-point sums
-// {bond (colour 1 only), mx, my, mz, (s.axis)^2} to acc.
-__device__ __forceinline__ void heis_site(const ChainDev& C, Stream& st, bool& live,
-                                          uint32_t colour, const float o[3], const float h[3],
-                                          float n[3], int64_t acc[5]) {
-    const float ox = o[0], oy = o[1], oz = o[2];
-    uint32_t wu = 0;
-    float px = -ox, py = -oy, pz = -oz;
-    live = live && st.next(wu);
-    const float u = unit24(wu);
-    if (C.proposal) {
-        px = 0.0f;
-        py = 0.0f;
-        pz = 1.0f;
-        while (live) {
-            uint32_t w1, w2;
-            if (!st.next(w1) || !st.next(w2)) {
-                live = false;
-                break;
-            }
-            const float x1 = sym24(w1), x2 = sym24(w2);
-            const float qq = fa(fm(x1, x1), fm(x2, x2));
-            if (qq < 1.0f) {
-                const float root = sqrt_rn(fs(1.0f, qq));
-                px = fm(fm(2.0f, x1), root);
-                py = fm(fm(2.0f, x2), root);
-                pz = fs(1.0f, fm(2.0f, qq));
-                break;
-            }
+struct Try {
+    float x1, x2, q;
+};
+__device__ __forceinline__ Try mtry(uint32_t a, uint32_t b) {
+    Try t;
+    t.x1 = f24s(a);
+    t.x2 = f24s(b);
+    t.q = fa(fm(t.x1, t.x1), fm(t.x2, t.x2));
+    return t;
+}
+
+// Fresh uniform-sphere proposal of one site (Spin::rand -> marsaglia,
+// util.rs:21-34) from its own Philox calls k = 0, 1, ... with counter
+// (pos, row, sweep, purpose 16 + k): u = value 0 of call 0; Marsaglia tries
+// (1,2), (3,4) of call 0 and (0,1), (2,3) of call 1 -- all four computed
+// without branches, the first inside the unit disc is taken -- then, rarely
+// (p = 0.2%), tries (0,1), (2,3) of calls 2..15; exhausted -> (0,0,1).
+// Returns u; the proposal goes to p.
+__device__ __forceinline__ float random_proposal(const ChainDev& C, uint32_t pos, uint32_t row,
+                                                 uint64_t t, uint32_t colour, float p[3]) {
+    const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
+    const U4 c0 = philox(U4{pos, row, (uint32_t)t, ctr_w(t, kHeisStream, colour)}, k0, k1);
+    const U4 c1 = philox(U4{pos, row, (uint32_t)t, ctr_w(t, kHeisStream + 1, colour)}, k0, k1);
+    const float u = f24u(v24(c0, 0));
+    const Try t0 = mtry(v24(c0, 1), v24(c0, 2));
+    const Try t1 = mtry(v24(c0, 3), v24(c0, 4));
+    const Try t2 = mtry(v24(c1, 0), v24(c1, 1));
+    const Try t3 = mtry(v24(c1, 2), v24(c1, 3));
+    Try pick = t3;
+    if (t2.q < 1.0f) pick = t2;
+    if (t1.q < 1.0f) pick = t1;
+    if (t0.q < 1.0f) pick = t0;
+    bool ok = pick.q < 1.0f;
+    for (uint32_t k = 2; !ok && k < kHeisMaxCalls; ++k) {
+        const U4 ck = philox(U4{pos, row, (uint32_t)t, ctr_w(t, kHeisStream + k, colour)}, k0, k1);
+        const Try a = mtry(v24(ck, 0), v24(ck, 1));
+        const Try b = mtry(v24(ck, 2), v24(ck, 3));
+        if (a.q < 1.0f) {
+            pick = a;
+            ok = true;
+        } else if (b.q < 1.0f) {
+            pick = b;
+            ok = true;
         }
     }
+    if (ok) {
+        const float root = sqrt_rn(fs(1.0f, pick.q));
+        p[0] = fm(fm(2.0f, pick.x1), root);
+        p[1] = fm(fm(2.0f, pick.x2), root);
+        p[2] = fs(1.0f, fm(2.0f, pick.q));
+    } else {
+        p[0] = 0.0f;
+        p[1] = 0.0f;
+        p[2] = 1.0f;
+    }
+    return u;
+}
+
+// Flip proposals (-s) take their acceptance uniform from word (i & 3) of one
+// call per 4 positions, counter (i >> 2, row, sweep, purpose 16).
+__device__ __forceinline__ U4 flip_words(const ChainDev& C, uint32_t i0, uint32_t row, uint64_t t,
+                                         uint32_t colour) {
+    return philox(U4{i0 >> 2, row, (uint32_t)t, ctr_w(t, kHeisStream, colour)},
+                  (uint32_t)C.seed, (uint32_t)(C.seed >> 32));
+}
+
+// One site: old spin o, neighbour sum h, acceptance uniform u, proposal p.
+// Writes the new spin to n and adds the site's fixed-point sums
+// {bond (colour 1 only), mx, my, mz, (s.axis)^2} to acc.
+__device__ __forceinline__ void heis_update(const ChainDev& C, uint32_t colour, const float o[3],
+                                            const float h[3], float u, const float p[3],
+                                            float n[3], int64_t acc[5]) {
+    const float ox = o[0], oy = o[1], oz = o[2];
+    const float px = p[0], py = p[1], pz = p[2];
     const float ax = C.ax, ay = C.ay, az = C.az;
     const float dx = fs(px, ox), dy = fs(py, oy), dz = fs(pz, oz);
     const float e_ex = fm(C.jn, dot3(dx, dy, dz, h[0], h[1], h[2]));
@@ -118,6 +142,23 @@ __device__ __forceinline__ void heis_site(const ChainDev& C, Stream& st, bool& l
     const float se = dot3(n[0], n[1], n[2], ax, ay, az);
     acc[4] += fix27(fm(se, se));
     if (colour == 1) acc[0] += fix27(dot3(n[0], n[1], n[2], h[0], h[1], h[2]));
+}
+
+// The site update of either proposal kind.
+__device__ __forceinline__ void heis_site(const ChainDev& C, uint32_t colour, uint32_t pos,
+                                          uint32_t row, uint64_t t, const U4& fw, uint32_t j,
+                                          const float o[3], const float h[3], float n[3],
+                                          int64_t acc[5]) {
+    float p[3], u;
+    if (C.proposal) {
+        u = random_proposal(C, pos, row, t, colour, p);
+    } else {
+        u = unit24(u4_word(fw, j));
+        p[0] = -o[0];
+        p[1] = -o[1];
+        p[2] = -o[2];
+    }
+    heis_update(C, colour, o, h, u, p, n, acc);
 }
 
 // Workgroup sums of the five quantities -> partials[sweep][colour][block][5].

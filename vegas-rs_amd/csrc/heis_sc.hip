@@ -9,10 +9,9 @@
 // colour-1 site.
 //
 // Work item: 4 consecutive positions of one row (one aligned 16-byte load per
-// component per row touched).  The 4 positions share one Philox stream
-// (counter x = i/4, purpose 16 + call index), consumed in position order:
-// each site takes one word for its acceptance uniform, then (random proposal)
-// two words per Marsaglia try until a try lands inside the unit disc.
+// component per row touched).  Random proposals draw from each site's own
+// Philox calls (counter x = i), so the four sites are independent; flip
+// proposals take word i & 3 of one call per item (counter x = i / 4).
 // All arithmetic is single IEEE fp32 operations in a fixed order (no
 // contraction), so the CPU oracle reproduces every bit.
 #include "heis_common.hpp"
@@ -83,9 +82,9 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
             for (int c = 0; c < 3; ++c) wrap0[c] = orow[c * plane];
         }
 
-        Stream st;
-        st.start(C, i0, row, C.sweep0 + s, colour);
-        bool live = true;
+        const uint64_t t = C.sweep0 + s;
+        U4 fw = U4{0u, 0u, 0u, 0u};
+        if (!C.proposal) fw = flip_words(C, i0, row, t, colour);
         float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
@@ -121,7 +120,7 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
             }
             const float o[3] = {ox, oy, oz};
             float n[3];
-            heis_site(C, st, live, colour, o, h, n, acc);
+            heis_site(C, colour, pos, row, t, fw, j, o, h, n, acc);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];

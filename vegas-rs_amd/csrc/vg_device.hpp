@@ -75,9 +75,9 @@ enum Purpose : uint32_t {
     kIsingProposal = 2,  // random-proposal bit, 128 positions per call
     kIsingInit = 3,      // initial spin, 4 positions per call
     kHeisInit = 4,       // initial spin stream, one position per stream
-    kHeisStream = 16,    // sweep stream call k uses purpose 16 + k, k < 112
+    kHeisStream = 16,    // Heisenberg sweep: proposal call k of a site uses purpose 16 + k
 };
-constexpr uint32_t kHeisMaxCalls = 112;
+constexpr uint32_t kHeisMaxCalls = 16;
 constexpr uint32_t kHeisInitMaxCalls = 64;
 
 VG_HD uint32_t ctr_w(uint64_t sweep, uint32_t purpose, uint32_t colour) {
