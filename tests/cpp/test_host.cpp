@@ -367,6 +367,59 @@ static void heisenberg_machine_runs() {
     EXPECT(std::fabs(st.magnetization().magnitude() - rec->records.back().magnetization) < 1e-5);
 }
 
+static void job_server_programs() {
+    // A CoolDown job's per-stage results equal StatSensor's values computed
+    // from the same chain driven stage by stage through a one-chain batch.
+    using H = Compound<Exchange, Zeeman<IsingSpin>>;
+    const Lattice l = Lattice::sc().expand(12, 12, 12);
+    H h = hamiltonian(Exchange::from_lattice(1.0, l), Zeeman<IsingSpin>());
+    const CoolDown cool(4.5, 4.0, 0.25, 30, 120);
+    JobServer srv;
+    const uint64_t a = srv.submit<IsingSpin>(h, MetropolisFlipIntegrator(), cool, 77);
+    const uint64_t b = srv.submit<IsingSpin>(h, MetropolisFlipIntegrator(),
+                                             HysteresisLoop(40, 10, 3.0, 0.2, 0.1), 78);
+    EXPECT(srv.wait(a, 60.0).state == VG_JOB_DONE);
+    EXPECT(srv.wait(b, 60.0).state == VG_JOB_DONE);
+    const auto ra = srv.results(a);
+    EXPECT(ra.size() == cool.schedule().size());
+    EXPECT(srv.results(b).size() == HysteresisLoop(40, 10, 3.0, 0.2, 0.1).schedule().size());
+    // the same program through the C ABI batch, accumulated like StatSensor
+    vg_chain_desc d{};
+    d.lattice = l.descriptor();
+    d.spin = VG_SPIN_ISING;
+    d.proposal = VG_PROPOSAL_FLIP;
+    d.hamiltonian = describe(h);
+    d.seed = 77;
+    vg_batch* batch = nullptr;
+    check(vg_batch_create(default_context(), 1, &d, &batch));
+    check(vg_batch_randomize(batch));
+    bool same = true;
+    size_t k = 0;
+    const double dir[3] = {0.0, 0.0, 1.0};
+    for (const Stage& st : cool.schedule()) {
+        check(vg_batch_set_thermostat(batch, 0, st.temperature, 0.0, dir));
+        check(vg_batch_run(batch, st.relax, nullptr, nullptr));
+        std::vector<double> e(st.measure), m(st.measure);
+        check(vg_batch_run(batch, st.measure, e.data(), m.data()));
+        Accumulator ae, am;
+        for (size_t i = 0; i < st.measure; ++i) {
+            ae.collect(e[i]);
+            am.collect(m[i]);
+        }
+        const double n = 12.0 * 12 * 12, T = st.temperature;
+        same &= ra[k].energy == ae.mean();
+        same &= ra[k].specific_heat == ae.variance() / (n * (T * T));
+        same &= ra[k].magnetization == am.mean();
+        same &= ra[k].susceptibility == am.variance() / (n * T);
+        same &= ra[k].binder == am.binder_cumulant();
+        same &= ra[k].measured == st.measure;
+        ++k;
+    }
+    vg_batch_destroy(batch);
+    EXPECT(same);
+    EXPECT(srv.stats().completed == 2);
+}
+
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
     std::vector<std::pair<const char*, std::function<void()>>> tests;
@@ -388,7 +441,8 @@ int main(int argc, char** argv) {
                  {"stat_sensor_lines", stat_sensor_lines},
                  {"state_sensor_snapshots", state_sensor_snapshots},
                  {"integrator_step_adapter", integrator_step_adapter},
-                 {"heisenberg_machine_runs", heisenberg_machine_runs}};
+                 {"heisenberg_machine_runs", heisenberg_machine_runs},
+                 {"job_server_programs", job_server_programs}};
     }
     for (auto& t : tests) {
         const int before = failures;

@@ -13,6 +13,7 @@
 // relax_for / measure_for; later calls continue the chain's counters.
 #pragma once
 
+#include <map>
 #include <memory>
 #include <random>
 #include <vector>

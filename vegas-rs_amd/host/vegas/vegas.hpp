@@ -1,7 +1,7 @@
 // Host library of the MI355X Metropolis engine: the reference's Rust trait
 // surface (state.rs, thermostat.rs, energy.rs, integrator.rs, machine.rs,
 // instrument.rs, accumulator.rs, program.rs) in C++ over the C ABI of
-// libvegas_hip.so.
+// libvegas_hip.so, and a job server for many programs at once.
 #pragma once
 
 #include "vegas_hip.h"
@@ -13,5 +13,6 @@
 #include "lattice.hpp"
 #include "machine.hpp"
 #include "program.hpp"
+#include "server.hpp"
 #include "state.hpp"
 #include "thermostat.hpp"
