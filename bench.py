@@ -258,15 +258,40 @@ def served_jobs(vg, n_jobs, gen_seed=2025):
             L = int(rng.integers(8, 49)) * 2
             T = float(rng.uniform(2.0, 6.0))
             d = vg.chain(vg.lattice(vg.SC, L, L, L), vg.ISING, vg.FLIP, ising_h, k)
-            jobs.append(("ising-relax", d, vg.relax(2000, T), L ** 3 * 2000))
+            jobs.append(("ising-relax", d, vg.relax(2000, T), L ** 3 * 2000, ("ising", L, 2000)))
         elif u < 0.9:
             L = int(rng.integers(8, 33)) * 2
             d = vg.chain(vg.lattice(vg.SC, L, L, L), vg.HEISENBERG, vg.RANDOM, heis_h, k)
-            jobs.append(("heis-cooldown", d, cool, L ** 3 * 500 * len(cool)))
+            jobs.append(("heis-cooldown", d, cool, L ** 3 * 500 * len(cool),
+                         ("heis", L, 500 * len(cool))))
         else:
             d = vg.chain(vg.lattice(vg.SC, 64, 64, 64), vg.HEISENBERG, vg.RANDOM, heis_h, k)
-            jobs.append(("heis-hysteresis", d, loop, 64 ** 3 * 1000 * len(loop)))
+            jobs.append(("heis-hysteresis", d, loop, 64 ** 3 * 1000 * len(loop),
+                         ("heis", 64, 1000 * len(loop))))
     return jobs
+
+
+def one_at_a_time_seconds(vg, ctx, jobs, probe_sweeps=60):
+    """Device time the job list would take run one job after another, each
+    as its own one-chain batch with plain launches: per-sweep time measured
+    for every (spin kind, L) in the list over probe_sweeps sweeps, times each
+    job's sweeps."""
+    per_sweep = {}
+    ising_h = vg.hamiltonian(exchange=1.0, zeeman=True)
+    heis_h = vg.hamiltonian(exchange=1.0, zeeman=True, anisotropy=0.1)
+    for key in sorted({j[4][:2] for j in jobs}):
+        spin, L = key
+        lat = vg.lattice(vg.SC, L, L, L)
+        d = (vg.chain(lat, vg.ISING, vg.FLIP, ising_h, 1) if spin == "ising" else
+             vg.chain(lat, vg.HEISENBERG, vg.RANDOM, heis_h, 1))
+        with vg.Batch(ctx, [d]) as b:
+            b.randomize()
+            b.set_thermostat(0, 2.0, 0.05)
+            b.run(5, observe=False)
+            t0 = time.perf_counter()
+            b.run(probe_sweeps)
+            per_sweep[key] = (time.perf_counter() - t0) / probe_sweeps
+    return sum(j[4][2] * per_sweep[j[4][:2]] for j in jobs)
 
 
 def run_served(vg, ctx, n_jobs, rates, gen_seed=2025):
@@ -284,7 +309,7 @@ def run_served(vg, ctx, n_jobs, rates, gen_seed=2025):
                        tile=2) as srv:
             t0 = time.perf_counter()
             ids, sub_t = [], []
-            for (kind, d, stages, _), ta in zip(jobs, arrivals):
+            for (kind, d, stages, _, _), ta in zip(jobs, arrivals):
                 delay = t0 + ta - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
@@ -324,6 +349,14 @@ def run_served(vg, ctx, n_jobs, rates, gen_seed=2025):
             f"{res['p50_latency_s']:.3f} s, p99 {res['p99_latency_s']:.3f} s, peak "
             f"{stats.peak_jobs} jobs")
         out.append(res)
+    # The same jobs one at a time with per-chain plain launches.
+    seq_s = one_at_a_time_seconds(vg, ctx, jobs)
+    attempts = sum(j[3] for j in jobs)
+    for res in out:
+        res["one_at_a_time_throughput"] = attempts / seq_s
+        res["speedup_vs_one_at_a_time"] = res["throughput"] / (attempts / seq_s)
+    log(f"served: one job at a time with plain launches {attempts / seq_s:.3e} attempts/s "
+        f"({seq_s:.1f} s)")
     return out
 
 
