@@ -294,7 +294,7 @@ def one_at_a_time_seconds(vg, ctx, jobs, probe_sweeps=60):
     return sum(j[4][2] * per_sweep[j[4][:2]] for j in jobs)
 
 
-def run_served(vg, ctx, n_jobs, rates, gen_seed=2025):
+def run_served(vg, ctx, n_jobs, rates, gen_seed=2025, budget=0):
     """Poisson arrivals at each rate: one thread submits each job at its
     arrival time; latency = finish - arrival.  Throughput = attempts of all
     jobs / (last finish - first arrival)."""
@@ -306,7 +306,7 @@ def run_served(vg, ctx, n_jobs, rates, gen_seed=2025):
         arrivals = np.cumsum(arr_rng.exponential(1.0 / rate, size=len(jobs)))
         log(f"served: {len(jobs)} jobs at {rate}/s")
         with vg.Server(ctx, memory_bytes=16 << 30, max_jobs=1024, quantum_sweeps=100,
-                       tile=2) as srv:
+                       tile=2, sweep_budget=budget) as srv:
             t0 = time.perf_counter()
             ids, sub_t = [], []
             for (kind, d, stages, _, _), ta in zip(jobs, arrivals):
@@ -334,6 +334,7 @@ def run_served(vg, ctx, n_jobs, rates, gen_seed=2025):
         res = {
             "rate_per_s": rate,
             "jobs": len(jobs),
+            "sweep_budget": budget,
             "attempts": attempts,
             "throughput": attempts / makespan,
             "makespan_s": makespan,
@@ -381,6 +382,8 @@ def main():
     ap.add_argument("--heis-side", type=int, default=128)
     ap.add_argument("--heis-sweeps", type=int, default=500)
     ap.add_argument("--serve-rates", default="50,500")
+    ap.add_argument("--serve-budget", type=int, default=0,
+                    help="server per-sweep work budget (Ising-site units; 0 = none)")
     ap.add_argument("--batched-sweeps", type=int, default=10000)
     ap.add_argument("--baseline-sweeps", type=int, default=100)
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -418,7 +421,8 @@ def main():
     if args.workload == "served":
         rates = [float(r) for r in args.serve_rates.split(",")]
         with vg.Context(0) as ctx:
-            print(json.dumps(run_served(vg, ctx, args.serve_jobs, rates)), flush=True)
+            print(json.dumps(run_served(vg, ctx, args.serve_jobs, rates,
+                                        budget=args.serve_budget)), flush=True)
         return
     temps = [3.5, 4.5115, 6.0]
     side = args.side
@@ -522,7 +526,7 @@ def main():
         out["batched"] = run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps)
         out["heisenberg"] = run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)
         rates = [float(r) for r in args.serve_rates.split(",")]
-        out["served"] = run_served(vg, ctx, args.serve_jobs, rates)
+        out["served"] = run_served(vg, ctx, args.serve_jobs, rates, budget=args.serve_budget)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -17,7 +17,9 @@
  *  - Every function returns an int status (vg_status); 0 is success.  On
  *    failure vg_last_error() returns a thread-local message.
  *  - Handles are opaque and owned by the library; device memory is owned by
- *    the library.  A handle is used by one host thread at a time.
+ *    the library.  A handle is used by one host thread at a time (the job
+ *    server's functions excepted).  Destroy batches and servers before the
+ *    context they were created on.
  *  - Host buffers are caller-owned plain arrays.  Site order at the boundary
  *    is the lattice's natural order (vg_lattice, below).
  */
@@ -231,6 +233,10 @@ typedef struct vg_server_config {
     uint32_t quantum_sweeps; /* sweeps per scheduling quantum, 1..256 */
     uint32_t tile;           /* Ising sc tile: 1, 2, 4 or 8 */
     int32_t use_graph;       /* 1: replay quanta from hipGraphs while the job set is unchanged */
+    uint64_t sweep_budget;   /* work per sweep of a quantum, in Ising-site units (a Heisenberg
+                                site counts 15): jobs with the least remaining work are taken
+                                first up to this sum, the rest wait for a later quantum;
+                                0 = every running job in every quantum */
 } vg_server_config;
 
 /* One stage: set the thermostat, run `relax` sweeps, then `measure`

@@ -80,6 +80,6 @@ def test_program_stage_lists(vg):
 def test_server_rejects_bad_config_without_device(vg):
     """Argument checks come before any device work."""
     import ctypes as C
-    cfg = vg.ServerConfig(0, 1, 100, 2, 0)
+    cfg = vg.ServerConfig(0, 1, 100, 2, 0, 0)
     h = C.c_void_p()
     assert vg.library().vg_server_create(None, C.byref(cfg), C.byref(h)) == 20

@@ -216,3 +216,18 @@ def test_autotune_leaves_trajectory_unchanged(vg, orc, ctx):
     assert np.array_equal(runs[0][0], runs[1][0])
     assert all(np.array_equal(a, c) for a, c in zip(runs[0][1], runs[1][1]))
     assert runs[0][2] == runs[1][2] == [8, 8]
+
+
+def test_sweep_counter_limit(vg, ctx):
+    """Philox takes 56 bits of the sweep counter: setting or running past
+    2^56 is refused, up to it works and continues the counter."""
+    d = vg.chain(vg.lattice(vg.SC, 8, 8, 8), vg.ISING, vg.FLIP, vg.hamiltonian(exchange=1.0), 3)
+    with vg.Batch(ctx, [d]) as b:
+        b.randomize()
+        with pytest.raises(vg.VegasError):
+            b.set_sweep_count(0, 1 << 56)
+        b.set_sweep_count(0, (1 << 56) - 4)
+        b.run(4)
+        assert b.sweep_count(0) == 1 << 56
+        with pytest.raises(vg.VegasError):
+            b.run(1)

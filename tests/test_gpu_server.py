@@ -40,21 +40,25 @@ def test_alone_equals_in_mix(vg, ctx):
             assert st.state == vg.JOB_DONE
             alone.append(srv.results(jid))
     # All at once, with few slots and a small pool: jobs queue, get admitted
-    # as others finish, and share sweep blocks with different quantum cuts.
+    # as others finish, and share sweep blocks with different quantum cuts;
+    # then with a per-sweep budget that makes the shortest jobs run first and
+    # the others sit quanta out.
     pool = 400 << 10  # the jobs need ~700 KiB together
-    with vg.Server(ctx, memory_bytes=pool, max_jobs=5, quantum_sweeps=48) as srv:
-        ids = [srv.submit(d, stages) for d, stages in jobs]
-        mixed = []
-        for jid in ids:
-            st = srv.wait(jid, 120.0)
-            assert st.state == vg.JOB_DONE, st.state
-            mixed.append(srv.results(jid))
-        stats = srv.stats()
-        assert stats.peak_jobs > 1            # work overlapped on the device
-        assert stats.pool_peak <= pool
-        assert stats.completed == len(jobs)
-    for k, (a, b) in enumerate(zip(alone, mixed)):
-        assert np.array_equal(a, b), k
+    for budget in (0, 40000):
+        with vg.Server(ctx, memory_bytes=pool, max_jobs=5, quantum_sweeps=48,
+                       sweep_budget=budget) as srv:
+            ids = [srv.submit(d, stages) for d, stages in jobs]
+            mixed = []
+            for jid in ids:
+                st = srv.wait(jid, 120.0)
+                assert st.state == vg.JOB_DONE, st.state
+                mixed.append(srv.results(jid))
+            stats = srv.stats()
+            assert stats.peak_jobs > 1            # work overlapped on the device
+            assert stats.pool_peak <= pool
+            assert stats.completed == len(jobs)
+        for k, (a, b) in enumerate(zip(alone, mixed)):
+            assert np.array_equal(a, b), (budget, k)
 
 
 def test_server_equals_batch_program(vg, ctx):

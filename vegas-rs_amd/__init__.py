@@ -63,7 +63,8 @@ class SweepSums(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_bytes", C.c_uint64), ("max_jobs", C.c_uint32),
-                ("quantum_sweeps", C.c_uint32), ("tile", C.c_uint32), ("use_graph", C.c_int32)]
+                ("quantum_sweeps", C.c_uint32), ("tile", C.c_uint32), ("use_graph", C.c_int32),
+                ("sweep_budget", C.c_uint64)]
 
 
 class Stage(C.Structure):
@@ -417,8 +418,9 @@ class Server:
     """Job server with continuous batching (vg_server_* of vegas_hip.h)."""
 
     def __init__(self, ctx: Context, memory_bytes=1 << 30, max_jobs=256, quantum_sweeps=100,
-                 tile=2, use_graph=False):
-        cfg = ServerConfig(memory_bytes, max_jobs, quantum_sweeps, tile, 1 if use_graph else 0)
+                 tile=2, use_graph=False, sweep_budget=0):
+        cfg = ServerConfig(memory_bytes, max_jobs, quantum_sweeps, tile, 1 if use_graph else 0,
+                           sweep_budget)
         h = _vp()
         _check(_lib.vg_server_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h

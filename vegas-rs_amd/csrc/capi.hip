@@ -425,6 +425,10 @@ int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
              double* magnetization) {
     if (!B) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
     const uint64_t nch = B->chains.size();
+    // The sweep counter keys Philox with 56 bits (low word + 24 high bits).
+    for (const ChainHost& ch : B->chains)
+        if (sweeps > (1ull << 56) - ch.sweeps)
+            return fail(VG_ERR_INVALID_ARGUMENT, "sweep counter would pass 2^56");
     hipStream_t st = B->ctx->stream;
     VG_HIP(hipSetDevice(B->ctx->device));
     B->last_ms = 0.0;

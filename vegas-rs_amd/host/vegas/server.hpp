@@ -37,6 +37,7 @@ struct JobServerConfig {
     uint32_t quantum_sweeps = 100;       // sweeps per scheduling quantum
     uint32_t tile = 2;                   // Ising sc tile
     bool graph = false;                  // replay quanta from hipGraphs
+    uint64_t sweep_budget = 0;           // per-sweep work cap, shortest jobs first (0: none)
 };
 
 class JobServer {
@@ -49,6 +50,7 @@ class JobServer {
         cfg.quantum_sweeps = c.quantum_sweeps;
         cfg.tile = c.tile;
         cfg.use_graph = c.graph ? 1 : 0;
+        cfg.sweep_budget = c.sweep_budget;
         check(vg_server_create(ctx, &cfg, &server_));
     }
     ~JobServer() { vg_server_destroy(server_); }
