@@ -420,10 +420,134 @@ static void job_server_programs() {
     EXPECT(srv.stats().completed == 2);
 }
 
+// Output tables (output.rs): the sensors' parquet files, written here by
+// driving the hooks directly, are read back and checked by pyarrow in
+// tests/test_host_cpp.py.
+static void parquet_tables(const std::string& dir) {
+    using vegas::parquet_detail::with_extension;
+    EXPECT(with_extension("a/b.parquet", "parquet.tmp") == "a/b.parquet.tmp");
+    EXPECT(with_extension("a/b", "parquet.tmp") == "a/b.parquet.tmp");
+    EXPECT(with_extension("a.d/b", "parquet.tmp") == "a.d/b.parquet.tmp");
+    EXPECT(with_extension("a/.hidden", "parquet.tmp") == "a/.hidden.parquet.tmp");
+    EXPECT(with_extension("x.tar.gz", "parquet.tmp") == "x.tar.parquet.tmp");
+
+    using HI = Exchange;
+    {
+        ObservableSensor<HI, IsingSpin> obs(dir + "/obs.parquet");
+        // nothing is visible under the final name until the file is closed
+        std::FILE* tmp = std::fopen((dir + "/obs.parquet.tmp").c_str(), "rb");
+        std::FILE* fin = std::fopen((dir + "/obs.parquet").c_str(), "rb");
+        EXPECT(tmp != nullptr && fin == nullptr);
+        if (tmp) std::fclose(tmp);
+        if (fin) std::fclose(fin);
+        const HI h = Exchange::from_lattice(1.0, Lattice::sc().expand_all(2));
+        obs.on_relax_start(Thermostat<IsingSpin>(3.0, Field<IsingSpin>(IsingSpin::down(), 0.5)), h, 8);
+        for (int k = 0; k < 3; ++k) obs.after_step(SweepRecord{-1.0 - k, 0.25 * k});
+        obs.on_relax_end();
+        obs.on_measure_start(Thermostat<IsingSpin>(2.0, Field<IsingSpin>::zero()), h, 8);
+        for (int k = 0; k < 2; ++k) obs.after_step(SweepRecord{-10.0 - k, 0.5 + k});
+        obs.on_measure_end();
+        obs.after_step(SweepRecord{99.0, 99.0});  // outside a phase: no row
+        EXPECT(obs.rows().empty());
+        obs.close();
+    }
+    {
+        StateSensor<HI, IsingSpin> ss(dir + "/state_ising.parquet", 2);
+        const HI h = Exchange::from_lattice(1.0, Lattice::sc().expand_all(2));
+        auto st = State<IsingSpin>::up_with_size(5);
+        st.set_at(1, IsingSpin::down());
+        st.set_at(4, IsingSpin::down());
+        ss.on_relax_start(Thermostat<IsingSpin>(1.5, Field<IsingSpin>::zero()), h, 5);
+        ss.on_state(st);  // step 0
+        ss.after_step(SweepRecord{0, 0});
+        ss.after_step(SweepRecord{0, 0});
+        st.set_at(0, IsingSpin::down());
+        ss.on_state(st);  // step 2
+        ss.on_relax_end();
+        EXPECT(ss.snapshots().empty());
+    }  // closed by the destructor
+    {
+        StateSensor<HI, HeisenbergSpin> ss(dir + "/state_heis.parquet", 1);
+        const HI h = Exchange::from_lattice(1.0, Lattice::sc().expand_all(2));
+        auto st = State<HeisenbergSpin>::up_with_size(3);
+        st.set_at(2, HeisenbergSpin(0.6, 0.0, -0.8));
+        ss.on_measure_start(Thermostat<HeisenbergSpin>(0.7, Field<HeisenbergSpin>(HeisenbergSpin(0.0, 1.0, 0.0), 0.2)), h, 3);
+        ss.on_state(st);
+        ss.on_measure_end();
+        ss.close();
+    }
+    {
+        // more rows than one data page holds, over two row groups
+        ParquetWriter w(dir + "/big.parquet", {{"flag", ParquetWriter::Kind::Boolean},
+                                              {"k", ParquetWriter::Kind::UInt64},
+                                              {"x", ParquetWriter::Kind::Float64}});
+        const size_t n = ParquetWriter::kPageValues + 7;
+        std::vector<uint8_t> flag(n);
+        std::vector<uint64_t> k(n);
+        std::vector<double> x(n);
+        for (size_t i = 0; i < n; ++i) {
+            flag[i] = (i % 3) == 0;
+            k[i] = (uint64_t)i * 0x9E3779B97F4A7C15ull;
+            x[i] = (double)i * 0.5 - 3.0;
+        }
+        w.write(n, {flag.data(), k.data(), x.data()});
+        w.write(3, {flag.data(), k.data(), x.data()});
+        EXPECT(w.rows() == n + 3);
+    }
+    {
+        ParquetWriter w(dir + "/empty.parquet", observable_schema());
+    }
+    EXPECT(throws_kind([&] { ParquetWriter w(dir + "/missing/x.parquet", state_schema()); },
+                       ErrorKind::Io));
+}
+
+// A Machine run writing both output tables; the in-memory sensors' rows go
+// to CSV beside them for tests/test_host_cpp.py to compare.
+static void sensors_write_parquet(const std::string& dir) {
+    using H = Compound<Exchange, Zeeman<IsingSpin>>;
+    const Lattice l = Lattice::sc().expand(10, 10, 1).drop_z();
+    Machine<H, MetropolisFlipIntegrator, IsingSpin>::Instruments ins;
+    auto* mem = new ObservableSensor<H, IsingSpin>();
+    auto* pq = new ObservableSensor<H, IsingSpin>(dir + "/obs.parquet");
+    auto* smem = new StateSensor<H, IsingSpin>(3);
+    auto* spq = new StateSensor<H, IsingSpin>(dir + "/state.parquet", 3);
+    ins.emplace_back(mem);
+    ins.emplace_back(pq);
+    ins.emplace_back(smem);
+    ins.emplace_back(spq);
+    std::mt19937_64 rng(5);
+    Machine<H, MetropolisFlipIntegrator, IsingSpin> m(
+        Thermostat<IsingSpin>(2.8, Field<IsingSpin>::zero()),
+        hamiltonian(Exchange::from_lattice(1.0, l), Zeeman<IsingSpin>()),
+        MetropolisFlipIntegrator(), std::move(ins), State<IsingSpin>::rand_with_size(rng, 100));
+    CoolDown(4.0, 3.5, 0.1, 6, 10).run(rng, m);
+    pq->close();
+    spq->close();
+    mem->write_csv(dir + "/obs.csv");
+    std::FILE* f = std::fopen((dir + "/state.csv").c_str(), "w");
+    EXPECT(f != nullptr);
+    if (!f) return;
+    std::fprintf(f, "relax,stage,step,temperature,field,id,sx,sy,sz\n");
+    for (const auto& sn : smem->snapshots())
+        for (size_t i = 0; i < sn.state.len(); ++i)
+            std::fprintf(f, "%s,%llu,%llu,%.17g,%.17g,%zu,%.17g,%.17g,%.17g\n",
+                         sn.relax ? "true" : "false", (unsigned long long)sn.stage,
+                         (unsigned long long)sn.step, sn.temperature, sn.field, i,
+                         sn.state.at(i).sx(), sn.state.at(i).sy(), sn.state.at(i).sz());
+    std::fclose(f);
+    EXPECT(smem->snapshots().size() == 5 * (2 + 4));  // steps 0,3 of relax; 0,3,6,9 of measure
+}
+
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
     std::vector<std::pair<const char*, std::function<void()>>> tests;
-    if (!gpu) {
+    if (argc > 2 && std::strcmp(argv[1], "--parquet") == 0) {
+        const std::string dir = argv[2];
+        tests = {{"parquet_tables", [dir] { parquet_tables(dir); }}};
+    } else if (argc > 2 && std::strcmp(argv[1], "--gpu-parquet") == 0) {
+        const std::string dir = argv[2];
+        tests = {{"sensors_write_parquet", [dir] { sensors_write_parquet(dir); }}};
+    } else if (!gpu) {
         tests = {{"ising_spin_multiplies_correctly", ising_spin_multiplies_correctly},
                  {"ising_magnetization_can_be_added_with_spin", ising_magnetization_can_be_added_with_spin},
                  {"heisenberg_spin_multiplies_correctly", heisenberg_spin_multiplies_correctly},

@@ -32,3 +32,105 @@ def test_host_library_gpu():
     Machine draws; StatSensor lines; StateSensor snapshots; the step()
     adapter; Wolff and fcc rejected."""
     _run("--gpu")
+
+
+def test_parquet_tables(tmp_path):
+    """The sensors' parquet files (reference src/output.rs:37-46, 126-136):
+    column names, order and types as the reference's Arrow schemas, rows and
+    values as the hooks produced them, pages split past 2^20 values, several
+    row groups, an empty table, tmp-then-rename."""
+    import numpy as np
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+
+    _run_parquet(tmp_path)
+    obs = pq.read_table(tmp_path / "obs.parquet")
+    assert obs.schema == pa.schema([
+        pa.field("relax", pa.bool_(), nullable=False),
+        pa.field("stage", pa.uint64(), nullable=False),
+        pa.field("step", pa.uint64(), nullable=False),
+        pa.field("n", pa.uint64(), nullable=False),
+        pa.field("temperature", pa.float64(), nullable=False),
+        pa.field("field", pa.float64(), nullable=False),
+        pa.field("energy", pa.float64(), nullable=False),
+        pa.field("magnetization", pa.float64(), nullable=False),
+    ])
+    d = obs.to_pydict()
+    assert d["relax"] == [True] * 3 + [False] * 2
+    assert d["stage"] == [0, 0, 0, 1, 1]
+    assert d["step"] == [0, 1, 2, 0, 1]
+    assert d["n"] == [8] * 5
+    assert d["temperature"] == [3.0] * 3 + [2.0] * 2
+    assert d["field"] == [0.5] * 3 + [0.0] * 2   # |h| of a Down-oriented field
+    assert d["energy"] == [-1.0, -2.0, -3.0, -10.0, -11.0]
+    assert d["magnetization"] == [0.0, 0.25, 0.5, 0.5, 1.5]
+    assert pq.ParquetFile(tmp_path / "obs.parquet").num_row_groups == 2  # one per phase
+    assert not (tmp_path / "obs.parquet.tmp").exists()
+
+    si = pq.read_table(tmp_path / "state_ising.parquet")
+    assert si.schema.names == ["relax", "stage", "step", "temperature", "field", "id",
+                               "sx", "sy", "sz"]
+    assert [f.type for f in si.schema] == [pa.bool_(), pa.uint64(), pa.uint64(), pa.float64(),
+                                           pa.float64(), pa.uint64(), pa.float64(),
+                                           pa.float64(), pa.float64()]
+    d = si.to_pydict()
+    assert d["step"] == [0] * 5 + [2] * 5
+    assert d["id"] == list(range(5)) * 2
+    assert d["sx"] == [0.0] * 10 and d["sy"] == [0.0] * 10
+    assert d["sz"] == [1, -1, 1, 1, -1] + [-1, -1, 1, 1, -1]
+    assert d["relax"] == [True] * 10 and d["temperature"] == [1.5] * 10
+
+    sh = pq.read_table(tmp_path / "state_heis.parquet").to_pydict()
+    assert sh["relax"] == [False] * 3 and sh["field"] == [0.2] * 3
+    assert (sh["sx"], sh["sy"], sh["sz"]) == ([0.0, 0.0, 0.6], [0.0, 0.0, 0.0], [1.0, 1.0, -0.8])
+
+    big = pq.ParquetFile(tmp_path / "big.parquet")
+    assert big.num_row_groups == 2
+    t = big.read()
+    n = (1 << 20) + 7
+    i = np.arange(n, dtype=np.uint64)
+    flag = t.column("flag").to_numpy()
+    k = t.column("k").to_numpy()
+    x = t.column("x").to_numpy()
+    assert len(t) == n + 3
+    assert np.array_equal(flag[:n], i % 3 == 0)
+    assert np.array_equal(k[:n], i * np.uint64(0x9E3779B97F4A7C15))
+    assert np.array_equal(x[:n], i.astype(np.float64) * 0.5 - 3.0)
+    assert np.array_equal(k[n:], k[:3]) and np.array_equal(flag[n:], flag[:3])
+
+    empty = pq.read_table(tmp_path / "empty.parquet")
+    assert len(empty) == 0 and empty.schema.names == obs.schema.names
+
+
+@pytest.mark.gpu
+def test_machine_parquet_tables(tmp_path):
+    """A CoolDown on the engine with ObservableSensor and StateSensor writing
+    parquet: the files hold exactly the rows the in-memory sensors kept."""
+    import numpy as np
+    import pyarrow.csv as pc
+    import pyarrow.parquet as pq
+
+    _run_parquet(tmp_path, "--gpu-parquet")
+    for name in ("obs", "state"):
+        t = pq.read_table(tmp_path / f"{name}.parquet")
+        c = pc.read_csv(tmp_path / f"{name}.csv")
+        assert t.schema.names == c.schema.names
+        assert len(t) == len(c) > 0
+        for col in t.schema.names:
+            a = t.column(col).to_numpy()
+            b = c.column(col).to_numpy()
+            assert np.array_equal(a, b.astype(a.dtype)), (name, col)
+    obs = pq.read_table(tmp_path / "obs.parquet").to_pydict()
+    assert len(obs["step"]) == 5 * (6 + 10)
+    assert obs["stage"][0] == 0 and obs["stage"][-1] == 9
+
+
+def _run_parquet(out_dir, flag="--parquet"):
+    if not BIN.exists():
+        pytest.fail(f"{BIN} is not built (make -C tests/cpp)")
+    r = subprocess.run([str(BIN), flag, str(out_dir)], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout)
+    print(r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout

@@ -10,7 +10,8 @@
 //
 // Work item: 4 consecutive positions of one row; each of the four neighbour
 // rows is read with one aligned 16-byte load per component, plus the one
-// position beyond the item (x-1 or x+4).
+// position beyond the item (x-1 or x+4).  Same three phases as the sc pass
+// (fields and first proposal calls, the wave's queue, updates).
 #include "heis_common.hpp"
 
 namespace vg {
@@ -21,24 +22,37 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
                                                      int64_t* __restrict__ partials,
                                                      uint32_t colour, uint32_t s,
                                                      uint32_t total_blocks, uint32_t grid) {
+    __shared__ HeisQueue hq[kBlock / 64];
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
     if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
+    const bool active = local < C.items;
+    const uint64_t t = C.sweep0 + s;
     int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
 
-    if (local < C.items) {
+    // Phase 1: neighbour fields, old spins and proposals of the item's sites.
+    uint32_t row = 0, i0 = 0, nvalid = 0, pend = 0;
+    float* own = arena;
+    float os[4][3], hs[4][3], ps[4][3], us[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        us[j] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) os[j][c] = hs[j][c] = ps[j][c] = 0.0f;
+    }
+    if (active) {
         const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
         const uint64_t plane = C.plane;
-        const uint32_t row = fast_div(local, C.segs_m, C.segs_s);
+        row = fast_div(local, C.segs_m, C.segs_s);
         const uint32_t seg = local - row * C.segs;
         const uint32_t z = fast_div(row, C.ly_m, C.ly_s);
         const uint32_t y = row - z * ly;
         const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
-        const uint32_t i0 = seg * kHeisSeg;
-        const uint32_t nvalid = min(kHeisSeg, w - i0);
-        float* own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
+        i0 = seg * kHeisSeg;
+        nvalid = min(kHeisSeg, w - i0);
+        own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
         const float* oth = arena + C.off[colour ^ 1u];
 
         bool yok, zok;
@@ -85,19 +99,14 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
             }
         }
 
-        const uint64_t t = C.sweep0 + s;
         U4 fw = U4{0u, 0u, 0u, 0u};
         if (!C.proposal) fw = flip_words(C, i0, row, t, colour);
-        float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
-            const float o[3] = {f4get(so[0], j), f4get(so[1], j), f4get(so[2], j)};
-            ns[0][j] = o[0];
-            ns[1][j] = o[1];
-            ns[2][j] = o[2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) os[j][c] = f4get(so[c], j);
             if (j >= nvalid) continue;
             const uint32_t pos = i0 + j;
-            float h[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 float acc_h = 0.0f;
@@ -111,10 +120,23 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
                         bx = pos + 1 == w ? wrap0[r][c] : (j == 3 ? xe[r][c] : f4get(v[r][c], j + 1));
                     acc_h = r == 0 ? fa(a, bx) : fa(fa(acc_h, a), bx);
                 }
-                h[c] = acc_h;
+                hs[j][c] = acc_h;
             }
-            float n[3];
-            heis_site(C, colour, pos, row, t, fw, j, o, h, n, acc);
+            heis_propose(C, colour, pos, row, t, fw, j, os[j], us[j], ps[j], pend);
+        }
+    }
+
+    // Phase 2: the wave's sites whose first proposal call missed.
+    if (C.proposal) heis_resolve(C, hq[threadIdx.x >> 6], pend, i0, row, t, colour, ps);
+
+    // Phase 3: accept or reject, store, sums.
+    if (active) {
+        const uint64_t plane = C.plane;
+        float ns[3][4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            float n[3] = {os[j][0], os[j][1], os[j][2]};
+            if (j < nvalid) heis_update(C, colour, os[j], hs[j], us[j], ps[j], n, acc);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];

@@ -60,51 +60,124 @@ __device__ __forceinline__ Try mtry(uint32_t a, uint32_t b) {
     return t;
 }
 
+// A Marsaglia try inside the unit disc -> point on the sphere.
+__device__ __forceinline__ void sphere_point(const Try& a, float p[3]) {
+    const float root = sqrt_rn(fs(1.0f, a.q));
+    p[0] = fm(fm(2.0f, a.x1), root);
+    p[1] = fm(fm(2.0f, a.x2), root);
+    p[2] = fs(1.0f, fm(2.0f, a.q));
+}
+
 // Fresh uniform-sphere proposal of one site (Spin::rand -> marsaglia,
 // util.rs:21-34) from its own Philox calls k = 0, 1, ... with counter
 // (pos, row, sweep, purpose 16 + k): u = value 0 of call 0; Marsaglia tries
-// (1,2), (3,4) of call 0 and (0,1), (2,3) of call 1 -- all four computed
-// without branches, the first inside the unit disc is taken -- then, rarely
-// (p = 0.2%), tries (0,1), (2,3) of calls 2..15; exhausted -> (0,0,1).
-// Returns u; the proposal goes to p.
-__device__ __forceinline__ float random_proposal(const ChainDev& C, uint32_t pos, uint32_t row,
-                                                 uint64_t t, uint32_t colour, float p[3]) {
-    const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
-    const U4 c0 = philox(U4{pos, row, (uint32_t)t, ctr_w(t, kHeisStream, colour)}, k0, k1);
-    const U4 c1 = philox(U4{pos, row, (uint32_t)t, ctr_w(t, kHeisStream + 1, colour)}, k0, k1);
-    const float u = f24u(v24(c0, 0));
+// (1,2), (3,4) of call 0, then (0,1), (2,3) of calls 1..15, the first inside
+// the unit disc taken; exhausted -> (0,0,1).
+//
+// proposal_first does call 0 (both tries without branches) and returns
+// whether one hit (p = 0.954); proposal_rest does calls 1..15 for a site
+// whose first two tries missed.
+__device__ __forceinline__ bool proposal_first(const ChainDev& C, uint32_t pos, uint32_t row,
+                                               uint64_t t, uint32_t colour, float& u, float p[3]) {
+    const U4 c0 = philox(U4{pos, row, (uint32_t)t, ctr_w(t, kHeisStream, colour)},
+                         (uint32_t)C.seed, (uint32_t)(C.seed >> 32));
+    u = f24u(v24(c0, 0));
     const Try t0 = mtry(v24(c0, 1), v24(c0, 2));
     const Try t1 = mtry(v24(c0, 3), v24(c0, 4));
-    const Try t2 = mtry(v24(c1, 0), v24(c1, 1));
-    const Try t3 = mtry(v24(c1, 2), v24(c1, 3));
-    Try pick = t3;
-    if (t2.q < 1.0f) pick = t2;
-    if (t1.q < 1.0f) pick = t1;
+    Try pick = t1;
     if (t0.q < 1.0f) pick = t0;
-    bool ok = pick.q < 1.0f;
-    for (uint32_t k = 2; !ok && k < kHeisMaxCalls; ++k) {
+    const bool ok = pick.q < 1.0f;
+    if (!ok) pick.q = 0.0f;  // keeps the square root finite; p is replaced later
+    sphere_point(pick, p);
+    return ok;
+}
+
+__device__ __forceinline__ void proposal_rest(const ChainDev& C, uint32_t pos, uint32_t row,
+                                              uint64_t t, uint32_t colour, float p[3]) {
+    const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
+    for (uint32_t k = 1; k < kHeisMaxCalls; ++k) {
         const U4 ck = philox(U4{pos, row, (uint32_t)t, ctr_w(t, kHeisStream + k, colour)}, k0, k1);
         const Try a = mtry(v24(ck, 0), v24(ck, 1));
+        if (a.q < 1.0f) return sphere_point(a, p);
         const Try b = mtry(v24(ck, 2), v24(ck, 3));
-        if (a.q < 1.0f) {
-            pick = a;
-            ok = true;
-        } else if (b.q < 1.0f) {
-            pick = b;
-            ok = true;
+        if (b.q < 1.0f) return sphere_point(b, p);
+    }
+    p[0] = 0.0f;
+    p[1] = 0.0f;
+    p[2] = 1.0f;
+}
+
+// Per-wave queue of the sites whose first call missed (about 3 in 64).
+struct HeisQueue {
+    uint32_t pos[64];
+    uint32_t row[64];
+    float p[3][64];
+};
+
+// Resolves the wave's pending sites: bit j of pend marks position i0 + j of
+// this lane's row.  Entries are numbered by an exclusive prefix sum of the
+// lanes' counts (three ballots); in rounds of 64, the owners post
+// (position, row), lane e computes entry e's calls 1..15, and the owners
+// read their proposals back.  LDS operations of one wave complete in order,
+// so the wave needs no barrier, only no reordering by the compiler.  Every
+// lane of the wave must call this.
+__device__ __forceinline__ void heis_resolve(const ChainDev& C, HeisQueue& q, uint32_t pend,
+                                             uint32_t i0, uint32_t row, uint64_t t,
+                                             uint32_t colour, float p[4][3]) {
+    const uint32_t lane = __lane_id();
+    const uint32_t cnt = __builtin_popcount(pend);
+    uint32_t pre = 0, total = 0;
+#pragma unroll
+    for (uint32_t bit = 0; bit < 3; ++bit) {
+        const uint64_t m = __ballot((cnt >> bit) & 1u);
+        pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))
+               << bit;
+        total += (uint32_t)__builtin_popcountll(m) << bit;
+    }
+    for (uint32_t base = 0; base < total; base += 64) {  // wave-uniform
+        uint32_t slot = pre;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            if ((pend >> j) & 1u) {
+                const uint32_t e = slot - base;  // wraps past 64 when slot < base
+                if (e < 64) {
+                    q.pos[e] = i0 + j;
+                    q.row[e] = row;
+                }
+                ++slot;
+            }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < total - base) {
+            float r[3];
+            proposal_rest(C, q.pos[lane], q.row[lane], t, colour, r);
+            q.p[0][lane] = r[0];
+            q.p[1][lane] = r[1];
+            q.p[2][lane] = r[2];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        slot = pre;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            if ((pend >> j) & 1u) {
+                const uint32_t e = slot - base;
+                if (e < 64) {
+                    p[j][0] = q.p[0][e];
+                    p[j][1] = q.p[1][e];
+                    p[j][2] = q.p[2][e];
+                }
+                ++slot;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    if (ok) {
-        const float root = sqrt_rn(fs(1.0f, pick.q));
-        p[0] = fm(fm(2.0f, pick.x1), root);
-        p[1] = fm(fm(2.0f, pick.x2), root);
-        p[2] = fs(1.0f, fm(2.0f, pick.q));
-    } else {
-        p[0] = 0.0f;
-        p[1] = 0.0f;
-        p[2] = 1.0f;
-    }
-    return u;
 }
 
 // Flip proposals (-s) take their acceptance uniform from word (i & 3) of one
@@ -144,21 +217,21 @@ __device__ __forceinline__ void heis_update(const ChainDev& C, uint32_t colour, 
     if (colour == 1) acc[0] += fix27(dot3(n[0], n[1], n[2], h[0], h[1], h[2]));
 }
 
-// The site update of either proposal kind.
-__device__ __forceinline__ void heis_site(const ChainDev& C, uint32_t colour, uint32_t pos,
-                                          uint32_t row, uint64_t t, const U4& fw, uint32_t j,
-                                          const float o[3], const float h[3], float n[3],
-                                          int64_t acc[5]) {
-    float p[3], u;
+// Phase 1 of a site: its acceptance uniform u and proposal p.  Random
+// proposals whose first call missed set bit j of pend (resolved by
+// heis_resolve); flip proposals take word j of fw.
+__device__ __forceinline__ void heis_propose(const ChainDev& C, uint32_t colour, uint32_t pos,
+                                             uint32_t row, uint64_t t, const U4& fw, uint32_t j,
+                                             const float o[3], float& u, float p[3],
+                                             uint32_t& pend) {
     if (C.proposal) {
-        u = random_proposal(C, pos, row, t, colour, p);
+        if (!proposal_first(C, pos, row, t, colour, u, p)) pend |= 1u << j;
     } else {
         u = unit24(u4_word(fw, j));
         p[0] = -o[0];
         p[1] = -o[1];
         p[2] = -o[2];
     }
-    heis_update(C, colour, o, h, u, p, n, acc);
 }
 
 // Workgroup sums of the five quantities -> partials[sweep][colour][block][5].

@@ -11,7 +11,11 @@
 // Work item: 4 consecutive positions of one row (one aligned 16-byte load per
 // component per row touched).  Random proposals draw from each site's own
 // Philox calls (counter x = i), so the four sites are independent; flip
-// proposals take word i & 3 of one call per item (counter x = i / 4).
+// proposals take word i & 3 of one call per item (counter x = i / 4).  The
+// item runs in three phases -- fields and first proposal calls, the wave's
+// queue of sites whose first call missed, then the updates -- so the rare
+// further calls cost one wave-wide call per round of 64 such sites instead
+// of one per site.
 // All arithmetic is single IEEE fp32 operations in a fixed order (no
 // contraction), so the CPU oracle reproduces every bit.
 #include "heis_common.hpp"
@@ -24,23 +28,36 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
                                                     int64_t* __restrict__ partials,
                                                     uint32_t colour, uint32_t s,
                                                     uint32_t total_blocks, uint32_t grid) {
+    __shared__ HeisQueue hq[kBlock / 64];
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
     if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
+    const bool active = local < C.items;
+    const uint64_t t = C.sweep0 + s;
     int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
 
-    if (local < C.items) {
-        const uint32_t row = local / C.segs;
+    // Phase 1: neighbour fields, old spins and proposals of the item's sites.
+    uint32_t row = 0, i0 = 0, nvalid = 0, pend = 0;
+    float* own = arena;
+    float os[4][3], hs[4][3], ps[4][3], us[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        us[j] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) os[j][c] = hs[j][c] = ps[j][c] = 0.0f;
+    }
+    if (active) {
+        row = local / C.segs;
         const uint32_t seg = local - row * C.segs;
         const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
         const uint64_t plane = C.plane;
         const uint32_t y = row % ly, z = row / ly;
         const uint32_t q = (colour + y + z) & 1u;
-        const uint32_t i0 = seg * kHeisSeg;
-        const uint32_t nvalid = min(kHeisSeg, w - i0);
-        float* own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
+        i0 = seg * kHeisSeg;
+        nvalid = min(kHeisSeg, w - i0);
+        own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
         const float* oth = arena + C.off[colour ^ 1u];
         const float* orow = oth + (uint64_t)row * pitch;
 
@@ -82,45 +99,49 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
             for (int c = 0; c < 3; ++c) wrap0[c] = orow[c * plane];
         }
 
-        const uint64_t t = C.sweep0 + s;
         U4 fw = U4{0u, 0u, 0u, 0u};
         if (!C.proposal) fw = flip_words(C, i0, row, t, colour);
-        float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
-            const float ox = f4get(so[0], j), oy = f4get(so[1], j), oz = f4get(so[2], j);
-            ns[0][j] = ox;
-            ns[1][j] = oy;
-            ns[2][j] = oz;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) os[j][c] = f4get(so[c], j);
             if (j >= nvalid) continue;
             const uint32_t pos = i0 + j;
             // x neighbours, -x then +x: positions pos-1, pos (q = 0) or
             // pos, pos+1 (q = 1) of the other plane's row.
-            float nmx[3], npx[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float here = f4get(mid[c], j);
+                float nmx, npx;
                 if (q == 0) {
-                    nmx[c] = j == 0 ? xe[c] : f4get(mid[c], j - 1);
-                    npx[c] = here;
+                    nmx = j == 0 ? xe[c] : f4get(mid[c], j - 1);
+                    npx = here;
                 } else {
-                    nmx[c] = here;
-                    npx[c] = pos + 1 == w ? wrap0[c] : (j == 3 ? xe[c] : f4get(mid[c], j + 1));
+                    nmx = here;
+                    npx = pos + 1 == w ? wrap0[c] : (j == 3 ? xe[c] : f4get(mid[c], j + 1));
                 }
-            }
-            float h[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float v = fa(nmx[c], npx[c]);
+                float v = fa(nmx, npx);
                 v = fa(v, f4get(ym[c], j));
                 v = fa(v, f4get(yp[c], j));
                 v = fa(v, f4get(zm[c], j));
                 v = fa(v, f4get(zp[c], j));
-                h[c] = v;
+                hs[j][c] = v;
             }
-            const float o[3] = {ox, oy, oz};
-            float n[3];
-            heis_site(C, colour, pos, row, t, fw, j, o, h, n, acc);
+            heis_propose(C, colour, pos, row, t, fw, j, os[j], us[j], ps[j], pend);
+        }
+    }
+
+    // Phase 2: the wave's sites whose first proposal call missed.
+    if (C.proposal) heis_resolve(C, hq[threadIdx.x >> 6], pend, i0, row, t, colour, ps);
+
+    // Phase 3: accept or reject, store, sums.
+    if (active) {
+        const uint64_t plane = C.plane;
+        float ns[3][4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            float n[3] = {os[j][0], os[j][1], os[j][2]};
+            if (j < nvalid) heis_update(C, colour, os[j], hs[j], us[j], ps[j], n, acc);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];

@@ -1,6 +1,7 @@
 // Errors of the host library, mirroring src/error.rs of the reference:
 // ProgramError kinds (error.rs:30-46), VegasError::NotImplementedError
-// (error.rs:25-26), plus the device failures of the GPU backend.  Raised as
+// (error.rs:25-26), IoError (error.rs:48-56), plus the device failures of the
+// GPU backend.  Raised as
 // exceptions (the C++ counterpart of the reference's Result<_, Error>).
 #pragma once
 
@@ -24,6 +25,9 @@ enum class ErrorKind {
     Device = VG_ERR_DEVICE,
     OutOfMemory = VG_ERR_OUT_OF_MEMORY,
     NoDevice = VG_ERR_NO_DEVICE,
+    // Host-only: files of the output tables (IoError, error.rs:48-56).  The
+    // C ABI does no file I/O, so no status code maps here.
+    Io = 1000,
 };
 
 // Messages of error.rs:32-43.
@@ -37,6 +41,7 @@ inline const char* error_message(ErrorKind k) {
         case ErrorKind::ZeroField: return "maximum field must be greater than zero";
         case ErrorKind::ZeroFieldStep: return "field step must be greater than zero";
         case ErrorKind::NotImplemented: return "not implemented error";
+        case ErrorKind::Io: return "io error";
         default: return "device error";
     }
 }

@@ -15,7 +15,10 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "accumulator.hpp"
+#include "parquet.hpp"
 #include "thermostat.hpp"
 
 namespace vegas {
@@ -98,10 +101,16 @@ struct ObservableRow {
 };
 
 // ObservableSensor (instrument.rs:145-263): per sweep of every relax and
-// measure phase, a row; the stage counter grows by one per phase.
+// measure phase, a row; the stage counter grows by one per phase.  Built
+// with a path, it writes each phase's rows to that parquet file at the
+// phase's end (instrument.rs:196-205, 227-236) and finishes the file when
+// closed or destroyed; built without, it keeps every row in memory.
 template <class H, class S>
 class ObservableSensor : public Instrument<H, S> {
   public:
+    ObservableSensor() = default;
+    explicit ObservableSensor(const std::string& parquet_path)
+        : io_(std::make_unique<ParquetWriter>(parquet_path, observable_schema())) {}
     void on_relax_start(const Thermostat<S>& th, const H&, size_t n) override { begin(th, n, true); }
     void on_relax_end() override { end(); }
     void on_measure_start(const Thermostat<S>& th, const H&, size_t n) override {
@@ -113,14 +122,19 @@ class ObservableSensor : public Instrument<H, S> {
         rows_.push_back(ObservableRow{relax_, stage_, step_++, n_, temperature_, field_, r.energy,
                                       r.magnetization});
     }
+    // Every row (no path), or the rows of the phase in progress (path).
     const std::vector<ObservableRow>& rows() const { return rows_; }
+    // Finishes the parquet file (no-op without a path).
+    void close() {
+        if (io_) io_->close();
+    }
     // Writes the rows as CSV (header = the parquet column names) to
     // `path`.tmp and renames it into place, as the parquet writer does on
     // close (output.rs:101-113).
     void write_csv(const std::string& path) const {
         const std::string tmp = path + ".tmp";
         FILE* f = std::fopen(tmp.c_str(), "w");
-        if (!f) throw Error(ErrorKind::InvalidArgument, "cannot open " + tmp);
+        if (!f) throw Error(ErrorKind::Io, "cannot open " + tmp);
         std::fprintf(f, "relax,stage,step,n,temperature,field,energy,magnetization\n");
         for (const auto& r : rows_)
             std::fprintf(f, "%s,%llu,%llu,%llu,%.17g,%.17g,%.17g,%.17g\n", r.relax ? "true" : "false",
@@ -128,7 +142,7 @@ class ObservableSensor : public Instrument<H, S> {
                          (unsigned long long)r.n, r.temperature, r.field, r.energy, r.magnetization);
         std::fclose(f);
         if (std::rename(tmp.c_str(), path.c_str()) != 0)
-            throw Error(ErrorKind::InvalidArgument, "cannot rename " + tmp);
+            throw Error(ErrorKind::Io, "cannot rename " + tmp);
     }
 
   private:
@@ -141,9 +155,31 @@ class ObservableSensor : public Instrument<H, S> {
         step_ = 0;
     }
     void end() {
+        if (io_ && !rows_.empty()) {
+            // one row group per phase, as the reference's write() per phase
+            const size_t m = rows_.size();
+            std::vector<uint8_t> relax(m);
+            std::vector<uint64_t> stage(m), step(m), n(m);
+            std::vector<double> t(m), h(m), e(m), mag(m);
+            for (size_t i = 0; i < m; ++i) {
+                const ObservableRow& r = rows_[i];
+                relax[i] = r.relax;
+                stage[i] = r.stage;
+                step[i] = r.step;
+                n[i] = r.n;
+                t[i] = r.temperature;
+                h[i] = r.field;
+                e[i] = r.energy;
+                mag[i] = r.magnetization;
+            }
+            io_->write(m, {relax.data(), stage.data(), step.data(), n.data(), t.data(), h.data(),
+                           e.data(), mag.data()});
+            rows_.clear();
+        }
         active_ = false;
         ++stage_;
     }
+    std::unique_ptr<ParquetWriter> io_;
     std::vector<ObservableRow> rows_;
     bool active_ = false, relax_ = false;
     uint64_t stage_ = 0, step_ = 0, n_ = 0;
@@ -151,7 +187,9 @@ class ObservableSensor : public Instrument<H, S> {
 };
 
 // StateSensor (instrument.rs:265-351): the spins every `frequency` sweeps of
-// each phase, kept as (relax, stage, step, temperature, field, state).
+// each phase.  Built with a path, each snapshot becomes one row per site of
+// that parquet file (relax, stage, step, temperature, field, id, sx, sy, sz;
+// output.rs:155-166); built without, snapshots are kept in memory.
 template <class H, class S>
 class StateSensor : public Instrument<H, S> {
   public:
@@ -164,6 +202,9 @@ class StateSensor : public Instrument<H, S> {
     explicit StateSensor(size_t frequency) : frequency_(frequency) {
         if (frequency == 0) throw Error(ErrorKind::InvalidArgument, "frequency must be positive");
     }
+    StateSensor(const std::string& parquet_path, size_t frequency) : StateSensor(frequency) {
+        io_ = std::make_unique<ParquetWriter>(parquet_path, state_schema());
+    }
     void on_relax_start(const Thermostat<S>& th, const H&, size_t) override { begin(th, true); }
     void on_relax_end() override { end(); }
     void on_measure_start(const Thermostat<S>& th, const H&, size_t) override { begin(th, false); }
@@ -171,11 +212,33 @@ class StateSensor : public Instrument<H, S> {
     void after_step(const SweepRecord&) override { ++step_; }
     size_t state_frequency() const override { return frequency_; }
     void on_state(const State<S>& st) override {
-        if (active_) snapshots_.push_back(Snapshot{relax_, stage_, step_, temperature_, field_, st});
+        if (!active_) return;
+        if (io_)
+            write(st);
+        else
+            snapshots_.push_back(Snapshot{relax_, stage_, step_, temperature_, field_, st});
     }
     const std::vector<Snapshot>& snapshots() const { return snapshots_; }
+    void close() {
+        if (io_) io_->close();
+    }
 
   private:
+    void write(const State<S>& st) {
+        const size_t m = st.len();
+        std::vector<uint8_t> relax(m, relax_ ? 1 : 0);
+        std::vector<uint64_t> stage(m, stage_), step(m, step_), id(m);
+        std::vector<double> t(m, temperature_), h(m, field_), sx(m), sy(m), sz(m);
+        for (size_t i = 0; i < m; ++i) {
+            const S& s = st.at(i);
+            id[i] = i;
+            sx[i] = s.sx();
+            sy[i] = s.sy();
+            sz[i] = s.sz();
+        }
+        io_->write(m, {relax.data(), stage.data(), step.data(), t.data(), h.data(), id.data(),
+                       sx.data(), sy.data(), sz.data()});
+    }
     void begin(const Thermostat<S>& th, bool relax) {
         active_ = true;
         relax_ = relax;
@@ -189,6 +252,7 @@ class StateSensor : public Instrument<H, S> {
         ++stage_;
     }
     size_t frequency_;
+    std::unique_ptr<ParquetWriter> io_;
     std::vector<Snapshot> snapshots_;
     bool active_ = false, relax_ = false;
     uint64_t stage_ = 0, step_ = 0;
