@@ -64,19 +64,23 @@ typedef enum vg_status {
 /*        lx must be even; a periodic axis must have even length.       */
 /*   bcc: site id = 2*(x + lx*(y + ly*z)) + sub; sub 0 = cell corner,   */
 /*        sub 1 = body centre; 8 neighbours on the other sublattice.    */
+/*   fcc: site id = 4*(x + lx*(y + ly*z)) + sub; sub 0 = corner,        */
+/*        subs 1, 2, 3 = the face centres (0,1/2,1/2), (1/2,0,1/2),     */
+/*        (1/2,1/2,0); 12 neighbours, 4 on each other sublattice.       */
 /* A periodic axis wraps; an open axis (drop_x/y/z) has no bonds across */
-/* its ends.  Both lattices are bipartite, which the two-colour sweep   */
-/* needs.                                                               */
+/* its ends.  sc and bcc are bipartite (a sweep is two colour passes);  */
+/* fcc is not, and a sweep is four passes, one per sublattice.          */
 /* ------------------------------------------------------------------ */
 #define VG_LATTICE_SC 0
 #define VG_LATTICE_BCC 1
+#define VG_LATTICE_FCC 2
 
 #define VG_PBC_X 1u
 #define VG_PBC_Y 2u
 #define VG_PBC_Z 4u
 
 typedef struct vg_lattice {
-    int32_t kind;      /* VG_LATTICE_SC or VG_LATTICE_BCC */
+    int32_t kind;      /* VG_LATTICE_SC, VG_LATTICE_BCC or VG_LATTICE_FCC */
     uint32_t lx, ly, lz; /* unit cells along x, y, z (Lattice::expand) */
     uint32_t pbc;      /* VG_PBC_* bits of the periodic axes */
 } vg_lattice;

@@ -45,8 +45,10 @@ static void draw(uint64_t seed, uint32_t x, uint32_t y, uint32_t z, uint32_t w, 
 }
 
 /* counter word w: colour | purpose << 1 | (sweep >> 32) << 8 */
+/* counter word w: bit 0 colour & 1, bits 1-6 purpose, bit 7 colour >> 1,
+ * bits 8-31 sweep >> 32 */
 static uint32_t wword(uint64_t sweep, uint32_t purpose, uint32_t colour) {
-    return (uint32_t)((sweep >> 32) << 8) | (purpose << 1) | colour;
+    return (uint32_t)((sweep >> 32) << 8) | ((colour >> 1) << 7) | (purpose << 1) | (colour & 1u);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -54,10 +56,18 @@ static uint32_t wword(uint64_t sweep, uint32_t purpose, uint32_t colour) {
 /* ------------------------------------------------------------------------ */
 uint64_t or_num_sites(const or_lattice* L) {
     uint64_t cells = (uint64_t)L->lx * L->ly * L->lz;
-    return L->kind == 0 ? cells : 2 * cells;
+    return L->kind == 0 ? cells : (L->kind == 1 ? 2 * cells : 4 * cells);
 }
 
-int or_coordination(const or_lattice* L) { return L->kind == 0 ? 6 : 8; }
+int or_coordination(const or_lattice* L) { return L->kind == 0 ? 6 : (L->kind == 1 ? 8 : 12); }
+
+/* colours of the checkerboard: sc and bcc are bipartite, fcc takes its four
+ * sublattices */
+static uint32_t ncolours(const or_lattice* L) { return L->kind == 2 ? 4u : 2u; }
+
+/* fcc sublattice positions in half cells: corner, then the face centres
+ * (0,1/2,1/2), (1/2,0,1/2), (1/2,1/2,0) */
+static const int kFccSub[4][3] = {{0, 0, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
 
 /* coordinate a+d along an axis of length n, or -1 when it leaves an open axis */
 static int64_t step_axis(int64_t a, int64_t d, int64_t n, int periodic) {
@@ -79,6 +89,40 @@ void or_neighbours(const or_lattice* L, uint64_t site, int64_t* slots) {
             int64_t b = step_axis(y, d[k][1], L->ly, py);
             int64_t c = step_axis(z, d[k][2], L->lz, pz);
             slots[k] = (a < 0 || b < 0 || c < 0) ? -1 : a + (int64_t)L->lx * (b + (int64_t)L->ly * c);
+        }
+    } else if (L->kind == 2) {
+        /* fcc: a site of sublattice a in cell c bonds to 4 sites of every
+         * other sublattice b, in cells c + d1*e1 + d2*e2 over the two axes
+         * e1 < e2 where a and b differ, d1, d2 in {0,1}, with e = -1 along
+         * an axis where b sits higher than a and +1 where lower.  Slots: b
+         * ascending, then (d1, d2) = (0,0), (1,0), (0,1), (1,1). */
+        uint64_t cell = site >> 2;
+        int a = (int)(site & 3);
+        int64_t xyz[3] = {(int64_t)(cell % L->lx), (int64_t)((cell / L->lx) % L->ly),
+                          (int64_t)(cell / ((uint64_t)L->lx * L->ly))};
+        const int64_t dims[3] = {L->lx, L->ly, L->lz};
+        const int per[3] = {px, py, pz};
+        int k = 0;
+        for (int b = 0; b < 4; ++b) {
+            if (b == a) continue;
+            int ax[2], m = 0;
+            for (int e = 0; e < 3; ++e)
+                if (kFccSub[a][e] != kFccSub[b][e]) ax[m++] = e;
+            for (int d2 = 0; d2 < 2; ++d2)
+                for (int d1 = 0; d1 < 2; ++d1) {
+                    int64_t q[3];
+                    int ok = 1;
+                    for (int e = 0; e < 3; ++e) {
+                        int64_t d = 0;
+                        if (e == ax[0]) d = d1;
+                        if (e == ax[1]) d = d2;
+                        if (kFccSub[b][e] > kFccSub[a][e]) d = -d;
+                        q[e] = step_axis(xyz[e], d, dims[e], per[e]);
+                        if (q[e] < 0) ok = 0;
+                    }
+                    slots[k++] = ok ? 4 * (q[0] + (int64_t)L->lx * (q[1] + (int64_t)L->ly * q[2])) + b
+                                    : -1;
+                }
         }
     } else {
         /* bcc: sublattice 0 (corner of cell x,y,z) neighbours the body centres
@@ -113,11 +157,12 @@ void or_place(const or_lattice* L, uint64_t site, uint32_t* colour, uint32_t* ro
         *row = y + L->ly * z;
         *pos = x / 2;
     } else {
-        uint64_t cell = site >> 1;
+        const int sh = L->kind == 2 ? 2 : 1;
+        uint64_t cell = site >> sh;
         uint32_t x = (uint32_t)(cell % L->lx);
         uint32_t y = (uint32_t)((cell / L->lx) % L->ly);
         uint32_t z = (uint32_t)(cell / ((uint64_t)L->lx * L->ly));
-        *colour = (uint32_t)(site & 1);
+        *colour = (uint32_t)(site & ((1u << sh) - 1u));
         *row = y + L->ly * z;
         *pos = x;
     }
@@ -130,7 +175,7 @@ static uint64_t site_of(const or_lattice* L, uint32_t colour, uint32_t row, uint
         uint32_t x = 2 * pos + ((colour + y + z) & 1u);
         return x + (uint64_t)L->lx * (y + (uint64_t)L->ly * z);
     }
-    return 2 * (pos + (uint64_t)L->lx * (y + (uint64_t)L->ly * z)) + colour;
+    return ncolours(L) * (pos + (uint64_t)L->lx * (y + (uint64_t)L->ly * z)) + colour;
 }
 
 static uint32_t row_length(const or_lattice* L) { return L->kind == 0 ? L->lx / 2 : L->lx; }
@@ -147,8 +192,8 @@ void or_ising_table(const or_lattice* L, const or_model* M, uint32_t* thr, uint3
     for (int b = 0; b < 2; ++b) {
         always[b] = 0;
         double s = b ? 1.0 : -1.0;
-        for (int S = 0; S < 17; ++S) {
-            thr[b * 17 + S] = 0;
+        for (int S = 0; S < OR_THR_STRIDE; ++S) {
+            thr[b * OR_THR_STRIDE + S] = 0;
             if (S > 2 * z) continue;
             double sigma = (double)(S - z);
             /* flip s -> -s: exchange term 2 J s sigma, Zeeman term -2 |h| hz s */
@@ -162,7 +207,7 @@ void or_ising_table(const or_lattice* L, const or_model* M, uint32_t* thr, uint3
             if (th >= 4294967296.0)
                 always[b] |= 1u << S;
             else
-                thr[b * 17 + S] = (uint32_t)th;
+                thr[b * OR_THR_STRIDE + S] = (uint32_t)th;
         }
     }
 }
@@ -181,12 +226,12 @@ void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint
                      uint64_t nsweeps, int8_t* spins, int64_t* sums) {
     const uint64_t n = or_num_sites(L);
     const int z = or_coordination(L);
-    uint32_t thr[34], always[2];
+    uint32_t thr[2 * OR_THR_STRIDE], always[2];
     or_ising_table(L, M, thr, always);
-    int64_t nb[8];
+    int64_t nb[12];
     for (uint64_t k = 0; k < nsweeps; ++k) {
         const uint64_t t = sweep0 + k;
-        for (uint32_t c = 0; c < 2; ++c) {
+        for (uint32_t c = 0; c < ncolours(L); ++c) {
             for (uint64_t site = 0; site < n; ++site) {
                 uint32_t cc, row, pos;
                 or_place(L, site, &cc, &row, &pos);
@@ -211,21 +256,26 @@ void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint
                     uint32_t hi8 = (ra[(pos & 15) >> 2] >> (8 * (pos & 3))) & 0xFFu;
                     uint32_t lo24 = rb[pos & 3] >> 8;
                     uint32_t draw32 = (hi8 << 24) | lo24;
-                    flip = draw32 < thr[b * 17 + S];
+                    flip = draw32 < thr[b * OR_THR_STRIDE + S];
                 }
                 if (flip) spins[site] = (int8_t)-spins[site];
             }
         }
         if (sums) {
             int64_t bond = 0, mag = 0;
+            /* every bond once: from each site to its lower-colour neighbours */
             for (uint64_t site = 0; site < n; ++site) {
                 mag += spins[site];
                 uint32_t cc, row, pos;
                 or_place(L, site, &cc, &row, &pos);
-                if (cc != 1) continue;
+                if (cc == 0) continue;
                 or_neighbours(L, site, nb);
-                for (int j = 0; j < z; ++j)
-                    if (nb[j] >= 0) bond += (int64_t)spins[site] * spins[nb[j]];
+                for (int j = 0; j < z; ++j) {
+                    if (nb[j] < 0) continue;
+                    uint32_t cn, rn, pn;
+                    or_place(L, (uint64_t)nb[j], &cn, &rn, &pn);
+                    if (cn < cc) bond += (int64_t)spins[site] * spins[nb[j]];
+                }
             }
             int64_t* o = sums + 5 * k;
             o[0] = bond;
@@ -368,11 +418,11 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
     const float beta = (float)(1.0 / T);
     const float axis[3] = {(float)M->axis[0], (float)M->axis[1], (float)M->axis[2]};
     const float fdir[3] = {(float)M->field_dir[0], (float)M->field_dir[1], (float)M->field_dir[2]};
-    int64_t nb[8];
+    int64_t nb[12];
     for (uint64_t k = 0; k < nsweeps; ++k) {
         const uint64_t t = sweep0 + k;
         int64_t acc[5] = {0, 0, 0, 0, 0};
-        for (uint32_t c = 0; c < 2; ++c) {
+        for (uint32_t c = 0; c < ncolours(L); ++c) {
             for (uint32_t row = 0; row < rows; ++row) {
                 for (uint32_t g = 0; 4 * g < w; ++g) {
                     /* flip proposals: word (pos & 3) of one call per 4 positions */
@@ -382,13 +432,21 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
                         const uint64_t site = site_of(L, c, row, pos);
                         float* s = xyz + 3 * site;
                         or_neighbours(L, site, nb);
-                        float h[3];
+                        /* h: slots summed left to right; hb: the same sum
+                         * over the slots of lower colours (the first
+                         * z/ncol*c slots on fcc; every slot of a colour-1
+                         * site on sc and bcc), for the bond sum */
+                        const int nlow = L->kind == 2 ? 4 * (int)c : (c == 1 ? z : 0);
+                        float h[3], hb[3] = {0.0f, 0.0f, 0.0f};
                         for (int a = 0; a < 3; ++a) {
                             float v0 = nb[0] < 0 ? 0.0f : xyz[3 * nb[0] + a];
                             float v1 = nb[1] < 0 ? 0.0f : xyz[3 * nb[1] + a];
                             float acc_h = v0 + v1;
-                            for (int j = 2; j < z; ++j)
+                            for (int j = 2; j < z; ++j) {
+                                if (j == nlow) hb[a] = acc_h;
                                 acc_h = acc_h + (nb[j] < 0 ? 0.0f : xyz[3 * nb[j] + a]);
+                            }
+                            if (nlow == z) hb[a] = acc_h;
                             h[a] = acc_h;
                         }
                         float p[3] = {-s[0], -s[1], -s[2]};
@@ -416,7 +474,7 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
                         acc[3] += fix27(s[2]);
                         const float se = dot3f(s, axis);
                         acc[4] += fix27(se * se);
-                        if (c == 1) acc[0] += fix27(dot3f(s, h));
+                        if (nlow > 0) acc[0] += fix27(dot3f(s, hb));
                     }
                 }
             }
@@ -509,7 +567,7 @@ or_seq* or_seq_create(const or_lattice* L, const or_model* M, uint64_t seed) {
     s->indices = (uint64_t*)malloc(s->n * z * sizeof(uint64_t));
     s->values = (double*)malloc(s->n * z * sizeof(double));
     uint64_t nnz = 0;
-    int64_t nb[8];
+    int64_t nb[12];
     for (uint64_t i = 0; i < s->n; ++i) {
         s->indptr[i] = nnz;
         or_neighbours(L, i, nb);

@@ -41,7 +41,7 @@ extern "C" {
 #endif
 
 typedef struct or_lattice {
-    int32_t kind; /* 0 sc, 1 bcc */
+    int32_t kind; /* 0 sc, 1 bcc, 2 fcc */
     uint32_t lx, ly, lz;
     uint32_t pbc; /* bit 0 x, bit 1 y, bit 2 z */
 } or_lattice;
@@ -59,14 +59,16 @@ typedef struct or_model {
 void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 /* Lattice: number of sites and the neighbour slots of a site in slot order
- * (sc: -x,+x,-y,+y,-z,+z; bcc: 8 slots), -1 for an absent neighbour. */
+ * (sc: -x,+x,-y,+y,-z,+z; bcc: 8 slots; fcc: 12 slots, 4 per other
+ * sublattice in ascending order), -1 for an absent neighbour. */
 uint64_t or_num_sites(const or_lattice* L);
 int or_coordination(const or_lattice* L);
 void or_neighbours(const or_lattice* L, uint64_t site, int64_t* slots);
 /* colour, row and position of a site in the colour-split planes */
 void or_place(const or_lattice* L, uint64_t site, uint32_t* colour, uint32_t* row, uint32_t* pos);
 
-/* Ising acceptance table: thr[b*17+S], always[b] bit S. */
+/* Ising acceptance table: thr[b*OR_THR_STRIDE+S], always[b] bit S, S in [0, 2z]. */
+#define OR_THR_STRIDE 25
 void or_ising_table(const or_lattice* L, const or_model* M, uint32_t* thr, uint32_t* always);
 
 /* Checkerboard semantics.  spins: natural order, int8 +-1 (Ising) or

@@ -89,9 +89,12 @@ def _ptr(a: np.ndarray) -> C.c_void_p:
     return C.c_void_p(a.ctypes.data)
 
 
+THR_STRIDE = 25  # classes per current bit: S in [0, 2z], z <= 12
+
+
 def lattice(kind="sc", lx=4, ly=4, lz=4, pbc=(True, True, True)) -> Lattice:
     bits = (1 if pbc[0] else 0) | (2 if pbc[1] else 0) | (4 if pbc[2] else 0)
-    return Lattice(0 if kind == "sc" else 1, lx, ly, lz, bits)
+    return Lattice({"sc": 0, "bcc": 1, "fcc": 2}[kind], lx, ly, lz, bits)
 
 
 def model(spin="ising", proposal="flip", terms=("exchange", "zeeman"), exchange=1.0,
@@ -119,7 +122,7 @@ def num_sites(L: Lattice) -> int:
 
 def neighbours(L: Lattice, site: int) -> list:
     z = lib().or_coordination(C.byref(L))
-    out = (C.c_int64 * 8)()
+    out = (C.c_int64 * 12)()
     lib().or_neighbours(C.byref(L), site, out)
     return list(out)[:z]
 
@@ -131,10 +134,10 @@ def place(L: Lattice, site: int) -> tuple:
 
 
 def ising_table(L: Lattice, M: Model):
-    thr = (C.c_uint32 * 34)()
+    thr = (C.c_uint32 * (2 * THR_STRIDE))()
     alw = (C.c_uint32 * 2)()
     lib().or_ising_table(C.byref(L), C.byref(M), thr, alw)
-    return np.array(list(thr), dtype=np.uint64).reshape(2, 17), list(alw)
+    return np.array(list(thr), dtype=np.uint64).reshape(2, THR_STRIDE), list(alw)
 
 
 def ising_init(L: Lattice, seed: int) -> np.ndarray:

@@ -128,8 +128,36 @@ static void exchange_ground_states() {
     const auto ex = Exchange::from_lattice(1.0, Lattice::sc().expand_all(6));
     EXPECT(close(ex.total_energy(th, State<IsingSpin>::down_with_size(216)) / 216.0, -3.0));
     EXPECT(Lattice::bcc().expand_all(3).num_sites() == 54);
-    EXPECT(throws_kind([] { Lattice::fcc().expand_all(2).descriptor(); },
-                       ErrorKind::UnsupportedLattice));
+    // fcc: 12 bonds per site -> E/N = -6 J for aligned spins (full PBC);
+    // open z drops the bonds across its ends.
+    const Lattice f = Lattice::fcc().expand_all(3);
+    EXPECT(f.num_sites() == 108 && f.descriptor().kind == VG_LATTICE_FCC);
+    const auto exf = Exchange::from_lattice(1.0, f);
+    EXPECT(close(exf.total_energy(th, State<IsingSpin>::up_with_size(108)) / 108.0, -6.0));
+    const Lattice fo = Lattice::fcc().expand(3, 3, 4).drop_z();
+    const auto exo = Exchange::from_lattice(1.0, fo);
+    uint64_t bonds = 0;
+    for (uint64_t i = 0; i < fo.num_sites(); ++i)
+        for (int64_t j : fo.neighbours(i)) bonds += j >= 0 ? 1 : 0;
+    EXPECT(close(exo.total_energy(th, State<IsingSpin>::up_with_size(fo.num_sites())),
+                 -(double)bonds / 2.0));
+    // Of the 4 bonds per cell of each sublattice pair, the 2 with a z step
+    // cross a z plane when the pair differs along z (4 of the 6 pairs): 8
+    // bonds per cell of the 3x3 layer are lost at the open end.
+    EXPECT(bonds / 2 == 6 * fo.num_sites() - 8 * 9);
+    // the host's fcc neighbours are the oracle's, slot for slot
+    bool same = true;
+    for (const Lattice& l : {f, fo, Lattice::fcc().expand(2, 3, 1).drop_x()}) {
+        const vg_lattice d = l.descriptor();
+        or_lattice ol{d.kind, d.lx, d.ly, d.lz, d.pbc};
+        for (uint64_t i = 0; i < l.num_sites(); ++i) {
+            int64_t o[12];
+            or_neighbours(&ol, i, o);
+            const auto h = l.neighbours(i);
+            for (int k = 0; k < 12; ++k) same = same && h[k] == o[k];
+        }
+    }
+    EXPECT(same);
 }
 
 static void accumulator_moments() {

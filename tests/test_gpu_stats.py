@@ -40,8 +40,17 @@ CASES = {
                                ("exchange", "zeeman"), 0.0, 64, 6000, 60000),
     "heis_bcc5_T3": ("heisenberg", (5, 5, 5), (1, 1, 1), 3.0, 0.05, "random",
                      ("exchange", "zeeman", "anisotropy"), 0.1, 64, 4000, 30000),
+    # fcc: four colour passes per sweep (T_c of the fcc Ising model is ~9.8 J)
+    "ising_fcc4_T12": ("ising", (4, 4, 4), (1, 1, 1), 12.0, 0.1, "flip",
+                       ("exchange", "zeeman"), 0.0, 64, 6000, 60000),
+    "ising_fcc4x3x3_open_T8_random": ("ising", (4, 3, 3), (0, 1, 1), 8.0, 0.0, "random",
+                                      ("exchange", "zeeman"), 0.0, 64, 6000, 180000),
+    "heis_fcc4_T4": ("heisenberg", (4, 4, 4), (1, 1, 1), 4.0, 0.05, "random",
+                     ("exchange", "zeeman", "anisotropy"), 0.1, 64, 4000, 30000),
 }
 BCC = {"ising_bcc5_T8", "ising_bcc6x4x3_open_T5", "heis_bcc5_T3"}
+FCC = {"ising_fcc4_T12", "ising_fcc4x3x3_open_T8_random", "heis_fcc4_T4"}
+PER_CELL = {"sc": 1, "bcc": 2, "fcc": 4}
 
 
 def _observables(e, m, n, T):
@@ -53,7 +62,8 @@ ORDERED_START = {"ising_sc8_T4_h0.05_random"}
 
 
 def _engine(vg, ctx, kind, spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps, down):
-    L = vg.lattice(vg.BCC if kind == "bcc" else vg.SC, *dims, pbc=tuple(bool(p) for p in pbc))
+    kinds = {"sc": vg.SC, "bcc": vg.BCC, "fcc": vg.FCC}
+    L = vg.lattice(kinds[kind], *dims, pbc=tuple(bool(p) for p in pbc))
     kw = {"zeeman": "zeeman" in terms}
     if "exchange" in terms:
         kw["exchange"] = 1.0
@@ -63,7 +73,7 @@ def _engine(vg, ctx, kind, spin, dims, pbc, T, h, proposal, terms, K, chains, sw
     sk = vg.ISING if spin == "ising" else vg.HEISENBERG
     descs = [vg.chain(L, sk, vg.FLIP if proposal == "flip" else vg.RANDOM, H, s)
              for s in range(chains)]
-    n = int(np.prod(dims)) * (2 if kind == "bcc" else 1)
+    n = int(np.prod(dims)) * PER_CELL[kind]
     with vg.Batch(ctx, descs) as b:
         b.randomize()
         if down:
@@ -103,7 +113,7 @@ def _sequential(orc, kind, spin, dims, pbc, T, h, proposal, terms, K, sweeps, do
 def test_checkerboard_agrees_with_sequential_metropolis(vg, orc, ctx, name):
     spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps, seq_sweeps = CASES[name]
     down = name in ORDERED_START
-    kind = "bcc" if name in BCC else "sc"
+    kind = "bcc" if name in BCC else ("fcc" if name in FCC else "sc")
     gpu = _engine(vg, ctx, kind, spin, dims, pbc, T, h, proposal, terms, K, chains, sweeps,
                   down)
     seq = _sequential(orc, kind, spin, dims, pbc, T, h, proposal, terms, K, seq_sweeps, down)

@@ -34,8 +34,9 @@ def test_pcg64_matches_numpy(orc):
 @pytest.mark.parametrize("kind,dims,pbc", [
     ("sc", (4, 4, 4), (1, 1, 1)), ("sc", (10, 10, 1), (1, 1, 0)), ("sc", (2, 4, 6), (1, 1, 1)),
     ("sc", (6, 3, 5), (1, 0, 0)), ("bcc", (3, 3, 3), (1, 1, 1)), ("bcc", (2, 3, 1), (0, 1, 0)),
+    ("fcc", (3, 3, 3), (1, 1, 1)), ("fcc", (3, 2, 4), (0, 1, 1)), ("fcc", (2, 3, 1), (1, 0, 0)),
 ])
-def test_neighbours_symmetric_and_bipartite(orc, kind, dims, pbc):
+def test_neighbours_symmetric_and_coloured(orc, kind, dims, pbc):
     L = orc.lattice(kind, *dims, pbc=pbc)
     n = orc.num_sites(L)
     counts = {}
@@ -44,7 +45,7 @@ def test_neighbours_symmetric_and_bipartite(orc, kind, dims, pbc):
         for j in orc.neighbours(L, i):
             if j < 0:
                 continue
-            assert orc.place(L, j)[0] != ci  # two-colourable
+            assert orc.place(L, j)[0] != ci  # the colours are independent sets
             counts[(i, j)] = counts.get((i, j), 0) + 1
     for (i, j), c in counts.items():
         assert counts.get((j, i)) == c  # symmetric with multiplicity
@@ -69,10 +70,42 @@ def test_neighbour_counts(orc):
     for i in range(orc.num_sites(L)):
         nb = orc.neighbours(L, i)
         assert len(set(nb)) == 8 and all((j & 1) != (i & 1) for j in nb)
+    # fcc periodic: 12 distinct neighbours, 4 on each other sublattice
+    L = orc.lattice("fcc", 3, 3, 3)
+    for i in range(orc.num_sites(L)):
+        nb = orc.neighbours(L, i)
+        assert len(set(nb)) == 12
+        assert sorted(j & 3 for j in nb) == sorted([b for b in range(4) if b != (i & 3)] * 4)
+
+
+FCC_HALF = [(0, 0, 0), (0, 1, 1), (1, 0, 1), (1, 1, 0)]
+
+
+@pytest.mark.parametrize("dims,pbc", [((3, 3, 3), (1, 1, 1)), ((3, 4, 3), (0, 1, 0)),
+                                      ((4, 3, 5), (1, 0, 1))])
+def test_fcc_neighbours_are_the_geometric_ones(orc, dims, pbc):
+    """fcc neighbours from positions: the sites at distance a/sqrt(2) (the
+    nearest-neighbour distance of the fcc cell), by brute force over all
+    pairs with minimum-image distances along the periodic axes."""
+    L = orc.lattice("fcc", *dims, pbc=pbc)
+    n = orc.num_sites(L)
+    pos = np.zeros((n, 3))
+    for i in range(n):
+        cell, sub = i >> 2, i & 3
+        c = (cell % dims[0], (cell // dims[0]) % dims[1], cell // (dims[0] * dims[1]))
+        pos[i] = [c[e] + FCC_HALF[sub][e] / 2 for e in range(3)]
+    for i in range(n):
+        d = pos - pos[i]
+        for e in range(3):
+            if pbc[e]:
+                d[:, e] -= dims[e] * np.round(d[:, e] / dims[e])
+        geo = set(np.nonzero(np.abs((d * d).sum(axis=1) - 0.5) < 1e-9)[0].tolist())
+        nb = [j for j in orc.neighbours(L, i) if j >= 0]
+        assert len(nb) == len(set(nb)) and set(nb) == geo, i
 
 
 def test_place_round_trip(orc):
-    for kind, dims in [("sc", (6, 4, 2)), ("bcc", (3, 2, 2))]:
+    for kind, dims in [("sc", (6, 4, 2)), ("bcc", (3, 2, 2)), ("fcc", (3, 2, 2))]:
         L = orc.lattice(kind, *dims)
         seen = set()
         for i in range(orc.num_sites(L)):
@@ -139,6 +172,11 @@ def test_ground_state_energies(orc):
     s = orc.Sequential(L, orc.model(terms=("exchange",)), 1)
     s.set_state(np.ones(54, dtype=np.int8))
     assert s.total_energy() / 54 == pytest.approx(-4.0)
+    L = orc.lattice("fcc", 3, 3, 3)  # 12 bonds per site: E/N = -6
+    s = orc.Sequential(L, orc.model(terms=("exchange",)), 1)
+    s.set_state(np.ones(108, dtype=np.int8))
+    assert s.total_energy() / 108 == pytest.approx(-6.0)
+    assert s.nnz() == 108 * 12
     # Neel state: every bond broken
     L = orc.lattice("sc", 4, 4, 4)
     st = np.array([1 if orc.place(L, i)[0] == 0 else -1 for i in range(64)], dtype=np.int8)
@@ -260,6 +298,29 @@ def test_checkerboard_bcc_ising_matches_exact_enumeration(orc, dims, pbc, T, h):
     spins = orc.ising_init(L, 4)
     orc.ising_sweeps(L, M, 13, 0, 2000, spins)
     sums = orc.ising_sweeps(L, M, 13, 2000, 100000, spins)
+    bond, mag = sums[:, 0].astype(float), sums[:, 3].astype(float)
+    e = -2.0 * bond + abs(h) * mag
+    ee, em, _, _ = ising_exact(neigh, n, 1.0, h, 1.0, T)
+    me, se = binned(e)
+    mm, sm = binned(np.abs(mag))
+    assert abs(me - ee) < 4 * se, (me, ee, se)
+    assert abs(mm - em) < 4 * sm, (mm, em, sm)
+
+
+@pytest.mark.parametrize("dims,pbc,T,h", [((2, 1, 1), (1, 1, 1), 6.0, 0.3),
+                                          ((2, 2, 1), (1, 1, 0), 4.0, 0.0),
+                                          ((1, 2, 2), (0, 1, 0), 2.5, -0.5)])
+def test_checkerboard_fcc_ising_matches_exact_enumeration(orc, dims, pbc, T, h):
+    """fcc (four sublattice colours): the checkerboard oracle's long-run
+    averages equal the exact Boltzmann ones, duplicate bonds of short
+    periodic axes and open boundaries included."""
+    L = orc.lattice("fcc", *dims, pbc=pbc)
+    n = orc.num_sites(L)
+    neigh = [[j for j in orc.neighbours(L, i) if j >= 0] for i in range(n)]
+    M = orc.model(terms=("exchange", "zeeman"), temperature=T, field=h)
+    spins = orc.ising_init(L, 5)
+    orc.ising_sweeps(L, M, 17, 0, 2000, spins)
+    sums = orc.ising_sweeps(L, M, 17, 2000, 100000, spins)
     bond, mag = sums[:, 0].astype(float), sums[:, 3].astype(float)
     e = -2.0 * bond + abs(h) * mag
     ee, em, _, _ = ising_exact(neigh, n, 1.0, h, 1.0, T)

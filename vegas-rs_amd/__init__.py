@@ -20,7 +20,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libvegas_hip.so"
 
 # lattice kinds, pbc bits, terms, spins, proposals (vegas_hip.h)
-SC, BCC = 0, 1
+SC, BCC, FCC = 0, 1, 2
 PBC_X, PBC_Y, PBC_Z = 1, 2, 4
 EXCHANGE, ZEEMAN, ANISOTROPY, GAUGE = 1, 2, 4, 8
 ISING, ISING_BITS, HEISENBERG = 0, 1, 2

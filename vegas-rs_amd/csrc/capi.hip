@@ -19,11 +19,11 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-int coordination(int kind) { return kind == VG_LATTICE_SC ? 6 : 8; }
+int coordination(int kind) { return kind == VG_LATTICE_SC ? 6 : (kind == VG_LATTICE_BCC ? 8 : 12); }
 
 int validate_desc(const vg_chain_desc& d) {
     const vg_lattice& L = d.lattice;
-    if (L.kind != VG_LATTICE_SC && L.kind != VG_LATTICE_BCC)
+    if (L.kind != VG_LATTICE_SC && L.kind != VG_LATTICE_BCC && L.kind != VG_LATTICE_FCC)
         return fail(VG_ERR_UNSUPPORTED_LATTICE, "unknown lattice kind %d", L.kind);
     if (L.lx == 0 || L.ly == 0 || L.lz == 0)
         return fail(VG_ERR_INVALID_ARGUMENT, "lattice dimensions must be positive");
@@ -39,7 +39,7 @@ int validate_desc(const vg_chain_desc& d) {
                 return fail(VG_ERR_UNSUPPORTED_LATTICE,
                             "periodic sc axis %c has odd length %u: not bipartite", "xyz"[a],
                             dims[a]);
-    }  // bcc: the two sublattices are the colours for any sizes and boundaries
+    }  // bcc, fcc: the sublattices are the colours for any sizes and boundaries
     const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
     if ((uint64_t)L.ly * L.lz > 0xFFFFFFFFull || cells > (1ull << 34))
         return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
@@ -76,7 +76,7 @@ void ising_thresholds(const ChainHost& ch, ChainDev& d) {
     for (int b = 0; b < 2; ++b) {
         d.always[b] = 0;
         const double s = b ? 1.0 : -1.0;
-        for (int S = 0; S < 17; ++S) {
+        for (int S = 0; S < (int)vg::kThrS; ++S) {
             d.thr[b][S] = 0;
             if (S > 2 * z) continue;
             const double sigma = (double)(S - z);
@@ -152,6 +152,7 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
     d.kind = (uint32_t)L.kind;
     d.pbc = L.pbc;
     d.z = (uint32_t)coordination(L.kind);
+    d.ncol = colours_of(g);
     d.rows = L.ly * L.lz;
     d.w = L.kind == VG_LATTICE_SC ? L.lx / 2 : L.lx;
     d.proposal = (uint32_t)ch.desc.proposal;
@@ -189,7 +190,7 @@ uint32_t quantity_bytes(int group) { return is_ising(group) ? 4u : 8u; }
 
 void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev* dp,
                        hipStream_t st) {
-    const uint64_t n = 2ull * d.plane;  // one thread per plane byte
+    const uint64_t n = (uint64_t)d.ncol * d.plane;  // one thread per plane byte
     const dim3 grid((uint32_t)((n + 255) / 256));
     if (g == kIsingBits)
         hipLaunchKernelGGL(vg::ising_init_bits, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
@@ -242,7 +243,8 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
     G.tile = tile;
     G.total_blocks = (uint32_t)block_chain.size();
     G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
-    G.partial_bytes_per_sweep = 2ull * G.total_blocks * quantities(g) * quantity_bytes(g);
+    G.partial_bytes_per_sweep =
+        (uint64_t)colours_of(g) * G.total_blocks * quantities(g) * quantity_bytes(g);
     for (auto& kv : B->graphs) quiet(hipGraphExecDestroy(kv.second));
     B->graphs.clear();
     return VG_OK;
@@ -292,7 +294,27 @@ void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     }
 }
 
-// Enqueues the sweeps of one block (nsw sweeps, both colours of every group,
+template <uint32_t A>
+void launch_fcc_colour(const Group& G, hipStream_t st, uint32_t s, bool ising) {
+    const dim3 grid(G.grid), block(vg::kBlock);
+    if (ising)
+        hipLaunchKernelGGL(vg::ising_fcc_pass<A>, grid, block, 0, st, G.d_chains, G.d_block_chain,
+                           (uint8_t*)G.arena, (int32_t*)G.partials, s, G.total_blocks, G.grid);
+    else
+        hipLaunchKernelGGL(vg::heis_fcc_pass<A>, grid, block, 0, st, G.d_chains, G.d_block_chain,
+                           (float*)G.arena, (int64_t*)G.partials, s, G.total_blocks, G.grid);
+}
+
+void launch_fcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s, bool ising) {
+    switch (c) {
+        case 0: launch_fcc_colour<0>(G, st, s, ising); break;
+        case 1: launch_fcc_colour<1>(G, st, s, ising); break;
+        case 2: launch_fcc_colour<2>(G, st, s, ising); break;
+        default: launch_fcc_colour<3>(G, st, s, ising); break;
+    }
+}
+
+// Enqueues the sweeps of one block (nsw sweeps, every colour of every group,
 // then the per-chain reductions) on the batch stream.
 int enqueue_block(vg_batch* B, uint32_t nsw) {
     hipStream_t st = B->ctx->stream;
@@ -303,7 +325,7 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
         for (int g = 0; g < kNumGroups; ++g) {
             Group& G = B->groups[g];
             if (G.members.empty()) continue;
-            for (uint32_t c = 0; c < 2; ++c) {
+            for (uint32_t c = 0; c < colours_of(g); ++c) {
                 const bool timed = B->timing && !B->use_graph;
                 if (timed) {
                     const uint32_t k = B->pass_events_used;
@@ -319,14 +341,18 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                     hipLaunchKernelGGL(vg::ising_bcc_pass, grid, block, 0, st, G.d_chains,
                                        G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials,
                                        c, s, G.total_blocks, G.grid);
+                else if (g == kIsingFcc)
+                    launch_fcc_pass(G, st, c, s, true);
                 else if (g == kHeisSc)
                     hipLaunchKernelGGL(vg::heis_sc_pass, grid, block, 0, st, G.d_chains,
                                        G.d_block_chain, (float*)G.arena, (int64_t*)G.partials,
                                        c, s, G.total_blocks, G.grid);
-                else
+                else if (g == kHeisBcc)
                     hipLaunchKernelGGL(vg::heis_bcc_pass, grid, block, 0, st, G.d_chains,
                                        G.d_block_chain, (float*)G.arena, (int64_t*)G.partials,
                                        c, s, G.total_blocks, G.grid);
+                else
+                    launch_fcc_pass(G, st, c, s, false);
                 if (timed) {
                     VG_HIP(hipEventRecord(B->pass_events[B->pass_events_used + 1], st));
                     B->pass_events_used += 2;
@@ -340,10 +366,12 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
         dim3 grid((uint32_t)G.members.size(), nsw);
         if (is_ising(g))
             hipLaunchKernelGGL(vg::ising_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
-                               (const int32_t*)G.partials, G.total_blocks, nsw, B->d_records);
+                               (const int32_t*)G.partials, G.total_blocks, colours_of(g), nsw,
+                               B->d_records);
         else
             hipLaunchKernelGGL(vg::heis_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
-                               (const int64_t*)G.partials, G.total_blocks, nsw, B->d_records);
+                               (const int64_t*)G.partials, G.total_blocks, colours_of(g), nsw,
+                               B->d_records);
     }
     VG_HIP(hipGetLastError());
     return VG_OK;
@@ -444,9 +472,9 @@ int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
             if (rc) return rc;
         }
         if (B->timing && !B->use_graph) {
-            uint32_t groups = 0;
-            for (const Group& G : B->groups) groups += G.members.empty() ? 0 : 1;
-            const size_t need = (size_t)nsw * groups * 4;
+            size_t need = 0;  // a start/stop pair per colour pass
+            for (int g = 0; g < kNumGroups; ++g)
+                if (!B->groups[g].members.empty()) need += (size_t)nsw * 2 * colours_of(g);
             while (B->pass_events.size() < need) {
                 hipEvent_t e;
                 VG_HIP(hipEventCreate(&e));
@@ -470,7 +498,7 @@ int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
             for (uint32_t sw = 0; sw < nsw; ++sw)
                 for (int g = 0; g < kNumGroups; ++g) {
                     if (B->groups[g].members.empty()) continue;
-                    for (int c = 0; c < 2; ++c, k += 2) {
+                    for (uint32_t c = 0; c < colours_of(g); ++c, k += 2) {
                         float t = 0.0f;
                         VG_HIP(hipEventElapsedTime(&t, B->pass_events[k], B->pass_events[k + 1]));
                         B->pass_ms[g] += t;
@@ -583,8 +611,7 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         ChainHost& ch = B->chains[i];
         ch.desc = descs[i];
         const vg_lattice& L = ch.desc.lattice;
-        const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
-        ch.n_sites = L.kind == VG_LATTICE_SC ? cells : 2 * cells;
+        ch.n_sites = sites_of(L);
         ch.group = group_of(ch.desc);
         Group& G = B->groups[ch.group];
         ch.index = (uint32_t)G.members.size();
@@ -605,12 +632,11 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
                 return rc;
             }
             const uint64_t elem_align = 256 / elem_bytes(g);  // 256-byte chain alignment
-            offset = round_up(offset, elem_align);
-            d.off[0] = offset;
-            offset += colour_elems(g, d);
-            offset = round_up(offset, elem_align);
-            d.off[1] = offset;
-            offset += colour_elems(g, d);
+            for (uint32_t c = 0; c < d.ncol; ++c) {
+                offset = round_up(offset, elem_align);
+                d.off[c] = offset;
+                offset += colour_elems(g, d);
+            }
         }
         G.arena_bytes = std::max<uint64_t>(offset * elem_bytes(g), 256);
         hipError_t e = hipMalloc(&G.arena, G.arena_bytes);
@@ -732,7 +758,7 @@ int vg_batch_randomize(vg_batch* batch) {
         if (is_ising(ch.group)) {
             launch_ising_init(ch.group, G, d, dp, st);
         } else {
-            const uint64_t n = 2ull * d.rows * d.w;
+            const uint64_t n = (uint64_t)d.ncol * d.rows * d.w;
             hipLaunchKernelGGL(vg::heis_init, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
                                dp, (float*)G.arena);
         }

@@ -24,7 +24,8 @@ __global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_
 __global__ void ising_init_bits(const ChainDev*, uint8_t*);
 __global__ void ising_scatter_bits(const ChainDev*, uint8_t*, const int8_t*);
 __global__ void ising_gather_bits(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
-__global__ void ising_reduce(const ChainDev*, const int32_t*, uint32_t, uint32_t, int64_t*);
+__global__ void ising_reduce(const ChainDev*, const int32_t*, uint32_t, uint32_t, uint32_t,
+                             int64_t*);
 __global__ void ising_init(const ChainDev*, uint8_t*);
 __global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t);
 __global__ void ising_gather(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
@@ -34,7 +35,16 @@ __global__ void ising_bcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32
                                uint32_t, uint32_t, uint32_t);
 __global__ void heis_bcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                               uint32_t, uint32_t, uint32_t);
-__global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t, int64_t*);
+__global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t, uint32_t,
+                            int64_t*);
+// fcc passes: one instantiation per colour A (sublattice), args (chains,
+// block -> chain map, arena, partials, sweep, total blocks, grid).
+template <uint32_t A>
+__global__ void ising_fcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
+                               uint32_t, uint32_t);
+template <uint32_t A>
+__global__ void heis_fcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
+                              uint32_t, uint32_t);
 __global__ void heis_clear(const ChainDev*, float*);
 __global__ void heis_init(const ChainDev*, float*);
 __global__ void heis_scatter(const ChainDev*, float*, const double*, uint64_t);
@@ -72,18 +82,31 @@ enum GroupKind {
     kIsingBcc = 2,
     kHeisBcc = 3,
     kIsingBits = 4,  // sc, 1 bit per site
-    kNumGroups = 5
+    kIsingFcc = 5,
+    kHeisFcc = 6,
+    kNumGroups = 7
 };
 
-inline bool is_ising(int g) { return g == kIsingSc || g == kIsingBcc || g == kIsingBits; }
+inline bool is_ising(int g) {
+    return g == kIsingSc || g == kIsingBcc || g == kIsingBits || g == kIsingFcc;
+}
 // Groups whose pass marches a tile of z planes per work item.
 inline bool is_zmarch(int g) { return g == kIsingSc || g == kIsingBits; }
+// Colours (planes, passes per sweep) of a group's chains.
+inline uint32_t colours_of(int g) { return g == kIsingFcc || g == kHeisFcc ? 4u : 2u; }
 
 inline int group_of(const vg_chain_desc& d) {
-    const bool bcc = d.lattice.kind == VG_LATTICE_BCC;
-    if (d.spin == VG_SPIN_HEISENBERG) return bcc ? kHeisBcc : kHeisSc;
+    const int k = d.lattice.kind;
+    if (d.spin == VG_SPIN_HEISENBERG)
+        return k == VG_LATTICE_FCC ? kHeisFcc : (k == VG_LATTICE_BCC ? kHeisBcc : kHeisSc);
     if (d.spin == VG_SPIN_ISING_BITS) return kIsingBits;
-    return bcc ? kIsingBcc : kIsingSc;
+    return k == VG_LATTICE_FCC ? kIsingFcc : (k == VG_LATTICE_BCC ? kIsingBcc : kIsingSc);
+}
+
+// Sites of a lattice of lx*ly*lz cells: 1 (sc), 2 (bcc) or 4 (fcc) per cell.
+inline uint64_t sites_of(const vg_lattice& L) {
+    const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
+    return L.kind == VG_LATTICE_SC ? cells : (L.kind == VG_LATTICE_BCC ? 2 * cells : 4 * cells);
 }
 
 struct Group {

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             float n[3] = {os[j][0], os[j][1], os[j][2]};
-            if (j < nvalid) heis_update(C, colour, os[j], hs[j], us[j], ps[j], n, acc);
+            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, acc);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
                 make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
     }
 
-    heis_partials(acc, partials, colour, s, total_blocks, b);
+    heis_partials(acc, partials, colour, 2u, s, total_blocks, b);
 }
 
 }  // namespace vg

@@ -190,10 +190,12 @@ __device__ __forceinline__ U4 flip_words(const ChainDev& C, uint32_t i0, uint32_
 
 // One site: old spin o, neighbour sum h, acceptance uniform u, proposal p.
 // Writes the new spin to n and adds the site's fixed-point sums
-// {bond (colour 1 only), mx, my, mz, (s.axis)^2} to acc.
-__device__ __forceinline__ void heis_update(const ChainDev& C, uint32_t colour, const float o[3],
-                                            const float h[3], float u, const float p[3],
-                                            float n[3], int64_t acc[5]) {
+// {bond, mx, my, mz, (s.axis)^2} to acc; the bond term n.hb is added when
+// `bond` is set (hb: the sum over the neighbours of lower colours; for the
+// two-colour lattices every neighbour of a colour-1 site).
+__device__ __forceinline__ void heis_update(const ChainDev& C, bool bond, const float o[3],
+                                            const float h[3], const float hb[3], float u,
+                                            const float p[3], float n[3], int64_t acc[5]) {
     const float ox = o[0], oy = o[1], oz = o[2];
     const float px = p[0], py = p[1], pz = p[2];
     const float ax = C.ax, ay = C.ay, az = C.az;
@@ -214,7 +216,7 @@ __device__ __forceinline__ void heis_update(const ChainDev& C, uint32_t colour, 
     acc[3] += fix27(n[2]);
     const float se = dot3(n[0], n[1], n[2], ax, ay, az);
     acc[4] += fix27(fm(se, se));
-    if (colour == 1) acc[0] += fix27(dot3(n[0], n[1], n[2], h[0], h[1], h[2]));
+    if (bond) acc[0] += fix27(dot3(n[0], n[1], n[2], hb[0], hb[1], hb[2]));
 }
 
 // Phase 1 of a site: its acceptance uniform u and proposal p.  Random
@@ -234,10 +236,11 @@ __device__ __forceinline__ void heis_propose(const ChainDev& C, uint32_t colour,
     }
 }
 
-// Workgroup sums of the five quantities -> partials[sweep][colour][block][5].
+// Workgroup sums of the five quantities -> partials[sweep][colour][block][5]
+// (ncol colours per sweep).
 __device__ __forceinline__ void heis_partials(int64_t acc[5], int64_t* __restrict__ partials,
-                                              uint32_t colour, uint32_t s, uint32_t total_blocks,
-                                              uint32_t b) {
+                                              uint32_t colour, uint32_t ncol, uint32_t s,
+                                              uint32_t total_blocks, uint32_t b) {
     __shared__ int64_t red[5][kBlock / 64];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -254,7 +257,7 @@ __device__ __forceinline__ void heis_partials(int64_t acc[5], int64_t* __restric
         int64_t v = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kBlock / 64; ++k) v += red[threadIdx.x][k];
-        partials[((uint64_t)(s * 2u + colour) * total_blocks + b) * 5u + threadIdx.x] = v;
+        partials[((uint64_t)(s * ncol + colour) * total_blocks + b) * 5u + threadIdx.x] = v;
     }
 }
 

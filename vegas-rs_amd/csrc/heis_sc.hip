@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             float n[3] = {os[j][0], os[j][1], os[j][2]};
-            if (j < nvalid) heis_update(C, colour, os[j], hs[j], us[j], ps[j], n, acc);
+            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, acc);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];
@@ -152,15 +152,16 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
                 make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
     }
 
-    heis_partials(acc, partials, colour, s, total_blocks, b);
+    heis_partials(acc, partials, colour, 2u, s, total_blocks, b);
 }
 
-// record = {bond, mx, my, mz, axis2}: both colours for the on-site sums,
-// colour 1 for the bond sum.  grid = (chains, sweeps).
+// record = {bond, mx, my, mz, axis2}, summed over the ncol colour passes (a
+// pass adds the bonds to lower colours only, so each bond counts once).
+// grid = (chains, sweeps).
 __global__ __launch_bounds__(256) void heis_reduce(const ChainDev* __restrict__ chains,
                                                    const int64_t* __restrict__ partials,
-                                                   uint32_t total_blocks, uint32_t nsw,
-                                                   int64_t* __restrict__ records) {
+                                                   uint32_t total_blocks, uint32_t ncol,
+                                                   uint32_t nsw, int64_t* __restrict__ records) {
     __shared__ int64_t red[5][kBlock / 64];
     const ChainDev& C = chains[blockIdx.x];
     const uint32_t s = blockIdx.y;
@@ -168,11 +169,11 @@ __global__ __launch_bounds__(256) void heis_reduce(const ChainDev* __restrict__ 
     int64_t acc[5] = {0, 0, 0, 0, 0};
     for (uint32_t k = threadIdx.x; k < C.nblocks; k += kBlock) {
         const uint32_t blk = C.block0 + k;
-        const int64_t* p0 = partials + ((uint64_t)(s * 2u + 0u) * total_blocks + blk) * 5u;
-        const int64_t* p1 = partials + ((uint64_t)(s * 2u + 1u) * total_blocks + blk) * 5u;
-        acc[0] += p1[0];
+        for (uint32_t c = 0; c < ncol; ++c) {
+            const int64_t* p = partials + ((uint64_t)(s * ncol + c) * total_blocks + blk) * 5u;
 #pragma unroll
-        for (int q = 1; q < 5; ++q) acc[q] += p0[q] + p1[q];
+            for (int q = 0; q < 5; ++q) acc[q] += p[q];
+        }
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {

@@ -31,13 +31,13 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                                                       int32_t* __restrict__ partials,
                                                       uint32_t colour, uint32_t s,
                                                       uint32_t total_blocks, uint32_t grid) {
-    __shared__ uint32_t lut[34];
+    __shared__ uint32_t lut[2 * kThrS];
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
     if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid < 34) lut[tid] = (&C.thr[0][0])[tid];
+    if (tid < 2 * kThrS) lut[tid] = (&C.thr[0][0])[tid];
     __syncthreads();
 
     const uint32_t local = (b - C.block0) * kBlock + tid;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                 if (((bj ? always1 : always0) >> S) & 1u) {
                     flip = true;
                 } else {
-                    const uint32_t thr = lut[bj * 17u + S];
+                    const uint32_t thr = lut[bj * kThrS + S];
                     const uint32_t r8 = (rw[kk] >> sh) & 0xFFu, hi = thr >> 24;
                     if (r8 < hi) {
                         flip = true;

@@ -68,7 +68,7 @@ __device__ __forceinline__ bool tie_accept(const uint32_t* lut, uint32_t always_
                                            uint32_t colour, uint32_t k0, uint32_t k1) {
     if ((always_b >> S) & 1u) return true;
     const U4 rb = philox(U4{pos >> 2, row, (uint32_t)t, ctr_w(t, kIsingB, colour)}, k0, k1);
-    return (u4_word(rb, pos & 3u) >> 8) < (lut[bj * 17u + S] & 0xFFFFFFu);
+    return (u4_word(rb, pos & 3u) >> 8) < (lut[bj * kThrS + S] & 0xFFFFFFu);
 }
 
 // The two storage forms of a colour plane.  Int8: one byte per position,
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                      int32_t* __restrict__ partials,
                                                      uint32_t colour, uint32_t s,
                                                      uint32_t total_blocks, uint32_t grid) {
-    __shared__ uint32_t lut[34];
+    __shared__ uint32_t lut[2 * kThrS];
     __shared__ uint32_t tab[20];  // t8, [g][b][2]
     // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4]}
     __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     const ChainDev& C = chains[block_chain[b]];
     if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid < 34) lut[tid] = (&C.thr[0][0])[tid];
+    if (tid < 2 * kThrS) lut[tid] = (&C.thr[0][0])[tid];
     if (tid >= 64 && tid < 84) tab[tid - 64] = (&C.t8[0][0][0])[tid - 64];
     __syncthreads();
 
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                 const uint32_t bj = (pick4(kk, o[0], o[1], o[2], o[3]) >> sh) & 1u;
                 const uint32_t S = 2u * c + g + 1u;
                 const uint32_t alw = ((bj ? always1 : always0) >> S) & 1u;
-                const uint32_t th_hi = alw ? 0xFFu : lut[bj * 17u + S] >> 24;
+                const uint32_t th_hi = alw ? 0xFFu : lut[bj * kThrS + S] >> 24;
                 const uint32_t r8 = (pick4(kk, rw[0], rw[1], rw[2], rw[3]) >> sh) & 0xFFu;
                 const uint32_t lv = (pick4(kk, live[0], live[1], live[2], live[3]) >> sh) & 0x80u;
                 const uint32_t a = (lv && r8 < th_hi) ? 0x80u : 0u;
@@ -460,12 +460,14 @@ template __global__ void ising_sc_pass<8, true>(const ChainDev*, const uint32_t*
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
                                                   uint32_t);
 
-// Sums the pass partials of each (chain, sweep) into its record:
-// record = {bond, 0, 0, mag, 0} (int64).  grid = (chains, sweeps).
+// Sums the pass partials of each (chain, sweep) over its ncol colour passes
+// into its record: record = {bond, 0, 0, mag, 0} (int64).  A pass adds the
+// bonds to lower colours only (none in colour 0), so each bond counts once.
+// grid = (chains, sweeps).
 __global__ __launch_bounds__(256) void ising_reduce(const ChainDev* __restrict__ chains,
                                                     const int32_t* __restrict__ partials,
-                                                    uint32_t total_blocks, uint32_t nsw,
-                                                    int64_t* __restrict__ records) {
+                                                    uint32_t total_blocks, uint32_t ncol,
+                                                    uint32_t nsw, int64_t* __restrict__ records) {
     __shared__ int64_t red[2][kBlock / 64];
     const ChainDev& C = chains[blockIdx.x];
     const uint32_t s = blockIdx.y;
@@ -473,14 +475,14 @@ __global__ __launch_bounds__(256) void ising_reduce(const ChainDev* __restrict__
     int64_t mm = 0, bb = 0;
     for (uint32_t k = threadIdx.x; k < C.nblocks; k += kBlock) {
         const uint32_t blk = C.block0 + k;
-        const int32_t* p0 =
-            partials + ((uint64_t)(s * 2u + 0u) * total_blocks + blk) * (2u * kWaves);
-        const int32_t* p1 =
-            partials + ((uint64_t)(s * 2u + 1u) * total_blocks + blk) * (2u * kWaves);
+        for (uint32_t c = 0; c < ncol; ++c) {
+            const int32_t* p =
+                partials + ((uint64_t)(s * ncol + c) * total_blocks + blk) * (2u * kWaves);
 #pragma unroll
-        for (uint32_t v = 0; v < kWaves; ++v) {
-            mm += (int64_t)p0[2 * v] + (int64_t)p1[2 * v];
-            bb += (int64_t)p1[2 * v + 1];
+            for (uint32_t v = 0; v < kWaves; ++v) {
+                mm += (int64_t)p[2 * v];
+                bb += (int64_t)p[2 * v + 1];
+            }
         }
     }
 #pragma unroll

@@ -6,7 +6,8 @@
 
 namespace vg {
 
-// Natural site id -> (colour, row, position) for sc and bcc.
+// Natural site id -> (colour, row, position): sc site x + lx*(y + ly*z);
+// bcc and fcc site (x + lx*(y + ly*z)) * k + sublattice, k = 2 or 4.
 struct Where {
     uint32_t c, row, i;
 };
@@ -22,12 +23,13 @@ __device__ __forceinline__ Where locate(const ChainDev& C, uint64_t n) {
         r.row = y + C.ly * z;
         r.i = x >> 1;
     } else {
-        const uint64_t cell = n >> 1;
+        const uint32_t sh = C.ncol == 4 ? 2u : 1u;
+        const uint64_t cell = n >> sh;
         const uint32_t x = (uint32_t)(cell % C.lx);
         const uint64_t yz = cell / C.lx;
         const uint32_t y = (uint32_t)(yz % C.ly);
         const uint32_t z = (uint32_t)(yz / C.ly);
-        r.c = (uint32_t)(n & 1u);
+        r.c = (uint32_t)(n & ((1u << sh) - 1u));
         r.row = y + C.ly * z;
         r.i = x;
     }
@@ -41,8 +43,8 @@ __global__ void ising_init(const ChainDev* __restrict__ chain, uint8_t* __restri
     const ChainDev& C = *chain;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t per_plane = (uint64_t)C.rows * C.pitch;
-    if (idx >= 2 * per_plane) return;
-    const uint32_t c = idx >= per_plane ? 1u : 0u;
+    if (idx >= C.ncol * per_plane) return;
+    const uint32_t c = (uint32_t)(idx / per_plane);
     const uint64_t e = idx - c * per_plane;
     const uint32_t row = (uint32_t)(e / C.pitch);
     const uint32_t i = (uint32_t)(e % C.pitch);
@@ -138,7 +140,7 @@ __global__ void ising_gather_bits(const ChainDev* __restrict__ chain,
 __global__ void heis_clear(const ChainDev* __restrict__ chain, float* __restrict__ arena) {
     const ChainDev& C = *chain;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= 6 * C.plane) return;
+    if (idx >= 3ull * C.ncol * C.plane) return;
     const uint32_t c = (uint32_t)(idx / (3 * C.plane));
     const uint64_t e = idx - (uint64_t)c * 3 * C.plane;
     arena[C.off[c] + e] = 0.0f;
@@ -153,8 +155,8 @@ __global__ void heis_init(const ChainDev* __restrict__ chain, float* __restrict_
     const ChainDev& C = *chain;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t per_colour = (uint64_t)C.rows * C.w;
-    if (idx >= 2 * per_colour) return;
-    const uint32_t c = idx >= per_colour ? 1u : 0u;
+    if (idx >= C.ncol * per_colour) return;
+    const uint32_t c = (uint32_t)(idx / per_colour);
     const uint64_t e = idx - c * per_colour;
     const uint32_t row = (uint32_t)(e / C.w);
     const uint32_t i = (uint32_t)(e % C.w);

@@ -189,7 +189,7 @@ namespace {
 
 uint64_t job_bytes(const Job& j) {
     const int g = j.ch.group;
-    return 2 * round_up(colour_elems(g, j.dev) * elem_bytes(g), 256);
+    return j.dev.ncol * round_up(colour_elems(g, j.dev) * elem_bytes(g), 256);
 }
 
 // Makes the working batch's groups the running jobs and sizes its buffers.
@@ -302,10 +302,11 @@ int quantum(vg_server* S, uint32_t& nsw_out) {
         if (is_ising(j->ch.group)) {
             launch_ising_init(j->ch.group, G, j->dev, dp, st);
         } else {
-            const uint64_t n6 = 6ull * j->dev.plane, n2 = 2ull * j->dev.rows * j->dev.w;
-            hipLaunchKernelGGL(vg::heis_clear, dim3((uint32_t)((n6 + 255) / 256)), dim3(256), 0, st,
+            const uint64_t nc = 3ull * j->dev.ncol * j->dev.plane;
+            const uint64_t ni = (uint64_t)j->dev.ncol * j->dev.rows * j->dev.w;
+            hipLaunchKernelGGL(vg::heis_clear, dim3((uint32_t)((nc + 255) / 256)), dim3(256), 0, st,
                                dp, (float*)G.arena);
-            hipLaunchKernelGGL(vg::heis_init, dim3((uint32_t)((n2 + 255) / 256)), dim3(256), 0, st,
+            hipLaunchKernelGGL(vg::heis_init, dim3((uint32_t)((ni + 255) / 256)), dim3(256), 0, st,
                                dp, (float*)G.arena);
         }
         j->fresh = false;
@@ -446,8 +447,8 @@ void worker_main(vg_server* S) {
             j.pool_off = off;
             const int g = j.ch.group;
             const uint64_t e = elem_bytes(g);
-            j.dev.off[0] = off / e;
-            j.dev.off[1] = (off + round_up(colour_elems(g, j.dev) * e, 256)) / e;
+            for (uint32_t c = 0; c < j.dev.ncol; ++c)
+                j.dev.off[c] = (off + c * round_up(colour_elems(g, j.dev) * e, 256)) / e;
             j.state = VG_JOB_RUNNING;
             j.start_s = now_s(S->t0);
             j.fresh = true;
@@ -595,8 +596,7 @@ int vg_server_submit(vg_server* S, const vg_job_desc* job, uint64_t* job_id) {
     std::unique_ptr<Job> j(new Job);
     j->ch.desc = job->chain;
     const vg_lattice& L = job->chain.lattice;
-    const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
-    j->ch.n_sites = L.kind == VG_LATTICE_SC ? cells : 2 * cells;
+    j->ch.n_sites = sites_of(L);
     j->ch.group = group_of(job->chain);
     rc = init_chain_dev(j->ch, j->ch.group, 0, j->dev);
     if (rc) return rc;

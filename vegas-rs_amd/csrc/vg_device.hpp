@@ -3,12 +3,15 @@
 // (chain seed, sweep, colour, row, position), the chain descriptor the
 // kernels read, and the colour-split spin layout.
 //
-// Layout of one chain (sc lattice lx*ly*lz, bcc lattice lx*ly*lz cells):
-//   The lattice is bipartite.  Colour c of sc site (x,y,z) is (x+y+z)&1; of
-//   a bcc site it is its sublattice.  Each colour is stored as its own plane
-//   of rows; row r = y + ly*z; within a row, position i holds
-//     sc : the colour-c site at x = 2*i + ((c+y+z)&1)   (w = lx/2 positions)
-//     bcc: the sublattice-c site of cell x = i           (w = lx positions)
+// Layout of one chain (sc lattice lx*ly*lz, bcc and fcc lattices lx*ly*lz
+// cells):
+//   The colours are independent sets of the lattice: sc and bcc are
+//   bipartite (two colours), fcc takes four.  Colour c of sc site (x,y,z) is
+//   (x+y+z)&1; of a bcc or fcc site it is its sublattice.  Each colour is
+//   stored as its own plane of rows; row r = y + ly*z; within a row,
+//   position i holds
+//     sc     : the colour-c site at x = 2*i + ((c+y+z)&1)   (w = lx/2 positions)
+//     bcc/fcc: the sublattice-c site of cell x = i           (w = lx positions)
 //   Rows are `pitch` elements apart (pitch is a multiple of 64 bytes), so a
 //   row is read with aligned 16-byte loads.  Ising planes hold one byte per
 //   site (1 = Up, 0 = Down); Heisenberg chains hold three fp32 planes (x, y,
@@ -66,7 +69,8 @@ VG_HD uint32_t u4_word(const U4& v, uint32_t k) {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
 }
 
-// Counter word w: bit 0 colour, bits 1-7 purpose, bits 8-31 sweep >> 32.
+// Counter word w: bit 0 colour & 1, bits 1-6 purpose, bit 7 colour >> 1
+// (fcc colours 2 and 3), bits 8-31 sweep >> 32.
 // Counter words x, y, z: position group, row, sweep & 0xffffffff (or call
 // index for the initial state, which has no sweep).
 enum Purpose : uint32_t {
@@ -81,7 +85,7 @@ constexpr uint32_t kHeisMaxCalls = 16;
 constexpr uint32_t kHeisInitMaxCalls = 64;
 
 VG_HD uint32_t ctr_w(uint64_t sweep, uint32_t purpose, uint32_t colour) {
-    return ((uint32_t)(sweep >> 32) << 8) | (purpose << 1) | colour;
+    return ((uint32_t)(sweep >> 32) << 8) | ((colour >> 1) << 7) | (purpose << 1) | (colour & 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -91,13 +95,16 @@ VG_HD uint32_t ctr_w(uint64_t sweep, uint32_t purpose, uint32_t colour) {
 // ---------------------------------------------------------------------------
 constexpr float kFixScale = 134217728.0f;  // 2^27
 
+// Ising acceptance classes per current bit: S = 0 .. 2z, z <= 12.
+constexpr uint32_t kThrS = 25;
+
 // ---------------------------------------------------------------------------
 // Chain descriptor as the kernels see it (one per chain of a kernel group).
 // ---------------------------------------------------------------------------
 struct ChainDev {
     uint64_t seed;         // Philox key (k0 = low word, k1 = high word)
     uint64_t sweep0;       // sweep counter at the start of the current block
-    uint64_t off[2];       // element offset of colour plane c in the arena
+    uint64_t off[4];       // element offset of colour plane c in the arena (ncol planes)
     uint64_t plane;        // elements per plane (rows * pitch); Heisenberg y/z planes follow x
     uint32_t lx, ly, lz, kind;
     uint32_t w, pitch, rows, pbc;
@@ -107,7 +114,8 @@ struct ChainDev {
     uint32_t nblocks;      // blocks of this chain
     uint32_t proposal;     // 0 flip, 1 random
     uint32_t slot;         // chain index in its batch (record row)
-    uint32_t z;            // coordination number: 6 (sc) or 8 (bcc)
+    uint32_t z;            // coordination number: 6 (sc), 8 (bcc) or 12 (fcc)
+    uint32_t ncol;         // colours (planes): 2 (sc, bcc) or 4 (fcc)
     uint32_t limit;        // sweeps of the current block this chain runs (the rest skip it)
     // n / d == (n * m) >> s for n < 2^31 (segs, ly and hy divisors)
     uint32_t segs_m, segs_s, ly_m, ly_s;
@@ -117,7 +125,7 @@ struct ChainDev {
     uint32_t nzc;          // z chunks of one tile of planes
     // Ising acceptance: class (b, S) with b the current bit and S the sum over
     // neighbour slots of 2*b_j (present) or 1 (absent), S in [0, 2z].
-    uint32_t thr[2][17];   // accept iff draw < thr (32-bit draw)
+    uint32_t thr[2][kThrS];  // accept iff draw < thr (32-bit draw)
     uint32_t always[2];    // bit S set: class always accepted
     // The same classes as byte tables for the packed sc decision, indexed by
     // g (absent y/z neighbour slots, 0..4), b, and the up-neighbour count c
@@ -134,7 +142,7 @@ struct ChainDev {
     float beta;            // (float)(1/T)
     float ax, ay, az;      // anisotropy axis
     float fx, fy, fz;      // field orientation
-    float pad1, pad2, pad3;
+    float pad1, pad2;
 };
 
 constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass

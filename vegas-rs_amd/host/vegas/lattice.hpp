@@ -2,9 +2,10 @@
 // operations the reference takes from the vegas-lattice crate
 // (src/input.rs:296-322, src/main.rs:25): Lattice::sc / bcc / fcc(a),
 // expand(x,y,z), expand_all(n), expand_x/y/z(n), drop_x/y/z() (periodic ->
-// open).  The engine needs a bipartite lattice: sc (site id x + lx*(y + ly*z))
-// and bcc (site id 2*(cell) + sublattice); fcc is not two-colourable and is
-// rejected when a chain is built from it.
+// open).  Site ids: sc x + lx*(y + ly*z); bcc 2*(cell) + sublattice; fcc
+// 4*(cell) + sublattice (corner, then the face centres (0,1/2,1/2),
+// (1/2,0,1/2), (1/2,1/2,0)).  The engine sweeps sc and bcc in two colours and
+// fcc in four, one per sublattice.
 #pragma once
 
 #include <cstdint>
@@ -16,7 +17,7 @@ namespace vegas {
 
 class Lattice {
   public:
-    enum Cell { SC = VG_LATTICE_SC, BCC = VG_LATTICE_BCC, FCC = 2 };
+    enum Cell { SC = VG_LATTICE_SC, BCC = VG_LATTICE_BCC, FCC = VG_LATTICE_FCC };
     static Lattice sc(double /*a*/ = 1.0) { return Lattice(SC); }
     static Lattice bcc(double /*a*/ = 1.0) { return Lattice(BCC); }
     static Lattice fcc(double /*a*/ = 1.0) { return Lattice(FCC); }
@@ -41,9 +42,6 @@ class Lattice {
         return cell_ == SC ? cells : (cell_ == BCC ? 2 * cells : 4 * cells);
     }
     vg_lattice descriptor() const {
-        if (cell_ == FCC)
-            throw Error(ErrorKind::UnsupportedLattice,
-                        "fcc is not bipartite; the two-colour sweep needs sc or bcc");
         vg_lattice d;
         d.kind = cell_;
         d.lx = lx_;
@@ -52,8 +50,9 @@ class Lattice {
         d.pbc = pbc_;
         return d;
     }
-    // Neighbour slots of a site (sc: -x,+x,-y,+y,-z,+z; bcc: 8), -1 when the
-    // neighbour would cross an open boundary.
+    // Neighbour slots of a site (sc: -x,+x,-y,+y,-z,+z; bcc: 8; fcc: 12, 4
+    // per other sublattice), -1 when the neighbour would cross an open
+    // boundary.
     std::vector<int64_t> neighbours(uint64_t site) const;
 
   private:
@@ -101,7 +100,35 @@ inline std::vector<int64_t> Lattice::neighbours(uint64_t site) const {
                                       : 2 * (a + (int64_t)lx_ * (b + (int64_t)ly_ * c)) + (1 - sub));
                 }
     } else {
-        throw Error(ErrorKind::UnsupportedLattice, "fcc neighbours are not provided");
+        // Sublattice a bonds to 4 sites of each other sublattice b, in cells
+        // c + d1*e1 + d2*e2 over the two axes where a and b differ, e = -1
+        // where b sits higher in the cell than a, +1 where lower.
+        static const int half[4][3] = {{0, 0, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+        const uint64_t cellid = site >> 2;
+        const int a = (int)(site & 3);
+        const int64_t c0[3] = {(int64_t)(cellid % lx_), (int64_t)((cellid / lx_) % ly_),
+                               (int64_t)(cellid / ((uint64_t)lx_ * ly_))};
+        const int64_t dims[3] = {lx_, ly_, lz_};
+        const bool per[3] = {px, py, pz};
+        for (int b = 0; b < 4; ++b) {
+            if (b == a) continue;
+            int ax[2], m = 0;
+            for (int e = 0; e < 3; ++e)
+                if (half[a][e] != half[b][e]) ax[m++] = e;
+            for (int d2 = 0; d2 < 2; ++d2)
+                for (int d1 = 0; d1 < 2; ++d1) {
+                    int64_t q[3];
+                    bool ok = true;
+                    for (int e = 0; e < 3; ++e) {
+                        int64_t d = (e == ax[0] ? d1 : 0) + (e == ax[1] ? d2 : 0);
+                        if (half[b][e] > half[a][e]) d = -d;
+                        q[e] = step(c0[e], d, dims[e], per[e]);
+                        ok = ok && q[e] >= 0;
+                    }
+                    out.push_back(ok ? 4 * (q[0] + (int64_t)lx_ * (q[1] + (int64_t)ly_ * q[2])) + b
+                                     : -1);
+                }
+        }
     }
     return out;
 }
