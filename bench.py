@@ -15,7 +15,8 @@ sequential MetropolisFlipIntegrator, one core, a bounded sample).
 
 With the default --workload all, the line also carries the other forms of
 the path as sub-objects: the same chains bit-packed, the 512-chain mixed
-batch against per-chain plain launches, Heisenberg sc 128^3 x 3, and the
+batch against per-chain plain launches, Heisenberg sc 128^3 x 3, Ising and
+Heisenberg chains on fcc lattices (four colour passes per sweep), and the
 served Poisson job stream with p50/p99 latency (--serve-jobs jobs per rate).
 """
 import argparse
@@ -236,6 +237,57 @@ def run_heisenberg(vg, ctx, side, sweeps):
     }
 
 
+def run_fcc(vg, ctx, cells, heis_cells, sweeps):
+    """fcc lattices (four colour passes per sweep): 3 Ising chains of cells^3
+    fcc cells (4 sites each; J = 1, h = 0; T = 8, 9.8, 12 around T_c ~ 9.8)
+    and 3 Heisenberg chains of heis_cells^3 cells (J = 1, K = 0.1 z, B = 0.05
+    z, uniform-sphere proposals; T = 1.5, 3.2, 5), random starts from seed 7.
+    Attempts/s and each pass kernel's roofline (2 B / 24 B per attempt)."""
+    out = {}
+    for spin, side, temps in (("ising", cells, [8.0, 9.8, 12.0]),
+                              ("heisenberg", heis_cells, [1.5, 3.2, 5.0])):
+        ising = spin == "ising"
+        H = (vg.hamiltonian(exchange=1.0) if ising
+             else vg.hamiltonian(exchange=1.0, zeeman=True, anisotropy=0.1))
+        L = vg.lattice(vg.FCC, side, side, side)
+        descs = [vg.chain(L, vg.ISING if ising else vg.HEISENBERG,
+                          vg.FLIP if ising else vg.RANDOM, H, 7) for _ in temps]
+        n = 4 * side ** 3
+        with vg.Batch(ctx, descs) as b:
+            b.randomize()
+            for i, T in enumerate(temps):
+                b.set_thermostat(i, T, 0.0 if ising else 0.05)
+            b.set_graph(True)
+            b.run(5)
+            t0 = time.perf_counter()
+            e, m = b.run(sweeps)
+            wall = time.perf_counter() - t0
+            dev_ms = b.last_run_ms()
+        value = n * len(temps) * sweeps / wall
+        per_launch = n * len(temps) // 4
+        avg_ms = dev_ms / (4 * sweeps)
+        bpa = 2 if ising else 24
+        achieved = bpa * per_launch / (avg_ms * 1e-3) / 1e9
+        log(f"fcc {spin}: {value:.4e} attempts/s, pass {avg_ms * 1e3:.1f} us -> {achieved:.0f} GB/s")
+        out[spin] = {
+            "workload": (f"{spin}-fcc{side}: 3 chains of {side}^3 fcc cells ({n} sites), "
+                         f"T={temps}" + ("" if ising else ", K=0.1 z, B=0.05 z, random proposals")),
+            "value": value,
+            "unit": "spin-flip attempts/s",
+            "sweeps": sweeps,
+            "ms_per_step": wall * 1e3 / sweeps,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "kernel": "ising_fcc_pass" if ising else "heis_fcc_pass",
+                         "algorithmic_bytes_per_launch": bpa * per_launch,
+                         "mean_launch_us": avg_ms * 1e3},
+            "observables_last_sweep": {f"T={T}": {"E/N": float(e[i, -1]) / n,
+                                                  "|M|/N": float(m[i, -1]) / n}
+                                       for i, T in enumerate(temps)},
+        }
+    return out
+
+
 def served_jobs(vg, n_jobs, gen_seed=2025):
     """The served mix: 60% Ising Relax (sc L^3, L even in [16, 96], T uniform in
     [2, 6], 2000 sweeps), 30% Heisenberg CoolDown (sc L^3, L even in [16, 64],
@@ -374,13 +426,18 @@ def main():
     ap.add_argument("--timing-sweeps", type=int, default=100)
     ap.add_argument("--tile", type=int, default=0, help="Ising tile; 0 = autotune")
     ap.add_argument("--workload", default="all",
-                    choices=["all", "headline", "batched", "served", "heisenberg", "bitpacked"],
+                    choices=["all", "headline", "batched", "served", "heisenberg", "bitpacked",
+                             "fcc"],
                     help="headline: sc 256^3 x 3 only; batched: the 512-chain batch only; "
                          "served: the Poisson job stream only")
     ap.add_argument("--serve-jobs", type=int, default=400,
                     help="jobs per rate (the full served workload is 2000)")
     ap.add_argument("--heis-side", type=int, default=128)
     ap.add_argument("--heis-sweeps", type=int, default=500)
+    ap.add_argument("--fcc-cells", type=int, default=128, help="fcc Ising chains: cells per side")
+    ap.add_argument("--fcc-heis-cells", type=int, default=80,
+                    help="fcc Heisenberg chains: cells per side")
+    ap.add_argument("--fcc-sweeps", type=int, default=300)
     ap.add_argument("--serve-rates", default="50,500")
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="server per-sweep work budget (Ising-site units; 0 = none)")
@@ -413,6 +470,11 @@ def main():
         with vg.Context(0) as ctx:
             print(json.dumps(run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)),
                   flush=True)
+        return
+    if args.workload == "fcc":
+        with vg.Context(0) as ctx:
+            print(json.dumps(run_fcc(vg, ctx, args.fcc_cells, args.fcc_heis_cells,
+                                     args.fcc_sweeps)), flush=True)
         return
     if args.workload == "bitpacked":
         with vg.Context(0) as ctx:
@@ -525,6 +587,7 @@ def main():
         out["bitpacked"] = run_bitpacked(vg, ctx, side, args.steps)
         out["batched"] = run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps)
         out["heisenberg"] = run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)
+        out["fcc"] = run_fcc(vg, ctx, args.fcc_cells, args.fcc_heis_cells, args.fcc_sweeps)
         rates = [float(r) for r in args.serve_rates.split(",")]
         out["served"] = run_served(vg, ctx, args.serve_jobs, rates, budget=args.serve_budget)
     ctx.close()
