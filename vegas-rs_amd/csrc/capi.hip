@@ -198,11 +198,12 @@ void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev*
         hipLaunchKernelGGL(vg::ising_init, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
 }
 
-int upload_descriptors(vg_batch* B, int g) {
+int upload_descriptors(vg_batch* B, int g, int slot) {
     Group& G = B->groups[g];
     if (G.members.empty()) return VG_OK;
-    std::memcpy(G.h_staging, G.host.data(), G.host.size() * sizeof(ChainDev));
-    VG_HIP(hipMemcpyAsync(G.d_chains, G.h_staging, G.host.size() * sizeof(ChainDev),
+    ChainDev* staging = slot ? G.h_staging2 : G.h_staging;
+    std::memcpy(staging, G.host.data(), G.host.size() * sizeof(ChainDev));
+    VG_HIP(hipMemcpyAsync(G.d_chains, staging, G.host.size() * sizeof(ChainDev),
                           hipMemcpyHostToDevice, B->ctx->stream));
     return VG_OK;
 }
@@ -257,14 +258,16 @@ int free_batch(vg_batch* B) {
     for (Group& G : B->groups) {
         if (G.d_chains) quiet(hipFree(G.d_chains));
         if (G.h_staging) quiet(hipHostFree(G.h_staging));
+        if (G.h_staging2) quiet(hipHostFree(G.h_staging2));
         if (G.d_block_chain) quiet(hipFree(G.d_block_chain));
         if (G.arena && !G.shared_arena) quiet(hipFree(G.arena));
         if (G.partials) quiet(hipFree(G.partials));
     }
     if (B->d_records) quiet(hipFree(B->d_records));
     if (B->h_records) quiet(hipHostFree(B->h_records));
-    if (B->ev0) quiet(hipEventDestroy(B->ev0));
-    if (B->ev1) quiet(hipEventDestroy(B->ev1));
+    if (B->h_records2) quiet(hipHostFree(B->h_records2));
+    for (hipEvent_t e : {B->ev0, B->ev1, B->ev2, B->ev3, B->done[0], B->done[1]})
+        if (e) quiet(hipEventDestroy(e));
     delete B;
     return VG_OK;
 }
@@ -382,6 +385,8 @@ int launch_block(vg_batch* B, uint32_t nsw) {
     hipStream_t st = B->ctx->stream;
     auto it = B->graphs.find(nsw);
     if (it == B->graphs.end()) {
+        // Capture on an idle stream (an earlier block may still be queued).
+        VG_HIP(hipStreamSynchronize(st));
         hipGraph_t graph = nullptr;
         VG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_block(B, nsw);
@@ -449,6 +454,12 @@ void observables(const ChainHost& ch, const vg_sweep_sums& r, double* E, double*
     if (M) *M = mag;
 }
 
+// Runs `sweeps` sweeps of every chain in blocks of up to max_block_sweeps.
+// Blocks are pipelined two deep: block k+1 (descriptors, graph or launches,
+// record copy) is queued before the host waits for block k's records, so
+// the device runs from block to block without waiting for the host.  The
+// per-pass timing mode (plain launches with events per pass) runs one block
+// at a time.
 int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
              double* magnetization) {
     if (!B) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
@@ -460,39 +471,64 @@ int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
     hipStream_t st = B->ctx->stream;
     VG_HIP(hipSetDevice(B->ctx->device));
     B->last_ms = 0.0;
-    uint64_t done = 0;
-    while (done < sweeps) {
-        const uint32_t nsw = (uint32_t)std::min<uint64_t>(B->max_block_sweeps, sweeps - done);
-        for (uint64_t i = 0; i < nch; ++i) {
-            ChainHost& ch = B->chains[i];
-            B->groups[ch.group].host[ch.index].sweep0 = ch.sweeps;
-        }
-        for (int g = 0; g < kNumGroups; ++g) {
-            int rc = upload_descriptors(B, g);
-            if (rc) return rc;
-        }
-        if (B->timing && !B->use_graph) {
-            size_t need = 0;  // a start/stop pair per colour pass
-            for (int g = 0; g < kNumGroups; ++g)
-                if (!B->groups[g].members.empty()) need += (size_t)nsw * 2 * colours_of(g);
-            while (B->pass_events.size() < need) {
-                hipEvent_t e;
-                VG_HIP(hipEventCreate(&e));
-                B->pass_events.push_back(e);
+    const bool timed = B->timing && !B->use_graph;
+    const size_t depth = timed ? 1 : 2;
+    std::vector<uint64_t> start(nch);
+    for (uint64_t i = 0; i < nch; ++i) start[i] = B->chains[i].sweeps;
+    struct Block {
+        uint32_t nsw;
+        uint64_t first;  // sweeps of this run before the block
+        int slot;
+    };
+    Block inflight[2];
+    size_t n_inflight = 0, head = 0;
+    uint64_t queued = 0;
+    int slot = 0;
+    while (queued < sweeps || n_inflight > 0) {
+        if (queued < sweeps && n_inflight < depth) {
+            const uint32_t nsw = (uint32_t)std::min<uint64_t>(B->max_block_sweeps, sweeps - queued);
+            for (uint64_t i = 0; i < nch; ++i) {
+                ChainHost& ch = B->chains[i];
+                B->groups[ch.group].host[ch.index].sweep0 = start[i] + queued;
             }
-            B->pass_events_used = 0;
+            for (int g = 0; g < kNumGroups; ++g) {
+                int rc = upload_descriptors(B, g, slot);
+                if (rc) return rc;
+            }
+            if (timed) {
+                size_t need = 0;  // a start/stop pair per colour pass
+                for (int g = 0; g < kNumGroups; ++g)
+                    if (!B->groups[g].members.empty()) need += (size_t)nsw * 2 * colours_of(g);
+                while (B->pass_events.size() < need) {
+                    hipEvent_t e;
+                    VG_HIP(hipEventCreate(&e));
+                    B->pass_events.push_back(e);
+                }
+                B->pass_events_used = 0;
+            }
+            VG_HIP(hipEventRecord(slot ? B->ev2 : B->ev0, st));
+            int rc = launch_block(B, nsw);
+            if (rc) return rc;
+            VG_HIP(hipEventRecord(slot ? B->ev3 : B->ev1, st));
+            VG_HIP(hipMemcpyAsync(slot ? B->h_records2 : B->h_records, B->d_records,
+                                  nch * nsw * 5u * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+            VG_HIP(hipEventRecord(B->done[slot], st));
+            inflight[(head + n_inflight) % 2] = Block{nsw, queued, slot};
+            ++n_inflight;
+            queued += nsw;
+            slot ^= 1;
+            if (queued < sweeps && n_inflight < depth) continue;  // fill the pipeline
         }
-        VG_HIP(hipEventRecord(B->ev0, st));
-        int rc = launch_block(B, nsw);
-        if (rc) return rc;
-        VG_HIP(hipEventRecord(B->ev1, st));
-        VG_HIP(hipMemcpyAsync(B->h_records, B->d_records, nch * nsw * 5u * sizeof(int64_t),
-                              hipMemcpyDeviceToHost, st));
-        VG_HIP(hipStreamSynchronize(st));
+        // The oldest block: wait for its records and turn them into results.
+        const Block blk = inflight[head];
+        head = (head + 1) % 2;
+        --n_inflight;
+        VG_HIP(hipEventSynchronize(B->done[blk.slot]));
         float ms = 0.0f;
-        VG_HIP(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+        VG_HIP(hipEventElapsedTime(&ms, blk.slot ? B->ev2 : B->ev0, blk.slot ? B->ev3 : B->ev1));
         B->last_ms += ms;
-        if (B->timing && !B->use_graph) {
+        const uint32_t nsw = blk.nsw;
+        if (timed) {
             // Events were recorded in launch order: sweep, group, colour.
             uint32_t k = 0;
             for (uint32_t sw = 0; sw < nsw; ++sw)
@@ -506,25 +542,25 @@ int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
                     }
                 }
         }
+        const int64_t* rec = blk.slot ? B->h_records2 : B->h_records;
         for (uint64_t i = 0; i < nch; ++i) {
             ChainHost& ch = B->chains[i];
             for (uint32_t s = 0; s < nsw; ++s) {
-                const int64_t* r = B->h_records + (i * nsw + s) * 5u;
+                const int64_t* r = rec + (i * nsw + s) * 5u;
                 vg_sweep_sums v;
                 v.bond = r[0];
                 v.mag[0] = r[1];
                 v.mag[1] = r[2];
                 v.mag[2] = r[3];
                 v.axis2 = ch.desc.spin == VG_SPIN_HEISENBERG ? r[4] : (int64_t)ch.n_sites;
-                const uint64_t o = i * sweeps + done + s;
+                const uint64_t o = i * sweeps + blk.first + s;
                 if (sums) sums[o] = v;
                 if (energy || magnetization)
                     observables(ch, v, energy ? energy + o : nullptr,
                                 magnetization ? magnetization + o : nullptr);
             }
-            ch.sweeps += nsw;
+            ch.sweeps = start[i] + blk.first + nsw;
         }
-        done += nsw;
     }
     return VG_OK;
 }
@@ -643,6 +679,7 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         if (e == hipSuccess) e = hipMemset(G.arena, 0, G.arena_bytes);
         if (e == hipSuccess) e = hipMalloc(&G.d_chains, G.host.size() * sizeof(ChainDev));
         if (e == hipSuccess) e = hipHostMalloc(&G.h_staging, G.host.size() * sizeof(ChainDev));
+        if (e == hipSuccess) e = hipHostMalloc(&G.h_staging2, G.host.size() * sizeof(ChainDev));
         G.chains_cap = (uint32_t)G.host.size();
         if (e != hipSuccess) {
             free_batch(B);
@@ -681,8 +718,9 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
     const uint64_t rec_bytes = (uint64_t)n_chains * B->max_block_sweeps * 5u * sizeof(int64_t);
     hipError_t e = hipMalloc(&B->d_records, rec_bytes);
     if (e == hipSuccess) e = hipHostMalloc(&B->h_records, rec_bytes);
-    if (e == hipSuccess) e = hipEventCreate(&B->ev0);
-    if (e == hipSuccess) e = hipEventCreate(&B->ev1);
+    if (e == hipSuccess) e = hipHostMalloc(&B->h_records2, rec_bytes);
+    for (hipEvent_t* ev : {&B->ev0, &B->ev1, &B->ev2, &B->ev3, &B->done[0], &B->done[1]})
+        if (e == hipSuccess) e = hipEventCreate(ev);
     if (e != hipSuccess) {
         free_batch(B);
         return fail(VG_ERR_DEVICE, "allocating records: %s", hipGetErrorString(e));

@@ -114,6 +114,7 @@ struct Group {
     std::vector<ChainDev> host;     // descriptors, uploaded each sweep block
     ChainDev* d_chains = nullptr;
     ChainDev* h_staging = nullptr;  // pinned copy used for the upload
+    ChainDev* h_staging2 = nullptr;  // second one, for the block after (batches)
     uint32_t* d_block_chain = nullptr;
     uint32_t block_chain_cap = 0;
     uint32_t tile = 4;  // Ising: z planes per work item (template parameter of the pass)
@@ -155,9 +156,12 @@ struct vg_batch {
     uint32_t max_block_sweeps = 256;
     int64_t* d_records = nullptr;  // [chain][sweep][5]
     int64_t* h_records = nullptr;  // pinned
+    int64_t* h_records2 = nullptr;  // pinned, for the block after (batches)
     bool use_graph = false;
     std::map<uint32_t, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev2 = nullptr, ev3 = nullptr;  // the same, for the block after
+    hipEvent_t done[2] = {nullptr, nullptr};  // records of a block are on the host
     double last_ms = 0.0;
     // Per-launch timing of the colour-pass kernels (plain launches only).
     bool timing = false;
@@ -182,7 +186,9 @@ void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev*
                        hipStream_t st);
 uint32_t quantities(int group);
 uint32_t quantity_bytes(int group);
-int upload_descriptors(vg_batch* B, int g);
+// Copies group g's descriptors to the device through pinned staging buffer
+// `slot` (0, or 1 for the second buffer of a batch).
+int upload_descriptors(vg_batch* B, int g, int slot = 0);
 int layout_work(vg_batch* B, int g, uint32_t tile);
 int enqueue_block(vg_batch* B, uint32_t nsw);
 int launch_block(vg_batch* B, uint32_t nsw);
