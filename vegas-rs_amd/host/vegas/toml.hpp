@@ -173,6 +173,7 @@ class Parser {
                 out += c;
                 continue;
             }
+            if (eof()) error("unterminated string");
             const char e = t_[pos_++];
             switch (e) {
                 case 'b': out += '\b'; break;
