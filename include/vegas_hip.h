@@ -122,6 +122,11 @@ typedef struct vg_hamiltonian {
 
 #define VG_PROPOSAL_FLIP 0       /* MetropolisFlipIntegrator (integrator.rs:100-138) */
 #define VG_PROPOSAL_RANDOM 1     /* MetropolisIntegrator (integrator.rs:57-92) */
+/* WolffIntegrator (integrator.rs:146-230), int8 Ising chains only: one   */
+/* "sweep" of vg_batch_run is one Wolff step, i.e. one cluster grown from */
+/* a random site with bond probability 1 - exp(-2J/T) over equal spins    */
+/* and flipped (the move ignores the field, as the reference's does).     */
+#define VG_PROPOSAL_WOLFF 2
 
 typedef struct vg_chain_desc {
     vg_lattice lattice;

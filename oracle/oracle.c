@@ -484,6 +484,103 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
 }
 
 /* ------------------------------------------------------------------------ */
+/* Wolff single-cluster steps (integrator.rs:146-230) with the engine's     */
+/* draws: the seed site is floor(r64 * N / 2^64) for r64 = words 0,1 of     */
+/* Philox(0, 0, step, w(step, 6, 0)); a bond between two equal spins of the */
+/* cluster's sign is active iff word 0 of Philox(lo, slot, step, w(step, 5, */
+/* 0)) is below ceil((1 - exp(-2J/T)) * 2^32), (lo, slot) being the bond's  */
+/* endpoint with the lower site id and its neighbour slot.  The cluster is  */
+/* the seed's connected component over active bonds, so it does not depend */
+/* on the order the search visits it in.                                    */
+/* ------------------------------------------------------------------------ */
+/* slot of neighbour j's list that holds the bond of slot k of site i */
+static int back_slot(const or_lattice* L, uint64_t i, int k) {
+    if (L->kind == 0) return k ^ 1; /* -x <-> +x, -y <-> +y, -z <-> +z */
+    if (L->kind == 1) return k;     /* (dz, dy, dx) from either sublattice */
+    const int a = (int)(i & 3), r = k >> 2, b = r + (r >= a ? 1 : 0);
+    return 4 * (a - (a > b ? 1 : 0)) + (k & 3);
+}
+
+void or_wolff_threshold(const or_model* M, uint32_t* thr, int* all) {
+    const double T = M->temperature < DBL_EPSILON ? DBL_EPSILON : M->temperature;
+    const double J = (M->terms & 1u) ? M->exchange : 0.0;
+    const double p = 1.0 - exp(-2.0 * J / T);
+    *thr = 0;
+    *all = 0;
+    if (!(p > 0.0)) return;
+    const double th = ceil(p * 4294967296.0);
+    if (th >= 4294967296.0)
+        *all = 1;
+    else
+        *thr = (uint32_t)th;
+}
+
+void or_wolff_steps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t step0,
+                    uint64_t nsteps, int8_t* spins, int64_t* sums) {
+    const uint64_t n = or_num_sites(L);
+    const int z = or_coordination(L);
+    uint32_t thr;
+    int all;
+    or_wolff_threshold(M, &thr, &all);
+    uint64_t* queue = (uint64_t*)malloc(n * sizeof(uint64_t));
+    uint8_t* in = (uint8_t*)malloc(n);
+    int64_t nb[12];
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        const uint64_t t = step0 + k;
+        uint32_t r[4];
+        draw(seed, 0, 0, (uint32_t)t, wword(t, 6, 0), r);
+        const uint64_t r64 = (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+        const uint64_t src = (uint64_t)(((unsigned __int128)r64 * n) >> 64);
+        const int8_t ref = spins[src];
+        memset(in, 0, n);
+        in[src] = 1;
+        queue[0] = src;
+        uint64_t head = 0, tail = 1;
+        while (head < tail) {
+            const uint64_t i = queue[head++];
+            or_neighbours(L, i, nb);
+            for (int sl = 0; sl < z; ++sl) {
+                const int64_t j = nb[sl];
+                if (j < 0 || (uint64_t)j == i || in[j] || spins[j] != ref) continue;
+                uint64_t lo;
+                int ls;
+                if (i < (uint64_t)j) {
+                    lo = i;
+                    ls = sl;
+                } else {
+                    lo = (uint64_t)j;
+                    ls = back_slot(L, i, sl);
+                }
+                uint32_t b[4];
+                draw(seed, (uint32_t)lo, (uint32_t)ls, (uint32_t)t, wword(t, 5, 0), b);
+                if (all || b[0] < thr) {
+                    in[j] = 1;
+                    queue[tail++] = (uint64_t)j;
+                }
+            }
+        }
+        for (uint64_t q = 0; q < tail; ++q) spins[queue[q]] = (int8_t)-ref;
+        if (sums) {
+            int64_t bond = 0, mag = 0;
+            for (uint64_t i = 0; i < n; ++i) {
+                mag += spins[i];
+                or_neighbours(L, i, nb);
+                for (int sl = 0; sl < z; ++sl)
+                    if (nb[sl] > (int64_t)i) bond += (int64_t)spins[i] * spins[nb[sl]];
+            }
+            int64_t* o = sums + 5 * k;
+            o[0] = bond;
+            o[1] = 0;
+            o[2] = 0;
+            o[3] = mag;
+            o[4] = (int64_t)n;
+        }
+    }
+    free(queue);
+    free(in);
+}
+
+/* ------------------------------------------------------------------------ */
 /* PCG64: 128-bit LCG with multiplier 0x2360ED051FC65DA44385DF649FCCF645,   */
 /* output XSL-RR of the stepped state (O'Neill, PCG, 2014).                  */
 /* ------------------------------------------------------------------------ */

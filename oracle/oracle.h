@@ -82,6 +82,13 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
                     uint64_t nsweeps, float* xyz, int64_t* sums);
 float or_exp_nonpos(float x);
 
+/* Wolff single-cluster steps (integrator.rs:146-230) with the engine's
+ * Philox draws; sums as above (bond = each bond once, mag = sum s).  The
+ * bond threshold: accept a bond iff its 32-bit draw < thr, or always. */
+void or_wolff_threshold(const or_model* M, uint32_t* thr, int* all);
+void or_wolff_steps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t step0,
+                    uint64_t nsteps, int8_t* spins, int64_t* sums);
+
 /* PCG64 XSL-RR 128/64: state and increment as (hi, lo) 64-bit halves. */
 typedef struct or_pcg64 {
     uint64_t state_hi, state_lo, inc_hi, inc_lo;

@@ -63,6 +63,9 @@ def lib():
         L.or_heis_init.argtypes = [P(Lattice), C.c_uint64, C.c_void_p]
         L.or_heis_sweeps.argtypes = [P(Lattice), P(Model), C.c_uint64, C.c_uint64, C.c_uint64,
                                      C.c_void_p, C.c_void_p]
+        L.or_wolff_threshold.argtypes = [P(Model), P(C.c_uint32), P(C.c_int)]
+        L.or_wolff_steps.argtypes = [P(Lattice), P(Model), C.c_uint64, C.c_uint64, C.c_uint64,
+                                     C.c_void_p, C.c_void_p]
         L.or_exp_nonpos.argtypes = [C.c_float]
         L.or_exp_nonpos.restype = C.c_float
         L.or_pcg64_seed.argtypes = [P(Pcg64), C.c_uint64]
@@ -151,6 +154,21 @@ def ising_sweeps(L: Lattice, M: Model, seed: int, sweep0: int, n: int, spins: np
     assert spins.dtype == np.int8 and spins.flags.c_contiguous
     sums = np.zeros((n, 5), dtype=np.int64)
     lib().or_ising_sweeps(C.byref(L), C.byref(M), seed, sweep0, n, _ptr(spins), _ptr(sums))
+    return sums
+
+
+def wolff_threshold(M: Model) -> tuple:
+    thr, al = C.c_uint32(), C.c_int()
+    lib().or_wolff_threshold(C.byref(M), C.byref(thr), C.byref(al))
+    return thr.value, bool(al.value)
+
+
+def wolff_steps(L: Lattice, M: Model, seed: int, step0: int, n: int, spins: np.ndarray):
+    """Wolff single-cluster steps on `spins` (natural order, int8 +-1) in
+    place; returns sums (n, 5) after each step."""
+    assert spins.dtype == np.int8 and spins.flags.c_contiguous
+    sums = np.zeros((n, 5), dtype=np.int64)
+    lib().or_wolff_steps(C.byref(L), C.byref(M), seed, step0, n, _ptr(spins), _ptr(sums))
     return sums
 
 

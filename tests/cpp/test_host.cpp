@@ -369,10 +369,28 @@ static void integrator_step_adapter() {
     bool same = true;
     for (size_t i = 0; i < 512; ++i) same &= (int8_t)next.at(i).value() == s[i];
     EXPECT(same);
+    // Wolff (integrator.rs:172-230): one step = one cluster, the oracle's
+    // cluster for the seed the adapter draws; Heisenberg Wolff is refused.
+    {
+        std::mt19937_64 r(3), r_copy(3);
+        const uint64_t wseed = r_copy();
+        const auto wnext = step(WolffIntegrator(), r, th, h, st);
+        std::vector<int8_t> w(512, 1);
+        or_wolff_steps(&ol, &om, wseed, 0, 1, w.data(), nullptr);
+        bool wsame = true;
+        int flipped = 0;
+        for (size_t i = 0; i < 512; ++i) {
+            wsame &= (int8_t)wnext.at(i).value() == w[i];
+            flipped += w[i] < 0;
+        }
+        EXPECT(wsame && flipped > 0);
+    }
     EXPECT(throws_kind(
         [&] {
             std::mt19937_64 r(1);
-            step(WolffIntegrator(), r, th, h, st);
+            const auto hs = State<HeisenbergSpin>::up_with_size(512);
+            step(WolffIntegrator(), r, Thermostat<HeisenbergSpin>(5.0, Field<HeisenbergSpin>::zero()),
+                 h, hs);
         },
         ErrorKind::NotImplemented));
 }

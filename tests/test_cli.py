@@ -158,7 +158,8 @@ def test_cli_argument_and_input_errors(tmp_path):
     assert r.returncode == 1 and r.stderr.startswith("Error: io error: std io error: ")
     r = _cli("run", "-", stdin="model = \"Ising\"\n")
     assert r.returncode == 1 and "missing field `algorithm`" in r.stderr
-    wolff = DOCS[0].replace("Metropolis", "Wolff")
+    # Heisenberg + Wolff is NotImplementedError (input.rs:365), before any sweep
+    wolff = DOCS[0].replace("Metropolis", "Wolff").replace("Ising", "Heisenberg")
     r = _cli("run", "-", stdin=wolff)
     assert r.returncode == 1 and r.stderr.strip() == "Error: not implemented error"
 

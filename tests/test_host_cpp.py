@@ -30,7 +30,8 @@ def test_host_library_gpu():
     """Machine + Programs + instruments over the engine: hook order, per-sweep
     records and final spins equal to the oracle's trajectory for the seed the
     Machine draws; StatSensor lines; StateSensor snapshots; the step()
-    adapter; Wolff and fcc rejected."""
+    adapter (Metropolis and Wolff steps equal the oracle's); Heisenberg
+    Wolff rejected."""
     _run("--gpu")
 
 

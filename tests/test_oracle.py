@@ -407,3 +407,53 @@ def test_heisenberg_flip_proposal_conserves_norm(orc):
     before = np.linalg.norm(xyz, axis=1)
     orc.heis_sweeps(L, M, 6, 0, 20, xyz)
     assert np.array_equal(np.linalg.norm(xyz, axis=1), before)
+
+
+# ------------------------------------------------------------ Wolff ----
+def test_wolff_threshold_closed_form(orc):
+    """Bond probability 1 - exp(-2J/T) (integrator.rs:188-192) as a 32-bit
+    threshold ceil(p 2^32); J = 0 gives no bonds, T -> 0 bonds always."""
+    for T, J in [(2.0, 1.0), (4.5115, 1.0), (0.7, 0.5), (10.0, 2.0)]:
+        thr, al = orc.wolff_threshold(orc.model(terms=("exchange",), exchange=J, temperature=T))
+        p = 1.0 - math.exp(-2.0 * J / T)
+        assert not al and thr == math.ceil(p * 2 ** 32)
+    assert orc.wolff_threshold(orc.model(terms=("zeeman",), temperature=1.0)) == (0, False)
+    assert orc.wolff_threshold(orc.model(terms=("exchange",), temperature=1e-6))[1]
+
+
+@pytest.mark.parametrize("kind,dims,pbc,T", [("sc", (4, 4, 1), (1, 1, 0), 2.3),
+                                             ("sc", (4, 2, 2), (1, 1, 1), 3.5),
+                                             ("bcc", (2, 2, 2), (1, 1, 1), 5.0),
+                                             ("fcc", (2, 2, 1), (1, 1, 0), 7.0)])
+def test_wolff_matches_exact_enumeration(orc, kind, dims, pbc, T):
+    """Exchange only (the Wolff move ignores the field): long-run <E> and
+    <|M|> of the oracle's Wolff steps equal the exact Boltzmann averages."""
+    L = orc.lattice(kind, *dims, pbc=pbc)
+    n = orc.num_sites(L)
+    neigh = [[j for j in orc.neighbours(L, i) if j >= 0] for i in range(n)]
+    M = orc.model(terms=("exchange",), temperature=T)
+    spins = orc.ising_init(L, 2)
+    orc.wolff_steps(L, M, 19, 0, 2000, spins)
+    sums = orc.wolff_steps(L, M, 19, 2000, 60000, spins)
+    bond, mag = sums[:, 0].astype(float), sums[:, 3].astype(float)
+    e = -bond  # Exchange alone: total = -J sum over bonds
+    ee, em, _, _ = ising_exact(neigh, n, 1.0, 0.0, 1.0, T, compound=False)
+    me, se = binned(e)
+    mm, sm = binned(np.abs(mag))
+    assert abs(me - ee) < 4 * se, (me, ee, se)
+    assert abs(mm - em) < 4 * sm, (mm, em, sm)
+
+
+def test_wolff_clusters_do_not_depend_on_visit_order(orc):
+    """The flipped set is the seed's component over active bonds: sums equal
+    a recomputation from the state, and a second run is identical."""
+    L = orc.lattice("sc", 6, 6, 6)
+    M = orc.model(terms=("exchange",), temperature=4.0)
+    a = orc.ising_init(L, 5)
+    b = a.copy()
+    sa = orc.wolff_steps(L, M, 7, 0, 50, a)
+    sb = orc.wolff_steps(L, M, 7, 0, 50, b)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    assert sa[-1, 3] == a.astype(np.int64).sum()
+    s = a.astype(np.int64).reshape(6, 6, 6)
+    assert sa[-1, 0] == sum(int((s * np.roll(s, 1, axis=ax)).sum()) for ax in range(3))

@@ -24,7 +24,7 @@ SC, BCC, FCC = 0, 1, 2
 PBC_X, PBC_Y, PBC_Z = 1, 2, 4
 EXCHANGE, ZEEMAN, ANISOTROPY, GAUGE = 1, 2, 4, 8
 ISING, ISING_BITS, HEISENBERG = 0, 1, 2
-FLIP, RANDOM = 0, 1
+FLIP, RANDOM, WOLFF = 0, 1, 2
 
 STATUS = {
     0: "OK", 1: "TemperatureMaxLessThanMin", 2: "NoSteps", 3: "ZeroTemperature",

@@ -47,8 +47,16 @@ int validate_desc(const vg_chain_desc& d) {
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown spin kind %d", d.spin);
     if (d.spin == VG_SPIN_ISING_BITS && L.kind != VG_LATTICE_SC)
         return fail(VG_ERR_NOT_IMPLEMENTED, "bit-packed Ising chains run on sc lattices");
-    if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM)
+    if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM &&
+        d.proposal != VG_PROPOSAL_WOLFF)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown proposal %d", d.proposal);
+    if (d.proposal == VG_PROPOSAL_WOLFF) {
+        // integrator.rs:172-230 implements Wolff for Ising spins only
+        if (d.spin != VG_SPIN_ISING)
+            return fail(VG_ERR_NOT_IMPLEMENTED, "Wolff cluster steps run on int8 Ising chains");
+        if (sites_of(L) >= (1ull << 32))
+            return fail(VG_ERR_INVALID_ARGUMENT, "Wolff chains need fewer than 2^32 sites");
+    }
     const vg_hamiltonian& H = d.hamiltonian;
     if (H.terms & ~15u) return fail(VG_ERR_INVALID_ARGUMENT, "unknown term bits 0x%x", H.terms);
     if (H.terms == 0) return fail(VG_ERR_INVALID_ARGUMENT, "hamiltonian has no terms");
@@ -131,10 +139,28 @@ void heis_params(const ChainHost& ch, ChainDev& d) {
     d.fz = (float)ch.field_dir[2];
 }
 
+// Wolff bond threshold: a bond is active iff its 32-bit draw is below
+// ceil(p 2^32), p = 1 - exp(-2J/T) in f64 (integrator.rs:188-192).
+void wolff_threshold(const ChainHost& ch, ChainDev& d) {
+    const vg_hamiltonian& H = ch.desc.hamiltonian;
+    const double J = (H.terms & VG_TERM_EXCHANGE) ? H.exchange : 0.0;
+    const double p = 1.0 - std::exp(-2.0 * J / ch.temperature);
+    d.wolff_thr = 0;
+    d.wolff_all = 0;
+    if (!(p > 0.0)) return;
+    const double th = std::ceil(std::ldexp(p, 32));
+    if (th >= 4294967296.0)
+        d.wolff_all = 1;
+    else
+        d.wolff_thr = (uint32_t)th;
+}
+
 void refresh_params(vg_batch* B, uint32_t i) {
     ChainHost& ch = B->chains[i];
     ChainDev& d = B->groups[ch.group].host[ch.index];
-    if (is_ising(ch.group))
+    if (ch.group == kIsingWolff)
+        wolff_threshold(ch, d);
+    else if (is_ising(ch.group))
         ising_thresholds(ch, d);
     else
         heis_params(ch, d);
@@ -158,6 +184,12 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
     d.proposal = (uint32_t)ch.desc.proposal;
     d.slot = slot;
     d.limit = 0xFFFFFFFFu;
+    d.nsites = sites_of(L);
+    if (g == kIsingWolff) {  // natural order; no planes or work items
+        d.plane = d.nsites;
+        wolff_threshold(ch, d);
+        return VG_OK;
+    }
     const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
     const uint32_t align = is_ising(g) ? 64u : 16u;  // 64-byte aligned rows
     if (g == kIsingBits)  // bytes: whole 16-position segments, 16-byte aligned rows
@@ -219,7 +251,9 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
     for (size_t k = 0; k < G.members.size(); ++k) {
         ChainDev& d = G.host[k];
         uint64_t items;
-        if (is_zmarch(g)) {
+        if (g == kIsingWolff) {
+            items = 1;  // one workgroup per chain
+        } else if (is_zmarch(g)) {
             d.nzc = (d.lz + tile - 1) / tile;
             items = (uint64_t)d.nzc * 2 * d.hy * d.segs;
         } else {
@@ -244,8 +278,8 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
     G.tile = tile;
     G.total_blocks = (uint32_t)block_chain.size();
     G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
-    G.partial_bytes_per_sweep =
-        (uint64_t)colours_of(g) * G.total_blocks * quantities(g) * quantity_bytes(g);
+    G.partial_bytes_per_sweep = std::max<uint64_t>(
+        256, (uint64_t)colours_of(g) * G.total_blocks * quantities(g) * quantity_bytes(g));
     for (auto& kv : B->graphs) quiet(hipGraphExecDestroy(kv.second));
     B->graphs.clear();
     return VG_OK;
@@ -367,7 +401,10 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
         Group& G = B->groups[g];
         if (G.members.empty()) continue;
         dim3 grid((uint32_t)G.members.size(), nsw);
-        if (is_ising(g))
+        if (g == kIsingWolff)
+            hipLaunchKernelGGL(vg::wolff_run, dim3((uint32_t)G.members.size()), dim3(256), 0, st,
+                               G.d_chains, (uint8_t*)G.arena, nsw, B->d_records);
+        else if (is_ising(g))
             hipLaunchKernelGGL(vg::ising_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
                                (const int32_t*)G.partials, G.total_blocks, colours_of(g), nsw,
                                B->d_records);
@@ -668,6 +705,15 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
                 return rc;
             }
             const uint64_t elem_align = 256 / elem_bytes(g);  // 256-byte chain alignment
+            if (g == kIsingWolff) {  // bytes: spins, stamps, queue
+                const uint64_t sizes[3] = {d.nsites, 4 * d.nsites, 4 * d.nsites};
+                for (int part = 0; part < 3; ++part) {
+                    offset = round_up(offset, 256);
+                    d.off[part] = offset;
+                    offset += sizes[part];
+                }
+                continue;
+            }
             for (uint32_t c = 0; c < d.ncol; ++c) {
                 offset = round_up(offset, elem_align);
                 d.off[c] = offset;
@@ -793,7 +839,10 @@ int vg_batch_randomize(vg_batch* batch) {
         Group& G = batch->groups[ch.group];
         const ChainDev& d = G.host[ch.index];
         const ChainDev* dp = G.d_chains + ch.index;
-        if (is_ising(ch.group)) {
+        if (ch.group == kIsingWolff) {
+            hipLaunchKernelGGL(vg::wolff_init, dim3((uint32_t)((d.nsites + 255) / 256)), dim3(256),
+                               0, st, dp, (uint8_t*)G.arena);
+        } else if (is_ising(ch.group)) {
             launch_ising_init(ch.group, G, d, dp, st);
         } else {
             const uint64_t n = (uint64_t)d.ncol * d.rows * d.w;
@@ -822,6 +871,12 @@ int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
     VG_HIP(hipSetDevice(batch->ctx->device));
     rc = upload_descriptors(batch, ch.group);
     if (rc) return rc;
+    if (ch.group == kIsingWolff) {  // natural order, int8 +-1
+        VG_HIP(hipMemcpyAsync((uint8_t*)G.arena + G.host[ch.index].off[0], spins, ch.n_sites,
+                              hipMemcpyHostToDevice, st));
+        VG_HIP(hipStreamSynchronize(st));
+        return VG_OK;
+    }
     int8_t* tmp = nullptr;
     VG_HIP(hipMalloc(&tmp, ch.n_sites));
     hipError_t e = hipMemcpyAsync(tmp, spins, ch.n_sites, hipMemcpyHostToDevice, st);
@@ -854,6 +909,12 @@ int vg_batch_get_ising(vg_batch* batch, uint32_t chain, int8_t* spins) {
     VG_HIP(hipSetDevice(batch->ctx->device));
     rc = upload_descriptors(batch, ch.group);
     if (rc) return rc;
+    if (ch.group == kIsingWolff) {
+        VG_HIP(hipMemcpyAsync(spins, (const uint8_t*)G.arena + G.host[ch.index].off[0],
+                              ch.n_sites, hipMemcpyDeviceToHost, st));
+        VG_HIP(hipStreamSynchronize(st));
+        return VG_OK;
+    }
     int8_t* tmp = nullptr;
     VG_HIP(hipMalloc(&tmp, ch.n_sites));
     if (ch.group == kIsingBits)

@@ -45,6 +45,8 @@ __global__ void ising_fcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32
 template <uint32_t A>
 __global__ void heis_fcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                               uint32_t, uint32_t);
+__global__ void wolff_run(const ChainDev*, uint8_t*, uint32_t, int64_t*);
+__global__ void wolff_init(const ChainDev*, uint8_t*);
 __global__ void heis_clear(const ChainDev*, float*);
 __global__ void heis_init(const ChainDev*, float*);
 __global__ void heis_scatter(const ChainDev*, float*, const double*, uint64_t);
@@ -84,19 +86,25 @@ enum GroupKind {
     kIsingBits = 4,  // sc, 1 bit per site
     kIsingFcc = 5,
     kHeisFcc = 6,
-    kNumGroups = 7
+    kIsingWolff = 7,  // Wolff cluster steps, any lattice, natural site order
+    kNumGroups = 8
 };
 
 inline bool is_ising(int g) {
-    return g == kIsingSc || g == kIsingBcc || g == kIsingBits || g == kIsingFcc;
+    return g == kIsingSc || g == kIsingBcc || g == kIsingBits || g == kIsingFcc ||
+           g == kIsingWolff;
 }
 // Groups whose pass marches a tile of z planes per work item.
 inline bool is_zmarch(int g) { return g == kIsingSc || g == kIsingBits; }
-// Colours (planes, passes per sweep) of a group's chains.
-inline uint32_t colours_of(int g) { return g == kIsingFcc || g == kHeisFcc ? 4u : 2u; }
+// Colours (planes, passes per sweep) of a group's chains; Wolff chains
+// have no colour passes.
+inline uint32_t colours_of(int g) {
+    return g == kIsingWolff ? 0u : (g == kIsingFcc || g == kHeisFcc ? 4u : 2u);
+}
 
 inline int group_of(const vg_chain_desc& d) {
     const int k = d.lattice.kind;
+    if (d.proposal == VG_PROPOSAL_WOLFF) return kIsingWolff;
     if (d.spin == VG_SPIN_HEISENBERG)
         return k == VG_LATTICE_FCC ? kHeisFcc : (k == VG_LATTICE_BCC ? kHeisBcc : kHeisSc);
     if (d.spin == VG_SPIN_ISING_BITS) return kIsingBits;

@@ -79,6 +79,8 @@ enum Purpose : uint32_t {
     kIsingProposal = 2,  // random-proposal bit, 128 positions per call
     kIsingInit = 3,      // initial spin, 4 positions per call
     kHeisInit = 4,       // initial spin stream, one position per stream
+    kWolffBond = 5,      // Wolff: a bond's activation draw, keyed by its lower site and slot
+    kWolffSeed = 6,      // Wolff: the cluster's seed site, one call per step
     kHeisStream = 16,    // Heisenberg sweep: proposal call k of a site uses purpose 16 + k
 };
 constexpr uint32_t kHeisMaxCalls = 16;
@@ -143,6 +145,10 @@ struct ChainDev {
     float ax, ay, az;      // anisotropy axis
     float fx, fy, fz;      // field orientation
     float pad1, pad2;
+    // Wolff chains (natural site order): sites, and the bond threshold --
+    // a bond is active iff its 32-bit draw < wolff_thr, or always.
+    uint64_t nsites;
+    uint32_t wolff_thr, wolff_all;
 };
 
 constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass

@@ -277,15 +277,19 @@ inline Lattice Input::lattice() const {
 }
 
 // Input::run (input.rs:347-367): Ising + Metropolis runs the flip
-// integrator, Heisenberg + Metropolis the fresh-proposal one; Wolff is not
-// implemented (for Heisenberg the reference does not implement it either).
+// integrator, Ising + Wolff the cluster one, Heisenberg + Metropolis the
+// fresh-proposal one; Heisenberg + Wolff is NotImplementedError, as in the
+// reference.
 template <class Rng>
 void Input::run(Rng& rng, std::ostream& out) const {
-    if (algorithm == Algorithm::Wolff) throw Error(ErrorKind::NotImplemented);
-    if (model == Model::Ising)
+    if (model == Model::Ising && algorithm == Algorithm::Metropolis)
         run_with_spin<IsingSpin, MetropolisFlipIntegrator>(rng, out);
-    else
+    else if (model == Model::Ising)
+        run_with_spin<IsingSpin, WolffIntegrator>(rng, out);
+    else if (algorithm == Algorithm::Metropolis)
         run_with_spin<HeisenbergSpin, MetropolisIntegrator>(rng, out);
+    else
+        throw Error(ErrorKind::NotImplemented);
 }
 
 // Input::run_with_spin (input.rs:264-294): Compound(Exchange, Zeeman) on the

@@ -32,8 +32,10 @@ struct MetropolisIntegrator {
 struct MetropolisFlipIntegrator {
     static constexpr int proposal = VG_PROPOSAL_FLIP;
 };
+// One "step" of a Wolff machine is one cluster grown and flipped
+// (integrator.rs:172-230); Ising spins only.
 struct WolffIntegrator {
-    static constexpr int proposal = -1;  // cluster moves are not a checkerboard sweep
+    static constexpr int proposal = VG_PROPOSAL_WOLFF;
 };
 
 // A process-wide engine context on device 0.
@@ -52,8 +54,6 @@ class Chain {
   public:
     Chain(const vg_lattice& lattice, int proposal, const vg_hamiltonian& h, uint64_t seed,
           vg_ctx* ctx = default_context()) {
-        if (proposal < 0)
-            throw Error(ErrorKind::NotImplemented, "Wolff cluster updates are not implemented");
         vg_chain_desc d{};
         d.lattice = lattice;
         d.spin = S::kind;

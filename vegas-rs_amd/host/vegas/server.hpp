@@ -62,8 +62,9 @@ class JobServer {
     // State::rand_with_size drawn from `seed`.
     template <class S, class I, class H, class P>
     uint64_t submit(const H& h, const I&, const P& program, uint64_t seed, size_t n_sites = 0) {
-        if (I::proposal < 0)
-            throw Error(ErrorKind::NotImplemented, "Wolff cluster updates are not implemented");
+        if (I::proposal == VG_PROPOSAL_WOLFF)
+            throw Error(ErrorKind::NotImplemented,
+                        "the job server runs Metropolis chains; Wolff chains run in a Machine");
         vg_job_desc job{};
         job.chain.lattice = lattice_of(h, n_sites);
         job.chain.spin = S::kind;
