@@ -16,8 +16,10 @@ sequential MetropolisFlipIntegrator, one core, a bounded sample).
 With the default --workload all, the line also carries the other forms of
 the path as sub-objects: the same chains bit-packed, the 512-chain mixed
 batch against per-chain plain launches, Heisenberg sc 128^3 x 3, Ising and
-Heisenberg chains on fcc lattices (four colour passes per sweep), and the
-served Poisson job stream with p50/p99 latency (--serve-jobs jobs per rate).
+Heisenberg chains on fcc lattices (four colour passes per sweep), Wolff
+cluster steps (batched and single chain, with the oracle's Wolff on one
+core), and the served Poisson job stream with p50/p99 latency (--serve-jobs
+jobs per rate).
 """
 import argparse
 import importlib.util
@@ -288,6 +290,63 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps):
     return out
 
 
+def run_wolff(vg, ctx, batch_side, batch_chains, single_side, steps, cpu_seconds=5.0):
+    """Wolff cluster steps (integrator.rs:146-230) at T_c = 4.5115 of the sc
+    Ising model, J = 1: batch_chains chains of batch_side^3 (one workgroup
+    per chain) and one chain of single_side^3, random starts, 1000 steps of
+    warm-up (the single large chain is still far from equilibrium then, so
+    its clusters are small: the figure is the per-step cost).  Reports cluster steps/s and flipped sites/s (a cluster's size
+    is half the change of the signed magnetization), and the oracle's Wolff
+    (the same clusters) on one core for the single chain as the CPU figure."""
+    import numpy as np
+    T = 4.5115
+    out = {}
+    for name, side, nch in (("batched", batch_side, batch_chains), ("single", single_side, 1)):
+        L = vg.lattice(vg.SC, side, side, side)
+        descs = [vg.chain(L, vg.ISING, vg.WOLFF, vg.hamiltonian(exchange=1.0), s)
+                 for s in range(nch)]
+        with vg.Batch(ctx, descs) as b:
+            b.randomize()
+            for i in range(nch):
+                b.set_thermostat(i, T)
+            b.set_graph(True)
+            b.run(1000, observe=False)
+            m0 = b.run_sums(1)[:, -1, 3]
+            t0 = time.perf_counter()
+            sums = b.run_sums(steps)
+            wall = time.perf_counter() - t0
+        mags = np.concatenate([m0[:, None], sums[:, :, 3]], axis=1)
+        flipped = int(np.abs(np.diff(mags, axis=1)).sum() // 2)
+        out[name] = {
+            "workload": f"{nch} chains sc {side}^3 at T={T}, {steps} Wolff steps each",
+            "steps_per_s": nch * steps / wall,
+            "flipped_sites_per_s": flipped / wall,
+            "mean_cluster_size": flipped / (nch * steps),
+            "ms_per_step": wall * 1e3 / steps,
+        }
+        log(f"wolff {name}: {nch * steps / wall:.3e} steps/s, {flipped / wall:.3e} flipped sites/s, "
+            f"mean cluster {flipped / (nch * steps):.1f}")
+    # CPU: the oracle's Wolff on the single chain, for about cpu_seconds
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    OL = orc.lattice("sc", single_side, single_side, single_side)
+    OM = orc.model(terms=("exchange",), temperature=T)
+    sp = orc.ising_init(OL, 0)
+    orc.wolff_steps(OL, OM, 0, 0, 200, sp)
+    done, flipped, t0 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t0 < cpu_seconds:
+        m = orc.wolff_steps(OL, OM, 0, 200 + done, 50, sp)[:, 3]
+        flipped += int(np.abs(np.diff(m)).sum() // 2)
+        done += 50
+    dt = time.perf_counter() - t0
+    out["cpu"] = {"steps_per_s": done / dt, "flipped_sites_per_s": flipped / dt, "cores": 1,
+                  "kind": "port",
+                  "sample": (f"{done} Wolff steps of sc {single_side}^3 at T={T} (oracle "
+                             f"restatement of WolffIntegrator::step, 1 thread on {cpu_model()})")}
+    log(f"wolff cpu: {done / dt:.3e} steps/s, {flipped / dt:.3e} flipped sites/s")
+    return out
+
+
 def served_jobs(vg, n_jobs, gen_seed=2025):
     """The served mix: 60% Ising Relax (sc L^3, L even in [16, 96], T uniform in
     [2, 6], 2000 sweeps), 30% Heisenberg CoolDown (sc L^3, L even in [16, 64],
@@ -427,7 +486,7 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="Ising tile; 0 = autotune")
     ap.add_argument("--workload", default="all",
                     choices=["all", "headline", "batched", "served", "heisenberg", "bitpacked",
-                             "fcc"],
+                             "fcc", "wolff"],
                     help="headline: sc 256^3 x 3 only; batched: the 512-chain batch only; "
                          "served: the Poisson job stream only")
     ap.add_argument("--serve-jobs", type=int, default=400,
@@ -438,6 +497,7 @@ def main():
     ap.add_argument("--fcc-heis-cells", type=int, default=80,
                     help="fcc Heisenberg chains: cells per side")
     ap.add_argument("--fcc-sweeps", type=int, default=300)
+    ap.add_argument("--wolff-steps", type=int, default=2000)
     ap.add_argument("--serve-rates", default="50,500")
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="server per-sweep work budget (Ising-site units; 0 = none)")
@@ -470,6 +530,10 @@ def main():
         with vg.Context(0) as ctx:
             print(json.dumps(run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)),
                   flush=True)
+        return
+    if args.workload == "wolff":
+        with vg.Context(0) as ctx:
+            print(json.dumps(run_wolff(vg, ctx, 32, 64, 64, args.wolff_steps)), flush=True)
         return
     if args.workload == "fcc":
         with vg.Context(0) as ctx:
@@ -588,6 +652,7 @@ def main():
         out["batched"] = run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps)
         out["heisenberg"] = run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)
         out["fcc"] = run_fcc(vg, ctx, args.fcc_cells, args.fcc_heis_cells, args.fcc_sweeps)
+        out["wolff"] = run_wolff(vg, ctx, 32, 64, 64, args.wolff_steps)
         rates = [float(r) for r in args.serve_rates.split(",")]
         out["served"] = run_served(vg, ctx, args.serve_jobs, rates, budget=args.serve_budget)
     ctx.close()

@@ -247,3 +247,18 @@ def test_cli_run_reports_program_errors_when_the_stage_starts(tmp_path):
     r = _cli("run", "-", "--seed", "1", stdin=doc, cwd=tmp_path)
     assert r.returncode == 1
     assert r.stderr.strip() == "Error: program error: cooling rate must be greater than zero"
+
+
+@pytest.mark.gpu
+def test_cli_runs_ising_wolff(tmp_path):
+    """`algorithm = "Wolff"` with Ising spins runs the cluster integrator
+    (input.rs:351-355): one Stat line per CoolDown temperature and one
+    observable row per Wolff step."""
+    pa = pytest.importorskip("pyarrow.parquet")
+    doc = RUN_DOC.format(model="Ising", cell="bcc", dir=tmp_path).replace(
+        "algorithm = \"Metropolis\"", "algorithm = \"Wolff\"")
+    r = _cli("run", "-", "--seed", "5", stdin=doc, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert len(r.stdout.strip().splitlines()) == 3
+    obs = pa.read_table(tmp_path / "obs.parquet").to_pydict()
+    assert len(obs["step"]) == 200 + 3 * (50 + 100)
