@@ -89,17 +89,96 @@ def cpu_baseline(L_side, T, sweeps):
     }
 
 
+def cpu_threads():
+    """Threads for the all-cores CPU lines: the OpenMP setting when given,
+    else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_best_line(L_side, T, sweeps_one, sweeps_all, threads):
+    """The build's own CPU checkerboard (the oracle's fast sc path, the same
+    draws as the GPU) on one core and on `threads` cores: the best-CPU line."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+
+    OL = orc.lattice("sc", L_side, L_side, L_side)
+    OM = orc.model(terms=("exchange",), temperature=T)
+    spins = orc.ising_init(OL, 7)
+    n = orc.num_sites(OL)
+    out = {}
+    for label, sw, th in (("one_core", sweeps_one, 1), ("all_cores", sweeps_all, threads)):
+        t0 = time.perf_counter()
+        orc.ising_sweeps_sc_fast(OL, OM, 7, 0, sw, spins, th)
+        dt = time.perf_counter() - t0
+        out[label] = {"value": n * sw / dt, "cores": th, "sweeps": sw}
+    out["unit"] = "spin-flip attempts/s"
+    out["sample"] = (f"sc {L_side}^3 Ising checkerboard, Philox draws as on the GPU, T={T}; "
+                     f"oracle fast path (OpenMP) on {cpu_model()}")
+    return out
+
+
+def _sequential_worker(job):
+    """One chain of the batched set under the oracle's sequential
+    restatement for about `seconds`: (attempts, seconds)."""
+    side, T, h, seed, seconds = job
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+
+    OL = orc.lattice("sc", side, side, side)
+    hdir = (0.0, 0.0, 1.0 if h >= 0 else -1.0)
+    OM = orc.model(terms=("exchange", "zeeman"), temperature=T, field=abs(h), field_dir=hdir)
+    s = orc.Sequential(OL, OM, seed)
+    s.set_thermostat(T, abs(h), hdir)
+    s.set_state(orc.ising_init(OL, seed))
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        s.sweeps(1, observe=False)
+        done += 1
+    dt = time.perf_counter() - t0
+    s.close()
+    return side ** 3 * done, dt
+
+
+def cpu_batched_aggregate(procs, seconds=3.0, n_chains=512):
+    """The batched set on the CPU: min(procs, 512) concurrent processes, each
+    running one chain under the oracle's sequential restatement of the
+    reference's Metropolis (one core each); the aggregate attempts/s.  Runs
+    before the GPU is touched, so the worker processes fork from a process
+    without device state."""
+    import multiprocessing as mp
+    sides, temps, fields = batched_params(n_chains)
+    k = min(procs, n_chains)
+    jobs = [(int(sides[i]), float(temps[i]), float(fields[i]), i, seconds) for i in range(k)]
+    with mp.get_context("fork").Pool(k) as pool:
+        res = pool.map(_sequential_worker, jobs)
+    span = max(t for _, t in res)
+    total = sum(a for a, _ in res)
+    return {"value": total / span, "unit": "spin-flip attempts/s", "cores": k, "kind": "port",
+            "sample": (f"chains 0..{k - 1} of the batched set, one process each, {seconds:.0f} s of "
+                       f"sweeps of the oracle's sequential MetropolisFlipIntegrator restatement "
+                       f"on {cpu_model()}")}
+
+
+def batched_params(n_chains=512, gen_seed=2024):
+    """L, T and h of the batched set (numpy PCG64 seeded with gen_seed)."""
+    import numpy as np
+    rng = np.random.default_rng(gen_seed)
+    sides = rng.choice(np.array([16, 24, 32, 48, 64, 96, 128]), size=n_chains)
+    temps = rng.uniform(3.5, 5.5, size=n_chains)
+    fields = rng.uniform(-0.1, 0.1, size=n_chains)
+    return sides, temps, fields
+
+
 def batched_chains(vg, n_chains=512, gen_seed=2024):
     """The mixed-size batch: n_chains Ising chains on cubic sc lattices, L
     uniform in {16, 24, 32, 48, 64, 96, 128} (full PBC, J = 1), T uniform in
     [3.5, 5.5], h uniform in [-0.1, 0.1] (a negative h is a Down-oriented
     field of magnitude |h|), Compound(Exchange, Zeeman), chain seeds
     0..n_chains-1.  L, T and h come from numpy's PCG64 seeded with gen_seed."""
-    import numpy as np
-    rng = np.random.default_rng(gen_seed)
-    sides = rng.choice(np.array([16, 24, 32, 48, 64, 96, 128]), size=n_chains)
-    temps = rng.uniform(3.5, 5.5, size=n_chains)
-    fields = rng.uniform(-0.1, 0.1, size=n_chains)
+    sides, temps, fields = batched_params(n_chains, gen_seed)
     H = vg.hamiltonian(exchange=1.0, zeeman=True)
     descs = [vg.chain(vg.lattice(vg.SC, int(L), int(L), int(L)), vg.ISING, vg.FLIP, H, i)
              for i, L in enumerate(sides)]
@@ -108,7 +187,7 @@ def batched_chains(vg, n_chains=512, gen_seed=2024):
     return sides, descs, therm
 
 
-def run_batched(vg, ctx, sweeps, baseline_sweeps, n_chains=512):
+def run_batched(vg, ctx, sweeps, baseline_sweeps, n_chains=512, cpu=None):
     """All chains in one batch (one launch per colour pass for every chain,
     sweep blocks replayed from hipGraphs, tile autotuned) vs the same chains
     run one after another, each in its own batch with plain launches."""
@@ -153,6 +232,7 @@ def run_batched(vg, ctx, sweeps, baseline_sweeps, n_chains=512):
         "per_chain_launches": per_chain,
         "per_chain_sweeps": baseline_sweeps,
         "speedup": batched / per_chain,
+        "cpu_all_cores": cpu,
     }
 
 
@@ -520,11 +600,17 @@ def main():
               f"fits one MI355X and multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
 
+    # The batched set's all-cores CPU aggregate forks worker processes, so it
+    # runs before anything touches the GPU.
+    cpu_batched = None
+    if not args.no_cpu and args.workload in ("all", "batched"):
+        log(f"bench: CPU aggregate of the batched set ({cpu_threads()} processes)")
+        cpu_batched = cpu_batched_aggregate(cpu_threads())
     vg = load_engine()
     if args.workload == "batched":
         with vg.Context(0) as ctx:
-            print(json.dumps(run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps)),
-                  flush=True)
+            print(json.dumps(run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps,
+                                         cpu=cpu_batched)), flush=True)
         return
     if args.workload == "heisenberg":
         with vg.Context(0) as ctx:
@@ -602,6 +688,8 @@ def main():
     if not args.no_cpu:
         log(f"bench: CPU baseline ({args.cpu_sweeps} sweeps at sc {side}^3)")
         cpu = cpu_baseline(side, 4.5115, args.cpu_sweeps)
+        log("bench: best-CPU line (the oracle's fast checkerboard, 1 core and all cores)")
+        cpu["best_cpu_checkerboard"] = cpu_best_line(side, 4.5115, 2, 10, cpu_threads())
 
     out = {
         "metric": METRIC,
@@ -649,7 +737,8 @@ def main():
     if args.workload == "all":
         # The other forms of the path, each its own sub-object (not `value`).
         out["bitpacked"] = run_bitpacked(vg, ctx, side, args.steps)
-        out["batched"] = run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps)
+        out["batched"] = run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps,
+                                     cpu=cpu_batched)
         out["heisenberg"] = run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)
         out["fcc"] = run_fcc(vg, ctx, args.fcc_cells, args.fcc_heis_cells, args.fcc_sweeps)
         out["wolff"] = run_wolff(vg, ctx, 32, 64, 64, args.wolff_steps)

@@ -287,6 +287,98 @@ void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint
     }
 }
 
+/* The same sc checkerboard, written for speed rather than generality: the
+ * CPU baseline's best-CPU line (one Philox call per 16 positions for the
+ * high bytes, as the GPU does; rows of a colour shared among `threads`
+ * OpenMP threads).  Every draw is the one or_ising_sweeps makes, so the
+ * result equals it bit for bit for any thread count. */
+void or_ising_sweeps_sc_fast(const or_lattice* L, const or_model* M, uint64_t seed,
+                             uint64_t sweep0, uint64_t nsweeps, int8_t* spins, int64_t* sums,
+                             int threads) {
+    const uint32_t lx = L->lx, ly = L->ly, lz = L->lz, w = lx / 2, rows = ly * lz;
+    const int px = L->pbc & 1, py = (L->pbc >> 1) & 1, pz = (L->pbc >> 2) & 1;
+    uint32_t thr[2 * OR_THR_STRIDE], always[2];
+    or_ising_table(L, M, thr, always);
+    const int64_t sx = 1, sy = lx, sz = (int64_t)lx * ly;
+    for (uint64_t k = 0; k < nsweeps; ++k) {
+        const uint64_t t = sweep0 + k;
+        for (uint32_t c = 0; c < 2; ++c) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+            for (uint32_t row = 0; row < rows; ++row) {
+                const uint32_t y = row % ly, zz = row / ly;
+                const int64_t base = sy * y + sz * zz;
+                /* neighbour offsets along y and z, 0 when absent (open) */
+                const int ym = y > 0 || py, yp = y + 1 < ly || py;
+                const int zm = zz > 0 || pz, zp = zz + 1 < lz || pz;
+                const int64_t dym = y > 0 ? -sy : sy * (ly - 1), dyp = y + 1 < ly ? sy : -sy * (ly - 1);
+                const int64_t dzm = zz > 0 ? -sz : sz * (lz - 1), dzp = zz + 1 < lz ? sz : -sz * (lz - 1);
+                const uint32_t off = (c + y + zz) & 1u;
+                uint32_t ra[4] = {0, 0, 0, 0}, rp[4] = {0, 0, 0, 0};
+                for (uint32_t pos = 0; pos < w; ++pos) {
+                    if ((pos & 15) == 0) draw(seed, pos >> 4, row, (uint32_t)t, wword(t, 0, c), ra);
+                    if (M->proposal == 1 && (pos & 127) == 0)
+                        draw(seed, pos >> 7, row, (uint32_t)t, wword(t, 2, c), rp);
+                    const uint32_t x = 2 * pos + off;
+                    const int64_t site = base + x;
+                    const int xm = x > 0 || px, xp = x + 1 < lx || px;
+                    const int64_t dxm = x > 0 ? -sx : sx * (lx - 1), dxp = x + 1 < lx ? sx : -sx * (lx - 1);
+                    uint32_t S = 0;
+                    S += xm ? (spins[site + dxm] > 0 ? 2u : 0u) : 1u;
+                    S += xp ? (spins[site + dxp] > 0 ? 2u : 0u) : 1u;
+                    S += ym ? (spins[site + dym] > 0 ? 2u : 0u) : 1u;
+                    S += yp ? (spins[site + dyp] > 0 ? 2u : 0u) : 1u;
+                    S += zm ? (spins[site + dzm] > 0 ? 2u : 0u) : 1u;
+                    S += zp ? (spins[site + dzp] > 0 ? 2u : 0u) : 1u;
+                    const uint32_t b = spins[site] > 0;
+                    if (M->proposal == 1 && ((rp[(pos >> 5) & 3] >> (pos & 31)) & 1u) != b) continue;
+                    int flip;
+                    if ((always[b] >> S) & 1u) {
+                        flip = 1;
+                    } else {
+                        const uint32_t th = thr[b * OR_THR_STRIDE + S];
+                        const uint32_t hi8 = (ra[(pos & 15) >> 2] >> (8 * (pos & 3))) & 0xFFu;
+                        if (hi8 != th >> 24) {
+                            flip = hi8 < th >> 24;
+                        } else {
+                            uint32_t rb[4];
+                            draw(seed, pos >> 2, row, (uint32_t)t, wword(t, 1, c), rb);
+                            flip = (rb[pos & 3] >> 8) < (th & 0xFFFFFFu);
+                        }
+                    }
+                    if (flip) spins[site] = (int8_t)-spins[site];
+                }
+            }
+        }
+        if (sums) {
+            int64_t bond = 0, mag = 0;
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : bond, mag)
+            for (uint32_t row = 0; row < rows; ++row) {
+                const uint32_t y = row % ly, zz = row / ly;
+                const int64_t base = sy * y + sz * zz;
+                for (uint32_t x = 0; x < lx; ++x) {
+                    const int64_t site = base + x;
+                    mag += spins[site];
+                    if (((x + y + zz) & 1u) != 1u) continue;  /* bonds from colour-1 sites */
+                    int64_t nbsum = 0;
+                    if (x > 0 || px) nbsum += spins[x > 0 ? site - sx : site + sx * (lx - 1)];
+                    if (x + 1 < lx || px) nbsum += spins[x + 1 < lx ? site + sx : site - sx * (lx - 1)];
+                    if (y > 0 || py) nbsum += spins[y > 0 ? site - sy : site + sy * (ly - 1)];
+                    if (y + 1 < ly || py) nbsum += spins[y + 1 < ly ? site + sy : site - sy * (ly - 1)];
+                    if (zz > 0 || pz) nbsum += spins[zz > 0 ? site - sz : site + sz * (lz - 1)];
+                    if (zz + 1 < lz || pz) nbsum += spins[zz + 1 < lz ? site + sz : site - sz * (lz - 1)];
+                    bond += spins[site] * nbsum;
+                }
+            }
+            int64_t* o = sums + 5 * k;
+            o[0] = bond;
+            o[1] = 0;
+            o[2] = 0;
+            o[3] = mag;
+            o[4] = (int64_t)lx * ly * lz;
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Heisenberg checkerboard (fp32; every line is one IEEE operation)          */
 /* ------------------------------------------------------------------------ */

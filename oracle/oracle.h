@@ -77,6 +77,11 @@ void or_ising_table(const or_lattice* L, const or_model* M, uint32_t* thr, uint3
 void or_ising_init(const or_lattice* L, uint64_t seed, int8_t* spins);
 void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t sweep0,
                      uint64_t nsweeps, int8_t* spins, int64_t* sums);
+/* The sc Ising checkerboard again, fast and multi-threaded (OpenMP): the
+ * CPU baseline's best-CPU line; equal to or_ising_sweeps bit for bit. */
+void or_ising_sweeps_sc_fast(const or_lattice* L, const or_model* M, uint64_t seed,
+                             uint64_t sweep0, uint64_t nsweeps, int8_t* spins, int64_t* sums,
+                             int threads);
 void or_heis_init(const or_lattice* L, uint64_t seed, float* xyz);
 void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t sweep0,
                     uint64_t nsweeps, float* xyz, int64_t* sums);

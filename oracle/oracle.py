@@ -63,6 +63,8 @@ def lib():
         L.or_heis_init.argtypes = [P(Lattice), C.c_uint64, C.c_void_p]
         L.or_heis_sweeps.argtypes = [P(Lattice), P(Model), C.c_uint64, C.c_uint64, C.c_uint64,
                                      C.c_void_p, C.c_void_p]
+        L.or_ising_sweeps_sc_fast.argtypes = [P(Lattice), P(Model), C.c_uint64, C.c_uint64,
+                                              C.c_uint64, C.c_void_p, C.c_void_p, C.c_int]
         L.or_wolff_threshold.argtypes = [P(Model), P(C.c_uint32), P(C.c_int)]
         L.or_wolff_steps.argtypes = [P(Lattice), P(Model), C.c_uint64, C.c_uint64, C.c_uint64,
                                      C.c_void_p, C.c_void_p]
@@ -154,6 +156,16 @@ def ising_sweeps(L: Lattice, M: Model, seed: int, sweep0: int, n: int, spins: np
     assert spins.dtype == np.int8 and spins.flags.c_contiguous
     sums = np.zeros((n, 5), dtype=np.int64)
     lib().or_ising_sweeps(C.byref(L), C.byref(M), seed, sweep0, n, _ptr(spins), _ptr(sums))
+    return sums
+
+
+def ising_sweeps_sc_fast(L: Lattice, M: Model, seed: int, sweep0: int, n: int,
+                         spins: np.ndarray, threads: int = 1):
+    """or_ising_sweeps for sc lattices, fast and on `threads` OpenMP threads."""
+    assert spins.dtype == np.int8 and spins.flags.c_contiguous and L.kind == 0
+    sums = np.zeros((n, 5), dtype=np.int64)
+    lib().or_ising_sweeps_sc_fast(C.byref(L), C.byref(M), seed, sweep0, n, _ptr(spins),
+                                  _ptr(sums), threads)
     return sums
 
 

@@ -457,3 +457,19 @@ def test_wolff_clusters_do_not_depend_on_visit_order(orc):
     assert sa[-1, 3] == a.astype(np.int64).sum()
     s = a.astype(np.int64).reshape(6, 6, 6)
     assert sa[-1, 0] == sum(int((s * np.roll(s, 1, axis=ax)).sum()) for ax in range(3))
+
+
+@pytest.mark.parametrize("dims,pbc,proposal,h", [((8, 6, 4), (1, 1, 1), "flip", 0.0),
+                                                 ((6, 5, 3), (1, 0, 0), "random", 0.3),
+                                                 ((10, 10, 1), (1, 1, 0), "flip", 0.1)])
+def test_fast_sc_checkerboard_equals_the_oracle(orc, dims, pbc, proposal, h):
+    """The CPU baseline's fast sc checkerboard makes the oracle's draws: the
+    same states and sums, on 1 and on 4 threads."""
+    L = orc.lattice("sc", *dims, pbc=pbc)
+    M = orc.model(proposal=proposal, terms=("exchange", "zeeman"), temperature=3.0, field=h)
+    ref = orc.ising_init(L, 8)
+    want = orc.ising_sweeps(L, M, 8, 5, 7, ref)
+    for threads in (1, 4):
+        s = orc.ising_init(L, 8)
+        got = orc.ising_sweeps_sc_fast(L, M, 8, 5, 7, s, threads)
+        assert np.array_equal(s, ref) and np.array_equal(got, want), threads
