@@ -14,7 +14,8 @@ stream), and the CPU baseline (the oracle's restatement of the reference's
 sequential MetropolisFlipIntegrator, one core, a bounded sample).
 
 With the default --workload all, the line also carries the other forms of
-the path as sub-objects: the same chains bit-packed, the 512-chain mixed
+the path as sub-objects: the same chains bit-packed, one HBM-resident chain
+of sc 1024^3 (larger than the Infinity Cache), the 512-chain mixed
 batch against per-chain plain launches, Heisenberg sc 128^3 x 3, Ising and
 Heisenberg chains on fcc lattices (four colour passes per sweep), Wolff
 cluster steps (batched and single chain, with the oracle's Wolff on one
@@ -269,6 +270,42 @@ def run_bitpacked(vg, ctx, side, sweeps):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "kernel": "ising_sc_pass<bits>",
                      "algorithmic_bytes_per_launch": per_launch // 4,
+                     "mean_launch_us": avg_ms * 1e3},
+    }
+
+
+def run_resident(vg, ctx, side, sweeps):
+    """One int8 Ising chain of sc side^3 at T_c = 4.5115 (J = 1, h = 0, seed
+    7): at 1024^3 the two colour planes (1.07 GB) are far larger than the
+    256 MB Infinity Cache, so every pass streams from HBM.  Attempts/s and
+    the pass kernel's roofline at 2 B per attempt."""
+    L = vg.lattice(vg.SC, side, side, side)
+    n = side ** 3
+    with vg.Batch(ctx, [vg.chain(L, vg.ISING, vg.FLIP, vg.hamiltonian(exchange=1.0), 7)]) as b:
+        b.randomize()
+        b.set_thermostat(0, 4.5115)
+        tile = b.autotune(4)
+        b.set_tile(tile)
+        b.set_graph(True)
+        b.run(4)
+        t0 = time.perf_counter()
+        b.run(sweeps)
+        wall = time.perf_counter() - t0
+        dev_ms = b.last_run_ms()
+    value = n * sweeps / wall
+    per_launch = n // 2
+    avg_ms = dev_ms / (2 * sweeps)
+    achieved = 2 * per_launch / (avg_ms * 1e-3) / 1e9
+    log(f"resident sc {side}^3: {value:.4e} attempts/s, pass {avg_ms * 1e3:.1f} us, tile {tile}")
+    return {
+        "workload": f"ising-sc{side}-i8: one chain sc {side}^3 at T=4.5115, HBM-resident",
+        "value": value,
+        "unit": "spin-flip attempts/s",
+        "sweeps": sweeps,
+        "tile": tile,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "ising_sc_pass",
+                     "algorithmic_bytes_per_launch": 2 * per_launch,
                      "mean_launch_us": avg_ms * 1e3},
     }
 
@@ -566,7 +603,7 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="Ising tile; 0 = autotune")
     ap.add_argument("--workload", default="all",
                     choices=["all", "headline", "batched", "served", "heisenberg", "bitpacked",
-                             "fcc", "wolff"],
+                             "fcc", "wolff", "resident"],
                     help="headline: sc 256^3 x 3 only; batched: the 512-chain batch only; "
                          "served: the Poisson job stream only")
     ap.add_argument("--serve-jobs", type=int, default=400,
@@ -578,6 +615,9 @@ def main():
                     help="fcc Heisenberg chains: cells per side")
     ap.add_argument("--fcc-sweeps", type=int, default=300)
     ap.add_argument("--wolff-steps", type=int, default=2000)
+    ap.add_argument("--resident-side", type=int, default=1024,
+                    help="side of the HBM-resident Ising chain")
+    ap.add_argument("--resident-sweeps", type=int, default=100)
     ap.add_argument("--serve-rates", default="50,500")
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="server per-sweep work budget (Ising-site units; 0 = none)")
@@ -620,6 +660,11 @@ def main():
     if args.workload == "wolff":
         with vg.Context(0) as ctx:
             print(json.dumps(run_wolff(vg, ctx, 32, 64, 64, args.wolff_steps)), flush=True)
+        return
+    if args.workload == "resident":
+        with vg.Context(0) as ctx:
+            print(json.dumps(run_resident(vg, ctx, args.resident_side, args.resident_sweeps)),
+                  flush=True)
         return
     if args.workload == "fcc":
         with vg.Context(0) as ctx:
@@ -679,9 +724,13 @@ def main():
     alg_bytes = 2 * attempts_per_launch  # read + write of each site once per sweep
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     traffic = None
-    pmc = ROOT / "profiles" / f"ising_sc{side}_pmc.json"
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter passes
+    # (scaled by a device copy of known size), when profiled at this size.
+    pmc = ROOT / "profiles" / f"ising_sc{side}_traffic.txt"
     if pmc.exists():
-        traffic = json.loads(pmc.read_text()).get("traffic_bytes_per_launch")
+        for line in pmc.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = float(line.split(":", 1)[1])
     log(f"bench: pass kernel {avg_ms * 1e3:.1f} us/launch -> {achieved:.0f} GB/s algorithmic")
 
     cpu = None
@@ -737,6 +786,7 @@ def main():
     if args.workload == "all":
         # The other forms of the path, each its own sub-object (not `value`).
         out["bitpacked"] = run_bitpacked(vg, ctx, side, args.steps)
+        out["resident"] = run_resident(vg, ctx, args.resident_side, args.resident_sweeps)
         out["batched"] = run_batched(vg, ctx, args.batched_sweeps, args.baseline_sweeps,
                                      cpu=cpu_batched)
         out["heisenberg"] = run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)
