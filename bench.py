@@ -17,7 +17,8 @@ With the default --workload all, the line also carries the other forms of
 the path as sub-objects: the same chains bit-packed, one HBM-resident chain
 of sc 1024^3 (larger than the Infinity Cache), the 512-chain mixed
 batch against per-chain plain launches, Heisenberg sc 128^3 x 3, Ising and
-Heisenberg chains on fcc lattices (four colour passes per sweep), Wolff
+Heisenberg chains on fcc lattices (four colour passes per sweep) and on bcc
+lattices, Wolff
 cluster steps (batched and single chain, with the oracle's Wolff on one
 core), and the served Poisson job stream with p50/p99 latency (--serve-jobs
 jobs per rate).
@@ -356,22 +357,28 @@ def run_heisenberg(vg, ctx, side, sweeps):
     }
 
 
-def run_fcc(vg, ctx, cells, heis_cells, sweeps):
+def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
     """fcc lattices (four colour passes per sweep): 3 Ising chains of cells^3
     fcc cells (4 sites each; J = 1, h = 0; T = 8, 9.8, 12 around T_c ~ 9.8)
     and 3 Heisenberg chains of heis_cells^3 cells (J = 1, K = 0.1 z, B = 0.05
     z, uniform-sphere proposals; T = 1.5, 3.2, 5), random starts from seed 7.
-    Attempts/s and each pass kernel's roofline (2 B / 24 B per attempt)."""
+    Attempts/s and each pass kernel's roofline (2 B / 24 B per attempt).
+    With kind = vg.BCC the same on bcc lattices (2 sites per cell, two colour
+    passes; Ising T = 5, 6.35, 8 around T_c ~ 6.35, Heisenberg T = 1, 2.1, 3.5)."""
+    bcc = kind == vg.BCC
+    kind = vg.BCC if bcc else vg.FCC
+    per_cell, passes, name = (2, 2, "bcc") if bcc else (4, 4, "fcc")
     out = {}
-    for spin, side, temps in (("ising", cells, [8.0, 9.8, 12.0]),
-                              ("heisenberg", heis_cells, [1.5, 3.2, 5.0])):
+    for spin, side, temps in (("ising", cells, [5.0, 6.35, 8.0] if bcc else [8.0, 9.8, 12.0]),
+                              ("heisenberg", heis_cells,
+                               [1.0, 2.1, 3.5] if bcc else [1.5, 3.2, 5.0])):
         ising = spin == "ising"
         H = (vg.hamiltonian(exchange=1.0) if ising
              else vg.hamiltonian(exchange=1.0, zeeman=True, anisotropy=0.1))
-        L = vg.lattice(vg.FCC, side, side, side)
+        L = vg.lattice(kind, side, side, side)
         descs = [vg.chain(L, vg.ISING if ising else vg.HEISENBERG,
                           vg.FLIP if ising else vg.RANDOM, H, 7) for _ in temps]
-        n = 4 * side ** 3
+        n = per_cell * side ** 3
         with vg.Batch(ctx, descs) as b:
             b.randomize()
             for i, T in enumerate(temps):
@@ -383,13 +390,14 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps):
             wall = time.perf_counter() - t0
             dev_ms = b.last_run_ms()
         value = n * len(temps) * sweeps / wall
-        per_launch = n * len(temps) // 4
-        avg_ms = dev_ms / (4 * sweeps)
+        per_launch = n * len(temps) // passes
+        avg_ms = dev_ms / (passes * sweeps)
         bpa = 2 if ising else 24
         achieved = bpa * per_launch / (avg_ms * 1e-3) / 1e9
-        log(f"fcc {spin}: {value:.4e} attempts/s, pass {avg_ms * 1e3:.1f} us -> {achieved:.0f} GB/s")
+        log(f"{name} {spin}: {value:.4e} attempts/s, pass {avg_ms * 1e3:.1f} us -> "
+            f"{achieved:.0f} GB/s")
         out[spin] = {
-            "workload": (f"{spin}-fcc{side}: 3 chains of {side}^3 fcc cells ({n} sites), "
+            "workload": (f"{spin}-{name}{side}: 3 chains of {side}^3 {name} cells ({n} sites), "
                          f"T={temps}" + ("" if ising else ", K=0.1 z, B=0.05 z, random proposals")),
             "value": value,
             "unit": "spin-flip attempts/s",
@@ -397,7 +405,7 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps):
             "ms_per_step": wall * 1e3 / sweeps,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "kernel": "ising_fcc_pass" if ising else "heis_fcc_pass",
+                         "kernel": f"{'ising' if ising else 'heis'}_{name}_pass",
                          "algorithmic_bytes_per_launch": bpa * per_launch,
                          "mean_launch_us": avg_ms * 1e3},
             "observables_last_sweep": {f"T={T}": {"E/N": float(e[i, -1]) / n,
@@ -603,7 +611,7 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="Ising tile; 0 = autotune")
     ap.add_argument("--workload", default="all",
                     choices=["all", "headline", "batched", "served", "heisenberg", "bitpacked",
-                             "fcc", "wolff", "resident"],
+                             "fcc", "bcc", "wolff", "resident"],
                     help="headline: sc 256^3 x 3 only; batched: the 512-chain batch only; "
                          "served: the Poisson job stream only")
     ap.add_argument("--serve-jobs", type=int, default=400,
@@ -613,7 +621,10 @@ def main():
     ap.add_argument("--fcc-cells", type=int, default=128, help="fcc Ising chains: cells per side")
     ap.add_argument("--fcc-heis-cells", type=int, default=80,
                     help="fcc Heisenberg chains: cells per side")
-    ap.add_argument("--fcc-sweeps", type=int, default=300)
+    ap.add_argument("--fcc-sweeps", type=int, default=300, help="sweeps of the fcc and bcc lines")
+    ap.add_argument("--bcc-cells", type=int, default=160, help="bcc Ising chains: cells per side")
+    ap.add_argument("--bcc-heis-cells", type=int, default=100,
+                    help="bcc Heisenberg chains: cells per side")
     ap.add_argument("--wolff-steps", type=int, default=2000)
     ap.add_argument("--resident-side", type=int, default=1024,
                     help="side of the HBM-resident Ising chain")
@@ -670,6 +681,11 @@ def main():
         with vg.Context(0) as ctx:
             print(json.dumps(run_fcc(vg, ctx, args.fcc_cells, args.fcc_heis_cells,
                                      args.fcc_sweeps)), flush=True)
+        return
+    if args.workload == "bcc":
+        with vg.Context(0) as ctx:
+            print(json.dumps(run_fcc(vg, ctx, args.bcc_cells, args.bcc_heis_cells,
+                                     args.fcc_sweeps, kind=vg.BCC)), flush=True)
         return
     if args.workload == "bitpacked":
         with vg.Context(0) as ctx:
@@ -791,6 +807,8 @@ def main():
                                      cpu=cpu_batched)
         out["heisenberg"] = run_heisenberg(vg, ctx, args.heis_side, args.heis_sweeps)
         out["fcc"] = run_fcc(vg, ctx, args.fcc_cells, args.fcc_heis_cells, args.fcc_sweeps)
+        out["bcc"] = run_fcc(vg, ctx, args.bcc_cells, args.bcc_heis_cells, args.fcc_sweeps,
+                             kind=vg.BCC)
         out["wolff"] = run_wolff(vg, ctx, 32, 64, 64, args.wolff_steps)
         rates = [float(r) for r in args.serve_rates.split(",")]
         out["served"] = run_served(vg, ctx, args.serve_jobs, rates, budget=args.serve_budget)
