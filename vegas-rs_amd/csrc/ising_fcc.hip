@@ -13,12 +13,14 @@
 // when a and b differ along y and z, the four neighbour rows at the same x
 // summed byte-wise; when they differ along x and one of y, z, the two rows
 // summed and that sum plus itself shifted by one byte (x-1 for colours 0
-// and 1, x+1 for colours 2 and 3).  The decision is the bcc pass's: class
-// (b, S) with S = 2 * (up neighbours) + (absent slots) of the 25-entry
-// table, the 32-bit draw's high byte from one Philox call per 16 positions
-// and its low 24 bits only on a high-byte tie.  The pass of colour a adds the
-// bonds to the lower colours b < a, so over a sweep every bond counts once.
+// and 1, x+1 for colours 2 and 3).  The decision is the bcc pass's, four
+// sites per register (ising_swar.hpp): class (b, S) with S = 2 * (up
+// neighbours) + (absent slots) of the 25-entry table, the 32-bit draw's high
+// byte from one Philox call per 16 positions and its low 24 bits only on a
+// high-byte tie.  The pass of colour a adds the bonds to the lower colours
+// b < a, so over a sweep every bond counts once.
 #include "fcc_common.hpp"
+#include "ising_swar.hpp"
 
 namespace vg {
 
@@ -33,12 +35,14 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                                                       int32_t* __restrict__ partials, uint32_t s,
                                                       uint32_t total_blocks, uint32_t grid) {
     __shared__ uint32_t lut[2 * kThrS];
+    __shared__ uint32_t tab[13 * 8];  // class tables for 0..12 absent slots
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
     if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid < 2 * kThrS) lut[tid] = (&C.thr[0][0])[tid];
+    class_tables(C, 12u, tab);
     __syncthreads();
 
     const uint32_t local = (b - C.block0) * kBlock + tid;
@@ -150,48 +154,54 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
         }
 
         const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
-        uint32_t n[4] = {o[0], o[1], o[2], o[3]};
         const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, A)}, k0, k1);
         const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
-        uint32_t bits = 0;
+        const uint32_t vbits = nvalid >= kIsingSeg ? 0xFFFFu : (1u << nvalid) - 1u;
+        uint32_t valid[4], live[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) live[kk] = valid[kk] = nibble_bytes(vbits >> (4 * kk)) << 7;
         if (C.proposal) {
             const U4 rp = philox(U4{i0 >> 7, row, (uint32_t)t, ctr_w(t, kIsingProposal, A)}, k0, k1);
-            bits = (u4_word(rp, (i0 >> 5) & 3u) >> (i0 & 31u)) & 0xFFFFu;
-        }
-        const uint32_t always0 = C.always[0], always1 = C.always[1];
+            const uint32_t bits = (u4_word(rp, (i0 >> 5) & 3u) >> (i0 & 31u)) & 0xFFFFu;
+            // Spin::rand proposes Up on bit 0 (state.rs:76-84); a proposal
+            // equal to the current spin changes nothing.
 #pragma unroll
-        for (uint32_t j = 0; j < kIsingSeg; ++j) {
-            if (j >= nvalid) continue;
+            for (int kk = 0; kk < 4; ++kk)
+                live[kk] &= (~(nibble_bytes(bits >> (4 * kk)) ^ o[kk]) & kOne) << 7;
+        }
+        // absent slots of the segment's sites, and of the position gx
+        const uint32_t a_in = 12u - pres, a_edge = a_in + miss;
+        const uint32_t always0 = C.always[0], always1 = C.always[1];
+        uint32_t T0[4], T1[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            T0[k] = tab[a_in * 8u + k];
+            T1[k] = tab[a_in * 8u + 4u + k];
+        }
+        uint32_t n[4], accH[4], tieH[4];
+        decide16(cnt, o, rw, live, T0, T1, n, accH, tieH);
+        if (gx < nvalid) {
+            const uint32_t kk = gx >> 2, sh = 8u * (gx & 3u);
+            const uint32_t c = (cnt[kk] >> sh) & 0xFFu, bj = (o[kk] >> sh) & 1u;
+            const uint32_t S = 2u * c + a_edge;
+            const bool alw = ((bj ? always1 : always0) >> S) & 1u;
+            const uint32_t th = alw ? 0xFFu : lut[bj * kThrS + S] >> 24;
+            const uint32_t r8 = (rw[kk] >> sh) & 0xFFu;
+            const bool lv = (live[kk] >> sh) & 0x80u;
+            const uint32_t f = lv && r8 < th ? 1u : 0u;
+            tieH[kk] = (tieH[kk] & ~(0x80u << sh)) | ((lv && r8 == th ? 0x80u : 0u) << sh);
+            n[kk] = (n[kk] & ~(1u << sh)) | ((bj ^ f) << sh);
+        }
+        // Tied sites: the low 24 bits of the draw decide.
+        uint32_t tm = msb_mask16(tieH);
+        while (tm) {
+            const uint32_t j = __builtin_ctz(tm);
+            tm &= tm - 1u;
             const uint32_t kk = j >> 2, sh = 8u * (j & 3u);
             const uint32_t bj = (o[kk] >> sh) & 1u;
-            const uint32_t present = pres - (j == gx ? miss : 0u);
-            const uint32_t S = 2u * ((cnt[kk] >> sh) & 0xFFu) + (12u - present);
-            // Random proposal: Spin::rand proposes Up on bit 0 (state.rs:76-84);
-            // a proposal equal to the current spin changes nothing.
-            bool flip = false;
-            if (!C.proposal || ((bits >> j) & 1u) == bj) {
-                if (((bj ? always1 : always0) >> S) & 1u) {
-                    flip = true;
-                } else {
-                    const uint32_t thr = lut[bj * kThrS + S];
-                    const uint32_t r8 = (rw[kk] >> sh) & 0xFFu, hi = thr >> 24;
-                    if (r8 < hi) {
-                        flip = true;
-                    } else if (r8 == hi) {
-                        const uint32_t pos = i0 + j;
-                        const U4 rb =
-                            philox(U4{pos >> 2, row, (uint32_t)t, ctr_w(t, kIsingB, A)}, k0, k1);
-                        flip = (u4_word(rb, pos & 3u) >> 8) < (thr & 0xFFFFFFu);
-                    }
-                }
-            }
-            const uint32_t nj = bj ^ (flip ? 1u : 0u);
-            if (flip) n[kk] ^= 1u << sh;
-            // bonds to the lower colours: s_j * sum s_nb = s_j (2 up - present)
-            if (A > 0) {
-                const uint32_t pl = pres_low - (j == gx ? miss_low : 0u);
-                b_part += (2 * (int32_t)nj - 1) * (2 * (int32_t)((low[kk] >> sh) & 0xFFu) - (int32_t)pl);
-            }
+            const uint32_t S = 2u * ((cnt[kk] >> sh) & 0xFFu) + (j == gx ? a_edge : a_in);
+            if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t, A, k0, k1))
+                n[kk] ^= 1u << sh;
         }
         if ((n[0] ^ o[0]) | (n[1] ^ o[1]) | (n[2] ^ o[2]) | (n[3] ^ o[3])) {
             u32x4 nv;
@@ -200,6 +210,25 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
             nv.z = n[2];
             nv.w = n[3];
             __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+        }
+        if (A > 0) {
+            // bonds to the lower colours: sum_j s_j (2 low_j - pl_j) with
+            // s = 2n - 1 and pl the present lower slots (fewer by miss_low at gx)
+            uint32_t nl = 0, ls = 0;
+            int32_t upc = 0;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const uint32_t lv = low[kk] & ((valid[kk] >> 7) * 0xFFu);
+                nl = __builtin_amdgcn_udot4(n[kk], lv, nl, false);
+                ls = __builtin_amdgcn_udot4(lv, kOne, ls, false);
+                upc += __popc(n[kk]);
+            }
+            b_part = 4 * (int32_t)nl - 2 * (int32_t)ls -
+                     (int32_t)pres_low * (2 * upc - (int32_t)nvalid);
+            if (gx < nvalid) {
+                const int32_t nj = (int32_t)((n[gx >> 2] >> (8u * (gx & 3u))) & 1u);
+                b_part += (2 * nj - 1) * (int32_t)miss_low;
+            }
         }
         const int32_t up = __popc(n[0]) + __popc(n[1]) + __popc(n[2]) + __popc(n[3]);
         m_part = 2 * up - (int32_t)nvalid;

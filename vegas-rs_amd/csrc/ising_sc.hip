@@ -28,47 +28,17 @@
 // its planes, one lane per record resolves its ties, writes the flipped bytes
 // and corrects the wave's sums.  The result equals comparing the full 32-bit
 // draw.  No workgroup barrier is needed after the tables are loaded.
-#include "vg_device.hpp"
+#include "ising_swar.hpp"
 
 namespace vg {
 
 constexpr uint32_t kWaves = kBlock / 64;
 constexpr uint32_t kTieRows = 64;     // tie records per wave (one per row with ties)
 constexpr uint32_t kTieWords = 10;
-constexpr uint32_t kH = 0x80808080u;  // byte MSBs
-constexpr uint32_t kL = 0x7F7F7F7Fu;
-constexpr uint32_t kOne = 0x01010101u;
 
 __device__ __forceinline__ uint32_t pick4(uint32_t k, uint32_t a, uint32_t b, uint32_t c,
                                           uint32_t d) {
     return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
-}
-
-// Byte-wise unsigned r < t, as MSB-per-byte flags.
-__device__ __forceinline__ uint32_t bytes_lt(uint32_t r, uint32_t t) {
-    const uint32_t d = (r | kH) - (t & kL);
-    return ((~r & t) | (~(r ^ t) & ~d)) & kH;
-}
-
-// Byte-wise r == t, as MSB-per-byte flags.
-__device__ __forceinline__ uint32_t bytes_eq(uint32_t r, uint32_t t) {
-    const uint32_t x = r ^ t;
-    return ~(((x & kL) + kL) | x) & kH;
-}
-
-// Four bits -> one byte each (0 or 1).
-__device__ __forceinline__ uint32_t nibble_bytes(uint32_t v) {
-    return ((v & 0xFu) * 0x00204081u) & kOne;
-}
-
-// Low 24 bits of a tied draw decide: accept iff they are below the low 24
-// bits of the threshold; an always-accepted class needs no draw.
-__device__ __forceinline__ bool tie_accept(const uint32_t* lut, uint32_t always_b, uint32_t bj,
-                                           uint32_t S, uint32_t pos, uint32_t row, uint64_t t,
-                                           uint32_t colour, uint32_t k0, uint32_t k1) {
-    if ((always_b >> S) & 1u) return true;
-    const U4 rb = philox(U4{pos >> 2, row, (uint32_t)t, ctr_w(t, kIsingB, colour)}, k0, k1);
-    return (u4_word(rb, pos & 3u) >> 8) < (lut[bj * kThrS + S] & 0xFFFFFFu);
 }
 
 // The two storage forms of a colour plane.  Int8: one byte per position,
