@@ -55,6 +55,43 @@ def test_bcc_ising_bit_exact(vg, orc, ctx, dims, pbc, T, h, hz, proposal, terms)
         assert np.array_equal(b.get_ising(0), spins)
 
 
+@pytest.mark.parametrize("dims,pbc", [((16, 16, 16), (1, 1, 1)), ((21, 6, 13), (1, 1, 0)),
+                                      ((9, 5, 3), (0, 0, 1)), ((18, 7, 1), (1, 0, 0))])
+def test_bcc_tiles_give_identical_trajectories(vg, orc, ctx, dims, pbc):
+    """The bcc pass marches K = 1, 2, 4 or 8 z planes per work item (the
+    upper row pair of a plane is the lower pair of the next): every tile,
+    including partial last chunks and open z ends, reproduces the oracle."""
+    d, OL = _pair(vg, orc, dims, pbc, "ising", "flip", ("exchange", "zeeman"), seed=6)
+    OM = orc.model(spin="ising", proposal="flip", terms=("exchange", "zeeman"),
+                   temperature=5.0, field=0.1, field_dir=(0.0, 0.0, 1.0))
+    spins = orc.ising_init(OL, 6)
+    want = orc.ising_sweeps(OL, OM, 6, 0, 5, spins)
+    for tile in (1, 2, 4, 8):
+        with vg.Batch(ctx, [d]) as b:
+            b.set_tile(tile)
+            assert b.tile() == tile
+            b.randomize()
+            b.set_thermostat(0, 5.0, 0.1)
+            assert np.array_equal(b.run_sums(5)[0], want), tile
+            assert np.array_equal(b.get_ising(0), spins), tile
+
+
+def test_bcc_tie_queue_with_lanes_that_stop_early(vg, orc, ctx):
+    """bcc 32x6x13 cells, open z, tile 8: 12 lanes of each pass's wave march
+    8 planes and 12 the last 5; tie records past index 12 belong to lanes
+    that stopped early, and all are settled (30 sweeps equal the oracle's)."""
+    d, OL = _pair(vg, orc, (32, 6, 13), (1, 1, 0), "ising", "flip", ("exchange",), seed=13)
+    OM = orc.model(spin="ising", proposal="flip", terms=("exchange",), temperature=6.35)
+    spins = orc.ising_init(OL, 13)
+    want = orc.ising_sweeps(OL, OM, 13, 0, 30, spins)
+    with vg.Batch(ctx, [d]) as b:
+        b.set_tile(8)
+        b.randomize()
+        b.set_thermostat(0, 6.35)
+        assert np.array_equal(b.run_sums(30)[0], want)
+        assert np.array_equal(b.get_ising(0), spins)
+
+
 HEIS = [
     # dims, pbc, T, h, proposal, terms, K
     ((8, 8, 8), (1, 1, 1), 2.0, 0.05, "random", ("exchange", "zeeman", "anisotropy"), 0.1),

@@ -200,6 +200,23 @@ def test_tiles_give_identical_trajectories(vg, orc, ctx, dims, pbc):
             assert np.array_equal(b.get_ising(0), spins), tile
 
 
+def test_tie_queue_with_lanes_that_stop_early(vg, orc, ctx):
+    """sc 64x6x13, open z, tile 8: the one wave of each pass holds 12 lanes
+    marching 8 planes and 12 marching the last 5, and queues about ten tie
+    records per pass -- records past index 12 belong to lanes that stopped
+    early.  Every record is settled: 30 sweeps equal the oracle's."""
+    d, OL, OM = make_pair(vg, orc, (64, 6, 13), (1, 1, 0), 4.5115, 0.0, (0, 0, 1), "flip",
+                          ("exchange",), seed=13)
+    spins = orc.ising_init(OL, 13)
+    want = orc.ising_sweeps(OL, OM, 13, 0, 30, spins)
+    with vg.Batch(ctx, [d]) as b:
+        b.set_tile(8)
+        b.randomize()
+        b.set_thermostat(0, 4.5115)
+        assert np.array_equal(b.run_sums(30)[0], want)
+        assert np.array_equal(b.get_ising(0), spins)
+
+
 def test_autotune_leaves_trajectory_unchanged(vg, orc, ctx):
     descs = [make_pair(vg, orc, (32, 32, 32), seed=s)[0] for s in (1, 2)]
     runs = []

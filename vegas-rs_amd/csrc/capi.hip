@@ -331,6 +331,30 @@ void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     }
 }
 
+void launch_bcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
+    const dim3 grid(G.grid), block(vg::kBlock);
+    uint8_t* arena = (uint8_t*)G.arena;
+    int32_t* part = (int32_t*)G.partials;
+    switch (G.tile) {
+        case 1:
+            hipLaunchKernelGGL(vg::ising_bcc_pass<1>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+        case 2:
+            hipLaunchKernelGGL(vg::ising_bcc_pass<2>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+        case 8:
+            hipLaunchKernelGGL(vg::ising_bcc_pass<8>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+        default:
+            hipLaunchKernelGGL(vg::ising_bcc_pass<4>, grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+            break;
+    }
+}
+
 template <uint32_t A>
 void launch_fcc_colour(const Group& G, hipStream_t st, uint32_t s, bool ising) {
     const dim3 grid(G.grid), block(vg::kBlock);
@@ -375,9 +399,7 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                 else if (g == kIsingBits)
                     launch_ising_pass<true>(G, st, c, s);
                 else if (g == kIsingBcc)
-                    hipLaunchKernelGGL(vg::ising_bcc_pass, grid, block, 0, st, G.d_chains,
-                                       G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials,
-                                       c, s, G.total_blocks, G.grid);
+                    launch_bcc_pass(G, st, c, s);
                 else if (g == kIsingFcc)
                     launch_fcc_pass(G, st, c, s, true);
                 else if (g == kHeisSc)
@@ -1044,7 +1066,7 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
     if (tile != 1 && tile != 2 && tile != 4 && tile != 8)
         return fail(VG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8 (got %u)", tile);
     VG_HIP(hipSetDevice(batch->ctx->device));
-    for (int g : {kIsingSc, kIsingBits}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc}) {
         if (batch->groups[g].members.empty()) continue;
         int rc = layout_work(batch, g, tile);
         if (rc) return rc;
@@ -1052,10 +1074,17 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
     return VG_OK;
 }
 
+// The group whose tile vg_batch_get_tile and vg_batch_autotune report: the
+// first of sc int8, sc bit-packed and bcc that has chains.
+static int tile_group(const vg_batch* batch) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc})
+        if (!batch->groups[g].members.empty()) return g;
+    return kIsingSc;
+}
+
 int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile) {
     if (!batch || !tile) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
-    const int g = batch->groups[kIsingSc].members.empty() ? kIsingBits : kIsingSc;
-    *tile = batch->groups[g].tile;
+    *tile = batch->groups[tile_group(batch)].tile;
     return VG_OK;
 }
 
@@ -1064,7 +1093,8 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     uint32_t tile_now = 0;
     vg_batch_get_tile(batch, &tile_now);
     const bool any = !batch->groups[kIsingSc].members.empty() ||
-                     !batch->groups[kIsingBits].members.empty();
+                     !batch->groups[kIsingBits].members.empty() ||
+                     !batch->groups[kIsingBcc].members.empty();
     if (!any || trial_sweeps == 0) {
         if (chosen) *chosen = tile_now;
         return VG_OK;
@@ -1104,7 +1134,7 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     int rc = VG_OK;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(VG_ERR_DEVICE, "autotune backup: %s", hipGetErrorString(e));
-    for (int g : {kIsingSc, kIsingBits}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc}) {
         Group& G = batch->groups[g];
         if (G.members.empty() || rc != VG_OK) continue;
         const uint32_t start_tile = G.tile;
@@ -1129,10 +1159,7 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
         rc = layout_work(batch, g, best);
     }
     free_backup();
-    if (rc == VG_OK && chosen) {
-        const int g = batch->groups[kIsingSc].members.empty() ? kIsingBits : kIsingSc;
-        *chosen = batch->groups[g].tile;
-    }
+    if (rc == VG_OK && chosen) *chosen = batch->groups[tile_group(batch)].tile;
     return rc;
 }
 

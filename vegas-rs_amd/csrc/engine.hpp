@@ -31,6 +31,7 @@ __global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t
 __global__ void ising_gather(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
 __global__ void heis_sc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                              uint32_t, uint32_t, uint32_t);
+template <uint32_t K>
 __global__ void ising_bcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
                                uint32_t, uint32_t, uint32_t);
 __global__ void heis_bcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
@@ -95,7 +96,7 @@ inline bool is_ising(int g) {
            g == kIsingWolff;
 }
 // Groups whose pass marches a tile of z planes per work item.
-inline bool is_zmarch(int g) { return g == kIsingSc || g == kIsingBits; }
+inline bool is_zmarch(int g) { return g == kIsingSc || g == kIsingBits || g == kIsingBcc; }
 // Colours (planes, passes per sweep) of a group's chains; Wolff chains
 // have no colour passes.
 inline uint32_t colours_of(int g) {
@@ -125,7 +126,7 @@ struct Group {
     ChainDev* h_staging2 = nullptr;  // second one, for the block after (batches)
     uint32_t* d_block_chain = nullptr;
     uint32_t block_chain_cap = 0;
-    uint32_t tile = 4;  // Ising: z planes per work item (template parameter of the pass)
+    uint32_t tile = 4;  // Ising sc/bcc: z planes per work item (template parameter of the pass)
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
     void* arena = nullptr;

@@ -10,21 +10,31 @@
 // dx,dy,dz in {0,1}: four rows of the other plane, each at positions x and
 // x-1 (colour 0) or x and x+1 (colour 1).
 //
-// Work item: one 16-position segment of a row.  The neighbour counts of the
-// 16 sites are formed four sites per register (the four rows summed
-// byte-wise, then that sum plus itself shifted by one byte), and so is the
-// decision (ising_swar.hpp): class (b, S) with S = 2 * (up neighbours) +
-// (absent slots), accepted when always, or when the 32-bit draw (high byte
-// from one Philox call per 16 positions, low 24 bits drawn only on a
-// high-byte tie) is below the class threshold.
+// Work item: one 16-position segment of a row, marched through K (the tile)
+// consecutive z planes, in the sc pass's work order.  The four neighbour rows
+// of plane z are two pairs, (y, zl), (y', zl) and (y, zu), (y', zu), with
+// (zl, zu) = (z-1, z) for colour 0 and (z, z+1) for colour 1; marching up,
+// the upper pair of plane z is the lower pair of plane z+1, so a plane loads
+// its own row and one pair (three 16-byte loads and two bytes) instead of
+// five rows and four bytes.  The neighbour counts of the 16 sites are formed
+// four sites per register (the pair sums added byte-wise, then that sum plus
+// itself shifted by one byte), and so is the decision (ising_swar.hpp): class
+// (b, S) with S = 2 * (up neighbours) + (absent slots), accepted when always,
+// or when the 32-bit draw (high byte from one Philox call per 16 positions,
+// low 24 bits drawn only on a high-byte tie) is below the class threshold.
+// As in the sc pass, a row with ties leaves a record in the wave's LDS queue
+// and one lane per record settles them once the wave has finished its planes.
 #include "ising_swar.hpp"
 
 namespace vg {
 
 namespace {
 constexpr uint32_t kWaves = kBlock / 64;
+constexpr uint32_t kTieRows = 64;  // tie records per wave (one per row with ties)
+constexpr uint32_t kTieWords = 10;
 }
 
+template <uint32_t K>
 __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict__ chains,
                                                       const uint32_t* __restrict__ block_chain,
                                                       uint8_t* __restrict__ arena,
@@ -33,6 +43,8 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                                                       uint32_t total_blocks, uint32_t grid) {
     __shared__ uint32_t lut[2 * kThrS];
     __shared__ uint32_t tab[9 * 8];  // class tables for 0..8 absent slots
+    // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4]}
+    __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
@@ -43,79 +55,119 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
     __syncthreads();
 
     const uint32_t local = (b - C.block0) * kBlock + tid;
+    uint32_t (*const q_wave)[kTieWords] = tie_q[wave];
+    uint32_t nrec = 0;  // records queued by this wave, as counted by this lane
+    const uint64_t t = C.sweep0 + s;
+    const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
+    const uint32_t always0 = C.always[0], always1 = C.always[1];
+    uint8_t* const own_plane = arena + C.off[colour];
+    const __amdgpu_buffer_rsrc_t rw_own = plane_rsrc(own_plane, (uint32_t)C.plane);
     int32_t m_part = 0, b_part = 0;
+    uint32_t seg = 0, y = 0, zc = 0;
     if (local < C.items) {
-        const uint64_t t = C.sweep0 + s;
-        const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
+        const uint32_t r = fast_div(local, C.segs_m, C.segs_s);
+        seg = local - r * C.segs;
+        const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
+        y = 2u * (r - r2 * C.hy) + (r2 & 1u);
+        zc = r2 >> 1;
+    }
+    if (local < C.items && y < C.ly) {
         const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
-        const uint32_t row = fast_div(local, C.segs_m, C.segs_s);
-        const uint32_t seg = local - row * C.segs;
-        const uint32_t z = fast_div(row, C.ly_m, C.ly_s);
-        const uint32_t y = row - z * ly;
         const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
-        uint8_t* const own_plane = arena + C.off[colour];
-        const __amdgpu_buffer_rsrc_t rw_own = plane_rsrc(own_plane, (uint32_t)C.plane);
         const __amdgpu_buffer_rsrc_t ro = plane_rsrc(arena + C.off[colour ^ 1u], (uint32_t)C.plane);
+        const uint32_t zstep = ly * pitch;
 
-        // The neighbour rows: (y, z), (y', z), (y, z'), (y', z') with
-        // y' = y - 1, z' = z - 1 (colour 0) or y + 1, z + 1 (colour 1).
-        bool yok, zok;
-        uint32_t y2, z2;
+        // y' = y - 1 (colour 0) or y + 1 (colour 1), wrapping when periodic
+        bool yok;
+        uint32_t y2;
         if (colour == 0) {
             yok = y > 0 || py;
             y2 = y > 0 ? y - 1 : ly - 1;
-            zok = z > 0 || pz;
-            z2 = z > 0 ? z - 1 : lz - 1;
         } else {
             yok = y + 1 < ly || py;
             y2 = y + 1 < ly ? y + 1 : 0;
-            zok = z + 1 < lz || pz;
-            z2 = z + 1 < lz ? z + 1 : 0;
         }
-        const uint32_t rows[4] = {row, y2 + ly * z, y + ly * z2, y2 + ly * z2};
-        const bool rok[4] = {true, yok, zok, yok && zok};
-        const uint32_t nr = 1u + (yok ? 1u : 0u) + (zok ? 1u : 0u) + (yok && zok ? 1u : 0u);
-
         const uint32_t i0 = seg * kIsingSeg;
         const uint32_t nvalid = min(kIsingSeg, w - i0);
-        const uint32_t off = row * pitch + i0;
-        const u32x4 o4 = bload16(rw_own, off);
-        uint32_t P[4] = {0u, 0u, 0u, 0u};  // per byte: up sites at position x over the 4 rows
+        const uint32_t vbits = nvalid >= kIsingSeg ? 0xFFFFu : (1u << nvalid) - 1u;
+        uint32_t valid[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const u32x4 v = bload16(ro, rok[r] ? rows[r] * pitch + i0 : kNoRow);
-            P[0] += v.x;
-            P[1] += v.y;
-            P[2] += v.z;
-            P[3] += v.w;
-        }
-        // x-neighbour counts: position x-1 (colour 0) or x+1 (colour 1) of
-        // the same rows.  gx: the one position whose x neighbour is absent.
-        uint32_t xs[4];
-        uint32_t gx = 0xFFFFFFFFu;
+        for (int kk = 0; kk < 4; ++kk) valid[kk] = nibble_bytes(vbits >> (4 * kk)) << 7;
+        // The x-offset position outside the segment: x-1 (colour 0) or x+1
+        // (colour 1; position 0 across the periodic end).  gx: the one
+        // position whose x-offset neighbours are absent (open x).
+        bool x_ok;
+        uint32_t xpos, gx = 0xFFFFFFFFu;
+        const uint32_t pw = w - i0;  // positions left in the row, 1..16 in the last segment
+        const bool xr_in = i0 + kIsingSeg < w;
         if (colour == 0) {
-            const bool xl_ok = i0 > 0 || px;
-            const uint32_t xl = i0 > 0 ? i0 - 1 : w - 1;
-            uint32_t e = 0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) e += bload8(ro, rok[r] && xl_ok ? rows[r] * pitch + xl : kNoRow);
-            if (!xl_ok) gx = 0;
-            xs[0] = __builtin_amdgcn_alignbyte(P[0], e << 24, 3);
-            xs[1] = __builtin_amdgcn_alignbyte(P[1], P[0], 3);
-            xs[2] = __builtin_amdgcn_alignbyte(P[2], P[1], 3);
-            xs[3] = __builtin_amdgcn_alignbyte(P[3], P[2], 3);
+            x_ok = i0 > 0 || px;
+            xpos = i0 > 0 ? i0 - 1 : w - 1;
+            if (!x_ok) gx = 0;
         } else {
-            const bool xr_in = i0 + kIsingSeg < w;
-            const uint32_t xr = xr_in ? i0 + kIsingSeg : 0;  // position 0 across the end
-            const bool xr_ok = xr_in || px;
-            uint32_t e = 0;
+            x_ok = xr_in || px;
+            xpos = xr_in ? i0 + kIsingSeg : 0;
+            if (!xr_in && !px) gx = pw - 1;
+        }
+        // The pair of rows (y, zz), (y', zz): byte sums at the segment (q) and
+        // at xpos (e); absent rows read zeros.
+        auto load_pair = [&](uint32_t zz, bool zok, u32x4& q, uint32_t& e) {
+            const uint32_t r0 = (y + ly * zz) * pitch, r1 = (y2 + ly * zz) * pitch;
+            const u32x4 a = bload16(ro, zok ? r0 + i0 : kNoRow);
+            const u32x4 c = bload16(ro, zok && yok ? r1 + i0 : kNoRow);
+            q = a + c;
+            e = bload8(ro, zok && x_ok ? r0 + xpos : kNoRow) +
+                bload8(ro, zok && x_ok && yok ? r1 + xpos : kNoRow);
+        };
+
+        const uint32_t z0 = zc * K;
+        const uint32_t nk = min(K, lz - z0);
+        // The lower pair of the first plane: zl = z0-1 (colour 0) or z0.
+        u32x4 q_lo;
+        uint32_t e_lo;
+        bool lo_ok;
+        if (colour == 0) {
+            lo_ok = z0 > 0 || pz;
+            load_pair(z0 > 0 ? z0 - 1 : lz - 1, lo_ok, q_lo, e_lo);
+        } else {
+            lo_ok = true;
+            load_pair(z0, true, q_lo, e_lo);
+        }
+        uint32_t off = (y + ly * z0) * pitch + i0;  // own row (y, z), this segment
 #pragma unroll
-            for (int r = 0; r < 4; ++r) e += bload8(ro, rok[r] && xr_ok ? rows[r] * pitch + xr : kNoRow);
-            uint32_t m[4] = {P[0], P[1], P[2], P[3]};
-            if (!xr_in) {
-                const uint32_t pw = w - i0;  // 1..16 positions in this last segment
-                if (!px) gx = pw - 1;
-                if (pw != kIsingSeg) {
+        for (uint32_t k = 0; k < K; ++k) {
+            if (k >= nk) break;
+            const uint32_t z = z0 + k;
+            const uint32_t row = y + ly * z;
+            // the upper pair: zu = z (colour 0) or z+1 (colour 1)
+            bool hi_ok;
+            uint32_t zu;
+            if (colour == 0) {
+                hi_ok = true;
+                zu = z;
+            } else {
+                hi_ok = z + 1 < lz || pz;
+                zu = z + 1 < lz ? z + 1 : 0;
+            }
+            u32x4 q_hi;
+            uint32_t e_hi;
+            load_pair(zu, hi_ok, q_hi, e_hi);
+            const u32x4 o4 = bload16(rw_own, off);
+            // present rows of the four; absent slots 8 - 2 nr (8 - nr at gx)
+            const uint32_t nr = (lo_ok ? 1u : 0u) + (lo_ok && yok ? 1u : 0u) + (hi_ok ? 1u : 0u) +
+                                (hi_ok && yok ? 1u : 0u);
+            const u32x4 P4 = q_lo + q_hi;
+            uint32_t e = e_lo + e_hi;
+            const uint32_t P[4] = {P4.x, P4.y, P4.z, P4.w};
+            uint32_t xs[4];
+            if (colour == 0) {
+                xs[0] = __builtin_amdgcn_alignbyte(P[0], e << 24, 3);
+                xs[1] = __builtin_amdgcn_alignbyte(P[1], P[0], 3);
+                xs[2] = __builtin_amdgcn_alignbyte(P[2], P[1], 3);
+                xs[3] = __builtin_amdgcn_alignbyte(P[3], P[2], 3);
+            } else {
+                uint32_t m[4] = {P[0], P[1], P[2], P[3]};
+                if (!xr_in && pw != kIsingSeg) {
                     // byte pw (padding, zero) takes the wrapped neighbour of
                     // position w-1
 #pragma unroll
@@ -123,79 +175,59 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                         if ((pw >> 2) == kk) m[kk] |= e << (8u * (pw & 3u));
                     e = 0;
                 }
+                xs[0] = __builtin_amdgcn_alignbyte(m[1], m[0], 1);
+                xs[1] = __builtin_amdgcn_alignbyte(m[2], m[1], 1);
+                xs[2] = __builtin_amdgcn_alignbyte(m[3], m[2], 1);
+                xs[3] = __builtin_amdgcn_alignbyte(e, m[3], 1);
             }
-            xs[0] = __builtin_amdgcn_alignbyte(m[1], m[0], 1);
-            xs[1] = __builtin_amdgcn_alignbyte(m[2], m[1], 1);
-            xs[2] = __builtin_amdgcn_alignbyte(m[3], m[2], 1);
-            xs[3] = __builtin_amdgcn_alignbyte(e, m[3], 1);
-        }
-        uint32_t cnt[4];
+            uint32_t cnt[4];
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) cnt[kk] = P[kk] + xs[kk];  // 0..8 per byte
+            for (int kk = 0; kk < 4; ++kk) cnt[kk] = P[kk] + xs[kk];  // 0..8 per byte
 
-        const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
-        const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, colour)}, k0, k1);
-        const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
-        const uint32_t vbits = nvalid >= kIsingSeg ? 0xFFFFu : (1u << nvalid) - 1u;
-        uint32_t valid[4], live[4];
+            const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
+            const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, colour)}, k0, k1);
+            const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
+            uint32_t live[4] = {valid[0], valid[1], valid[2], valid[3]};
+            if (C.proposal) {
+                const U4 rp = philox(
+                    U4{i0 >> 7, row, (uint32_t)t, ctr_w(t, kIsingProposal, colour)}, k0, k1);
+                const uint32_t bits = (u4_word(rp, (i0 >> 5) & 3u) >> (i0 & 31u)) & 0xFFFFu;
+                // Spin::rand proposes Up on bit 0 (state.rs:76-84); a proposal
+                // equal to the current spin changes nothing.
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) live[kk] = valid[kk] = nibble_bytes(vbits >> (4 * kk)) << 7;
-        if (C.proposal) {
-            const U4 rp =
-                philox(U4{i0 >> 7, row, (uint32_t)t, ctr_w(t, kIsingProposal, colour)}, k0, k1);
-            const uint32_t bits = (u4_word(rp, (i0 >> 5) & 3u) >> (i0 & 31u)) & 0xFFFFu;
-            // Spin::rand proposes Up on bit 0 (state.rs:76-84); a proposal
-            // equal to the current spin changes nothing.
+                for (int kk = 0; kk < 4; ++kk)
+                    live[kk] &= (~(nibble_bytes(bits >> (4 * kk)) ^ o[kk]) & kOne) << 7;
+            }
+            const uint32_t a_in = 8u - 2u * nr, a_edge = 8u - nr;
+            uint32_t T0[4], T1[4];
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-                live[kk] &= (~(nibble_bytes(bits >> (4 * kk)) ^ o[kk]) & kOne) << 7;
-        }
-        // absent slots: 8 - (present rows) * (present x offsets)
-        const uint32_t a_in = 8u - 2u * nr, a_edge = 8u - nr;
-        const uint32_t always0 = C.always[0], always1 = C.always[1];
-        uint32_t T0[4], T1[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            T0[k] = tab[a_in * 8u + k];
-            T1[k] = tab[a_in * 8u + 4u + k];
-        }
-        uint32_t n[4], accH[4], tieH[4];
-        decide16(cnt, o, rw, live, T0, T1, n, accH, tieH);
-        // The position with an absent x neighbour (open x) has S = 2c + a_edge.
-        if (gx < nvalid) {
-            const uint32_t kk = gx >> 2, sh = 8u * (gx & 3u);
-            const uint32_t c = (cnt[kk] >> sh) & 0xFFu, bj = (o[kk] >> sh) & 1u;
-            const uint32_t S = 2u * c + a_edge;
-            const bool alw = ((bj ? always1 : always0) >> S) & 1u;
-            const uint32_t th = alw ? 0xFFu : lut[bj * kThrS + S] >> 24;
-            const uint32_t r8 = (rw[kk] >> sh) & 0xFFu;
-            const bool lv = (live[kk] >> sh) & 0x80u;
-            const uint32_t f = lv && r8 < th ? 1u : 0u;
-            tieH[kk] = (tieH[kk] & ~(0x80u << sh)) | ((lv && r8 == th ? 0x80u : 0u) << sh);
-            n[kk] = (n[kk] & ~(1u << sh)) | ((bj ^ f) << sh);
-        }
-        // Tied sites: the low 24 bits of the draw decide.
-        uint32_t tm = msb_mask16(tieH);
-        while (tm) {
-            const uint32_t j = __builtin_ctz(tm);
-            tm &= tm - 1u;
-            const uint32_t kk = j >> 2, sh = 8u * (j & 3u);
-            const uint32_t bj = (o[kk] >> sh) & 1u;
-            const uint32_t S = 2u * ((cnt[kk] >> sh) & 0xFFu) + (j == gx ? a_edge : a_in);
-            if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t, colour, k0, k1))
-                n[kk] ^= 1u << sh;
-        }
-        if ((n[0] ^ o[0]) | (n[1] ^ o[1]) | (n[2] ^ o[2]) | (n[3] ^ o[3])) {
-            u32x4 nv;
-            nv.x = n[0];
-            nv.y = n[1];
-            nv.z = n[2];
-            nv.w = n[3];
-            __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
-        }
-        if (colour == 1) {
-            // bond sum over colour-1 sites: sum_j s_j (S_j - 8) with
-            // s = 2n - 1 and S = 2c + a (a_edge at gx)
+            for (int kk = 0; kk < 4; ++kk) {
+                T0[kk] = tab[a_in * 8u + kk];
+                T1[kk] = tab[a_in * 8u + 4u + kk];
+            }
+            uint32_t n[4], accH[4], tieH[4];
+            decide16(cnt, o, rw, live, T0, T1, n, accH, tieH);
+            // The position with absent x-offset neighbours has S = 2c + a_edge.
+            if (gx < nvalid) {
+                const uint32_t kk = gx >> 2, sh = 8u * (gx & 3u);
+                const uint32_t c = (cnt[kk] >> sh) & 0xFFu, bj = (o[kk] >> sh) & 1u;
+                const uint32_t S = 2u * c + a_edge;
+                const bool alw = ((bj ? always1 : always0) >> S) & 1u;
+                const uint32_t th = alw ? 0xFFu : lut[bj * kThrS + S] >> 24;
+                const uint32_t r8 = (rw[kk] >> sh) & 0xFFu;
+                const bool lv = (live[kk] >> sh) & 0x80u;
+                const uint32_t f = lv && r8 < th ? 1u : 0u;
+                tieH[kk] = (tieH[kk] & ~(0x80u << sh)) | ((lv && r8 == th ? 0x80u : 0u) << sh);
+                n[kk] = (n[kk] & ~(1u << sh)) | ((bj ^ f) << sh);
+            }
+            if ((n[0] ^ o[0]) | (n[1] ^ o[1]) | (n[2] ^ o[2]) | (n[3] ^ o[3])) {
+                u32x4 nv;
+                nv.x = n[0];
+                nv.y = n[1];
+                nv.z = n[2];
+                nv.w = n[3];
+                __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+            }
             uint32_t nc = 0, cs = 0;
             int32_t upc = 0;
 #pragma unroll
@@ -205,15 +237,101 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                 cs = __builtin_amdgcn_udot4(cv, kOne, cs, false);
                 upc += __popc(n[kk]);
             }
-            b_part = 4 * (int32_t)nc - 2 * (int32_t)cs +
-                     ((int32_t)a_in - 8) * (2 * upc - (int32_t)nvalid);
-            if (gx < nvalid) {
-                const int32_t nj = (int32_t)((n[gx >> 2] >> (8u * (gx & 3u))) & 1u);
-                b_part += (2 * nj - 1) * ((int32_t)a_edge - (int32_t)a_in);
+            m_part += 2 * upc - (int32_t)nvalid;
+            if (colour == 1) {
+                // bond sum over colour-1 sites: sum_j s_j (S_j - 8) with
+                // s = 2n - 1 and S = 2c + a (a_edge at gx)
+                b_part += 4 * (int32_t)nc - 2 * (int32_t)cs +
+                          ((int32_t)a_in - 8) * (2 * upc - (int32_t)nvalid);
+                if (gx < nvalid) {
+                    const int32_t nj = (int32_t)((n[gx >> 2] >> (8u * (gx & 3u))) & 1u);
+                    b_part += (2 * nj - 1) * ((int32_t)a_edge - (int32_t)a_in);
+                }
+            }
+            // Rows with ties leave one record each in the wave's queue (their
+            // tied bytes were written unchanged above); slots by ballot.
+            const uint32_t tmask = msb_mask16(tieH);
+            const uint64_t has = __ballot(tmask != 0);
+            if (has) {
+                if (tmask) {
+                    const uint32_t slot =
+                        nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
+                    if (slot < kTieRows) {
+                        uint32_t* rec = q_wave[slot];
+                        rec[0] = lane | (k << 6) | (a_in << 9) | (a_edge << 13) |
+                                 (min(gx, 31u) << 17);
+                        rec[1] = tmask;
+                        *reinterpret_cast<uint4*>(rec + 2) = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
+                        *reinterpret_cast<uint4*>(rec + 6) = make_uint4(n[0], n[1], n[2], n[3]);
+                    } else {
+                        // Queue full (practically never): resolve in this lane.
+                        uint32_t tm = tmask;
+                        while (tm) {
+                            const uint32_t j = __builtin_ctz(tm);
+                            tm &= tm - 1u;
+                            const uint32_t kk = j >> 2, sh = 8u * (j & 3u);
+                            const uint32_t bj = (o[kk] >> sh) & 1u;
+                            const uint32_t S =
+                                2u * ((cnt[kk] >> sh) & 0xFFu) + (j == gx ? a_edge : a_in);
+                            if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t,
+                                           colour, k0, k1)) {
+                                own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                const int32_t ds = 2 - 4 * (int32_t)bj;
+                                m_part += ds;
+                                if (colour == 1) b_part += ds * ((int32_t)S - 8);
+                            }
+                        }
+                    }
+                }
+                nrec += __builtin_popcountll(has);
+            }
+            q_lo = q_hi;
+            e_lo = e_hi;
+            lo_ok = hi_ok;
+            off += zstep;
+        }
+    }
+
+    // Resolve this wave's queued ties: one lane per record draws the low 24
+    // bits of each tied site, writes the flipped byte and corrects the sums.
+    // The wave's own row stores must land first.  A lane stops counting
+    // records when it stops marching (last z chunk) or never starts, so the
+    // wave's maximum is the count.
+    const uint32_t nq = min(wave_max(nrec), kTieRows);
+    if (nq) {
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t ly = C.ly, pitch = C.pitch;
+        for (uint32_t e = lane; e < nq; e += 64) {
+            const uint32_t* rec = q_wave[e];
+            const uint32_t meta = rec[0];
+            const uint32_t owner = (wave << 6) | (meta & 63u), k = (meta >> 6) & 7u;
+            const uint32_t a_in = (meta >> 9) & 15u, a_edge = (meta >> 13) & 15u;
+            const uint32_t gx = (meta >> 17) & 31u;
+            const uint32_t l = (b - C.block0) * kBlock + owner;
+            const uint32_t r = fast_div(l, C.segs_m, C.segs_s);
+            const uint32_t sg = l - r * C.segs;
+            const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
+            const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
+            const uint32_t zz = (r2 >> 1) * K + k;
+            const uint32_t row = yy + ly * zz;
+            uint32_t tm = rec[1];
+            while (tm) {
+                const uint32_t j = __builtin_ctz(tm);
+                tm &= tm - 1u;
+                const uint32_t sh = 8u * (j & 3u);
+                const uint32_t c = (rec[2 + (j >> 2)] >> sh) & 0xFFu;
+                const uint32_t bj = (rec[6 + (j >> 2)] >> sh) & 1u;
+                const uint32_t S = 2u * c + (j == gx ? a_edge : a_in);
+                const uint32_t pos = sg * kIsingSeg + j;
+                if (tie_accept(lut, bj ? always1 : always0, bj, S, pos, row, t, colour, k0, k1)) {
+                    own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    const int32_t ds = 2 - 4 * (int32_t)bj;
+                    m_part += ds;
+                    if (colour == 1) b_part += ds * ((int32_t)S - 8);
+                }
             }
         }
-        const int32_t up = __popc(n[0]) + __popc(n[1]) + __popc(n[2]) + __popc(n[3]);
-        m_part = 2 * up - (int32_t)nvalid;
     }
 
     m_part = wave_sum(m_part);
@@ -225,5 +343,14 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
         p[1] = b_part;
     }
 }
+
+template __global__ void ising_bcc_pass<1>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                           uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_bcc_pass<2>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                           uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_bcc_pass<4>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                           uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_bcc_pass<8>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
+                                           uint32_t, uint32_t, uint32_t, uint32_t);
 
 }  // namespace vg

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     const uint32_t zstep = ly * pitch;  // bytes between planes z and z+1
     const uint32_t local = (b - C.block0) * kBlock + tid;
     uint32_t (*const q_wave)[kTieWords] = tie_q[wave];
-    uint32_t nrec = 0;  // records queued by this wave (wave-uniform)
+    uint32_t nrec = 0;  // records queued by this wave, as counted by this lane
 
     int32_t m_part = 0, b_part = 0;
     uint32_t seg = 0, y = 0, zc = 0;
@@ -351,9 +351,12 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
 
     // Resolve this wave's queued ties: one lane per record draws the low 24
     // bits of each tied site, writes the flipped byte and corrects the sums.
-    // The wave's own row stores must land first.  nrec is wave-uniform (it
-    // only grows by ballot counts), so every lane agrees on the loop bound.
-    const uint32_t nq = min(nrec, kTieRows);
+    // The wave's own row stores must land first.  nrec grows by ballot counts
+    // among the lanes still marching, and a lane that stops (last z chunk)
+    // or never started (past the chain's items, or the row y = ly of an odd
+    // ly) stops counting: the lanes marching longest saw every record, so
+    // the wave's maximum is the count, the same for every lane.
+    const uint32_t nq = min(wave_max(nrec), kTieRows);
     if (nq) {
         __builtin_amdgcn_s_waitcnt(0);
         for (uint32_t e = lane; e < nq; e += 64) {

@@ -229,6 +229,13 @@ __device__ __forceinline__ int32_t wave_sum(int32_t v) {
     return v;
 }
 
+// Maximum over the 64 lanes of a wave (every lane must call it).
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
 // Ising colour planes are read and written through raw buffer resources:
 // 32-bit offsets (a plane is below 4 GiB), and an offset past the plane
 // (kNoRow) reads as zeros, which is exactly an absent neighbour row on an
