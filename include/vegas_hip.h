@@ -200,14 +200,21 @@ int vg_batch_run(vg_batch* batch, uint64_t sweeps, double* energy, double* magne
 int vg_batch_run_sums(vg_batch* batch, uint64_t sweeps, vg_sweep_sums* sums);
 /* Execution options: 1 = capture sweep blocks into a hipGraph and replay. */
 int vg_batch_set_graph(vg_batch* batch, int enabled);
-/* Tile of the Ising sc (int8 and bit-packed) and bcc passes: z planes
- * marched by one work item (1, 2, 4 or 8; default 4), set for all three
+/* With graphs enabled: captures, instantiates and uploads the graphs that a
+ * run of `sweeps` sweeps replays (a block of the batch's full length and
+ * one of the remainder), without running any sweep, so that the run itself
+ * only launches.  No counterpart in the reference (its sweeps are plain
+ * loops); for latency-sensitive callers and timing.  A layout change (tile,
+ * autotune) drops the captured graphs. */
+int vg_batch_prepare(vg_batch* batch, uint64_t sweeps);
+/* Tile of the Ising sc (int8 and bit-packed), bcc and fcc passes: z planes
+ * marched by one work item (1, 2, 4 or 8; default 4), set for all four
  * groups; get_tile reports the first of them that has chains.  Results do
  * not depend on it. */
 int vg_batch_set_tile(vg_batch* batch, uint32_t tile);
 int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile);
 /* Times `trial_sweeps` sweeps with every tile and keeps the fastest, for
- * each of the three groups.  The chains' spins and sweep counters are
+ * each of the four groups.  The chains' spins and sweep counters are
  * restored after every trial, so tuning does not change any trajectory. */
 int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen_tile);
 /* Last run's device time in milliseconds (hipEvent pair around the sweeps). */
@@ -242,7 +249,7 @@ typedef struct vg_server_config {
     uint64_t memory_bytes;   /* device pool holding the admitted jobs' spins */
     uint32_t max_jobs;       /* jobs on the device at once (slots) */
     uint32_t quantum_sweeps; /* sweeps per scheduling quantum, 1..256 */
-    uint32_t tile;           /* Ising sc / bcc tile: 1, 2, 4 or 8 */
+    uint32_t tile;           /* Ising sc / bcc / fcc tile: 1, 2, 4 or 8 */
     int32_t use_graph;       /* 1: replay quanta from hipGraphs while the job set is unchanged */
     uint64_t sweep_budget;   /* work per sweep of a quantum, in Ising-site units (a Heisenberg
                                 site counts 15): jobs with the least remaining work are taken

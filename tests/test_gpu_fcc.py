@@ -58,6 +58,44 @@ def test_fcc_ising_bit_exact(vg, orc, ctx, dims, pbc, T, h, hz, proposal, terms)
         assert np.array_equal(b.get_ising(0), spins)
 
 
+@pytest.mark.parametrize("dims,pbc", [((16, 16, 16), (1, 1, 1)), ((21, 6, 13), (1, 1, 0)),
+                                      ((9, 5, 3), (0, 0, 1)), ((18, 7, 1), (1, 0, 0))])
+def test_fcc_tiles_give_identical_trajectories(vg, orc, ctx, dims, pbc):
+    """The fcc Ising pass marches K = 1, 2, 4 or 8 z planes per work item
+    (the rows a pair needs again on the next plane stay in registers): every
+    tile, including partial last chunks and open z ends, reproduces the
+    oracle."""
+    d, OL = _pair(vg, orc, dims, pbc, "ising", "flip", ("exchange", "zeeman"), seed=6)
+    OM = orc.model(spin="ising", proposal="flip", terms=("exchange", "zeeman"),
+                   temperature=8.0, field=0.1, field_dir=(0.0, 0.0, 1.0))
+    spins = orc.ising_init(OL, 6)
+    want = orc.ising_sweeps(OL, OM, 6, 0, 5, spins)
+    for tile in (1, 2, 4, 8):
+        with vg.Batch(ctx, [d]) as b:
+            b.set_tile(tile)
+            assert b.tile() == tile
+            b.randomize()
+            b.set_thermostat(0, 8.0, 0.1)
+            assert np.array_equal(b.run_sums(5)[0], want), tile
+            assert np.array_equal(b.get_ising(0), spins), tile
+
+
+def test_fcc_tie_queue_with_lanes_that_stop_early(vg, orc, ctx):
+    """fcc 32x6x13 cells, open z, tile 8: 12 lanes of each pass's wave march
+    8 planes and 12 the last 5; tie records past index 12 belong to lanes
+    that stopped early, and all are settled (20 sweeps equal the oracle's)."""
+    d, OL = _pair(vg, orc, (32, 6, 13), (1, 1, 0), "ising", "flip", ("exchange",), seed=13)
+    OM = orc.model(spin="ising", proposal="flip", terms=("exchange",), temperature=9.8)
+    spins = orc.ising_init(OL, 13)
+    want = orc.ising_sweeps(OL, OM, 13, 0, 20, spins)
+    with vg.Batch(ctx, [d]) as b:
+        b.set_tile(8)
+        b.randomize()
+        b.set_thermostat(0, 9.8)
+        assert np.array_equal(b.run_sums(20)[0], want)
+        assert np.array_equal(b.get_ising(0), spins)
+
+
 def test_fcc_ising_set_state_and_sums(vg, orc, ctx):
     """A state set through the C ABI in the natural fcc site order (4 * cell
     + sublattice) comes back unchanged, and the per-sweep sums are the bond

@@ -135,6 +135,24 @@ def test_graph_replay_equals_plain_launches(vg, orc, ctx):
         assert np.array_equal(a, c)
 
 
+def test_prepare_runs_no_sweep(vg, orc, ctx):
+    """vg_batch_prepare captures and uploads the graphs of a run without
+    running a sweep: counters and spins stay, and the prepared run equals the
+    oracle (a full block of 256 sweeps and a remainder of 44)."""
+    d, OL, OM = make_pair(vg, orc, (32, 8, 8), seed=2)
+    spins = orc.ising_init(OL, 2)
+    with vg.Batch(ctx, [d]) as b:
+        b.set_graph(True)
+        b.randomize()
+        b.set_thermostat(0, 4.5115)
+        before = b.get_ising(0)
+        b.prepare(300)
+        assert b.sweep_count(0) == 0 and np.array_equal(b.get_ising(0), before)
+        want = orc.ising_sweeps(OL, OM, 2, 0, 300, spins)
+        assert np.array_equal(b.run_sums(300)[0], want)
+        assert np.array_equal(b.get_ising(0), spins)
+
+
 def test_observables_follow_reference_total_energy(vg, orc, ctx):
     """E and |M| returned per sweep equal the reference's total_energy and
     magnetization of the downloaded state (computed by the oracle's

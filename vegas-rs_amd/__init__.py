@@ -125,6 +125,7 @@ _SIGS = {
     "vg_batch_run": [_vp, C.c_uint64, _vp, _vp],
     "vg_batch_run_sums": [_vp, C.c_uint64, _vp],
     "vg_batch_set_graph": [_vp, C.c_int],
+    "vg_batch_prepare": [_vp, C.c_uint64],
     "vg_batch_last_run_ms": [_vp, _P(C.c_double)],
     "vg_batch_set_timing": [_vp, C.c_int],
     "vg_batch_set_tile": [_vp, C.c_uint32],
@@ -313,6 +314,11 @@ class Batch:
 
     def set_graph(self, enabled: bool) -> None:
         _check(_lib.vg_batch_set_graph(self.h, 1 if enabled else 0))
+
+    def prepare(self, sweeps: int) -> None:
+        """Captures and uploads the graphs a run of `sweeps` sweeps replays
+        (graphs enabled), running no sweep."""
+        _check(_lib.vg_batch_prepare(self.h, sweeps))
 
     def set_tile(self, tile: int) -> None:
         _check(_lib.vg_batch_set_tile(self.h, tile))

@@ -355,13 +355,24 @@ void launch_bcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     }
 }
 
+template <uint32_t A, uint32_t K>
+void launch_ising_fcc(const Group& G, hipStream_t st, uint32_t s) {
+    hipLaunchKernelGGL((vg::ising_fcc_pass<A, K>), dim3(G.grid), dim3(vg::kBlock), 0, st,
+                       G.d_chains, G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials, s,
+                       G.total_blocks, G.grid);
+}
+
 template <uint32_t A>
 void launch_fcc_colour(const Group& G, hipStream_t st, uint32_t s, bool ising) {
     const dim3 grid(G.grid), block(vg::kBlock);
-    if (ising)
-        hipLaunchKernelGGL(vg::ising_fcc_pass<A>, grid, block, 0, st, G.d_chains, G.d_block_chain,
-                           (uint8_t*)G.arena, (int32_t*)G.partials, s, G.total_blocks, G.grid);
-    else
+    if (ising) {
+        switch (G.tile) {
+            case 1: launch_ising_fcc<A, 1>(G, st, s); break;
+            case 2: launch_ising_fcc<A, 2>(G, st, s); break;
+            case 8: launch_ising_fcc<A, 8>(G, st, s); break;
+            default: launch_ising_fcc<A, 4>(G, st, s); break;
+        }
+    } else
         hipLaunchKernelGGL(vg::heis_fcc_pass<A>, grid, block, 0, st, G.d_chains, G.d_block_chain,
                            (float*)G.arena, (int64_t*)G.partials, s, G.total_blocks, G.grid);
 }
@@ -439,11 +450,12 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
     return VG_OK;
 }
 
-int launch_block(vg_batch* B, uint32_t nsw) {
-    if (!B->use_graph) return enqueue_block(B, nsw);
-    hipStream_t st = B->ctx->stream;
+// The captured graph of a block of nsw sweeps, captured, instantiated and
+// uploaded to the device on first use.
+int block_graph(vg_batch* B, uint32_t nsw, hipGraphExec_t* out) {
     auto it = B->graphs.find(nsw);
     if (it == B->graphs.end()) {
+        hipStream_t st = B->ctx->stream;
         // Capture on an idle stream (an earlier block may still be queued).
         VG_HIP(hipStreamSynchronize(st));
         hipGraph_t graph = nullptr;
@@ -461,8 +473,18 @@ int launch_block(vg_batch* B, uint32_t nsw) {
         if (e != hipSuccess)
             return fail(VG_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
         it = B->graphs.emplace(nsw, exec).first;
+        VG_HIP(hipGraphUpload(exec, st));
     }
-    VG_HIP(hipGraphLaunch(it->second, st));
+    *out = it->second;
+    return VG_OK;
+}
+
+int launch_block(vg_batch* B, uint32_t nsw) {
+    if (!B->use_graph) return enqueue_block(B, nsw);
+    hipGraphExec_t exec = nullptr;
+    int rc = block_graph(B, nsw, &exec);
+    if (rc) return rc;
+    VG_HIP(hipGraphLaunch(exec, B->ctx->stream));
     return VG_OK;
 }
 
@@ -1032,6 +1054,24 @@ int vg_batch_run_sums(vg_batch* batch, uint64_t sweeps, vg_sweep_sums* sums) {
     return run_impl(batch, sweeps, sums, nullptr, nullptr);
 }
 
+int vg_batch_prepare(vg_batch* batch, uint64_t sweeps) {
+    if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    if (!batch->use_graph || sweeps == 0) return VG_OK;
+    VG_HIP(hipSetDevice(batch->ctx->device));
+    const uint64_t full = batch->max_block_sweeps;
+    hipGraphExec_t exec = nullptr;
+    if (sweeps >= full) {
+        int rc = block_graph(batch, (uint32_t)full, &exec);
+        if (rc) return rc;
+    }
+    if (sweeps % full) {
+        int rc = block_graph(batch, (uint32_t)(sweeps % full), &exec);
+        if (rc) return rc;
+    }
+    VG_HIP(hipStreamSynchronize(batch->ctx->stream));
+    return VG_OK;
+}
+
 int vg_batch_set_graph(vg_batch* batch, int enabled) {
     if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
     batch->use_graph = enabled != 0;
@@ -1066,7 +1106,7 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
     if (tile != 1 && tile != 2 && tile != 4 && tile != 8)
         return fail(VG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8 (got %u)", tile);
     VG_HIP(hipSetDevice(batch->ctx->device));
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc}) {
         if (batch->groups[g].members.empty()) continue;
         int rc = layout_work(batch, g, tile);
         if (rc) return rc;
@@ -1075,9 +1115,9 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
 }
 
 // The group whose tile vg_batch_get_tile and vg_batch_autotune report: the
-// first of sc int8, sc bit-packed and bcc that has chains.
+// first of sc int8, sc bit-packed, bcc and fcc that has chains.
 static int tile_group(const vg_batch* batch) {
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc})
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc})
         if (!batch->groups[g].members.empty()) return g;
     return kIsingSc;
 }
@@ -1092,9 +1132,9 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
     uint32_t tile_now = 0;
     vg_batch_get_tile(batch, &tile_now);
-    const bool any = !batch->groups[kIsingSc].members.empty() ||
-                     !batch->groups[kIsingBits].members.empty() ||
-                     !batch->groups[kIsingBcc].members.empty();
+    bool any = false;
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc})
+        any = any || !batch->groups[g].members.empty();
     if (!any || trial_sweeps == 0) {
         if (chosen) *chosen = tile_now;
         return VG_OK;
@@ -1134,7 +1174,7 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     int rc = VG_OK;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(VG_ERR_DEVICE, "autotune backup: %s", hipGetErrorString(e));
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc}) {
         Group& G = batch->groups[g];
         if (G.members.empty() || rc != VG_OK) continue;
         const uint32_t start_tile = G.tile;

@@ -40,7 +40,7 @@ __global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t,
                             int64_t*);
 // fcc passes: one instantiation per colour A (sublattice), args (chains,
 // block -> chain map, arena, partials, sweep, total blocks, grid).
-template <uint32_t A>
+template <uint32_t A, uint32_t K>
 __global__ void ising_fcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
                                uint32_t, uint32_t);
 template <uint32_t A>
@@ -96,7 +96,9 @@ inline bool is_ising(int g) {
            g == kIsingWolff;
 }
 // Groups whose pass marches a tile of z planes per work item.
-inline bool is_zmarch(int g) { return g == kIsingSc || g == kIsingBits || g == kIsingBcc; }
+inline bool is_zmarch(int g) {
+    return g == kIsingSc || g == kIsingBits || g == kIsingBcc || g == kIsingFcc;
+}
 // Colours (planes, passes per sweep) of a group's chains; Wolff chains
 // have no colour passes.
 inline uint32_t colours_of(int g) {
@@ -126,7 +128,7 @@ struct Group {
     ChainDev* h_staging2 = nullptr;  // second one, for the block after (batches)
     uint32_t* d_block_chain = nullptr;
     uint32_t block_chain_cap = 0;
-    uint32_t tile = 4;  // Ising sc/bcc: z planes per work item (template parameter of the pass)
+    uint32_t tile = 4;  // Ising sc/bcc/fcc: z planes per work item (template parameter)
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
     void* arena = nullptr;
