@@ -537,10 +537,11 @@ def one_at_a_time_seconds(vg, ctx, jobs, probe_sweeps=60):
     return sum(j[4][2] * per_sweep[j[4][:2]] for j in jobs)
 
 
-def run_served(vg, ctx, n_jobs, rates, gen_seed=2025, budget=0):
+def run_served(vg, ctx, n_jobs, rates, gen_seed=2025, budget=0, graph=False):
     """Poisson arrivals at each rate: one thread submits each job at its
     arrival time; latency = finish - arrival.  Throughput = attempts of all
-    jobs / (last finish - first arrival)."""
+    jobs / (last finish - first arrival).  graph: the server replays quanta
+    from hipGraphs while the running set is unchanged."""
     import numpy as np
     jobs = served_jobs(vg, n_jobs, gen_seed)
     out = []
@@ -549,7 +550,7 @@ def run_served(vg, ctx, n_jobs, rates, gen_seed=2025, budget=0):
         arrivals = np.cumsum(arr_rng.exponential(1.0 / rate, size=len(jobs)))
         log(f"served: {len(jobs)} jobs at {rate}/s")
         with vg.Server(ctx, memory_bytes=16 << 30, max_jobs=1024, quantum_sweeps=100,
-                       tile=2, sweep_budget=budget) as srv:
+                       tile=2, use_graph=graph, sweep_budget=budget) as srv:
             t0 = time.perf_counter()
             ids, sub_t = [], []
             for (kind, d, stages, _, _), ta in zip(jobs, arrivals):
@@ -637,6 +638,8 @@ def main():
                     help="side of the HBM-resident Ising chain")
     ap.add_argument("--resident-sweeps", type=int, default=100)
     ap.add_argument("--serve-rates", default="50,500")
+    ap.add_argument("--serve-graph", action="store_true",
+                    help="the server replays quanta from hipGraphs")
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="server per-sweep work budget (Ising-site units; 0 = none)")
     ap.add_argument("--batched-sweeps", type=int, default=10000)
@@ -701,7 +704,7 @@ def main():
     if args.workload == "served":
         rates = [float(r) for r in args.serve_rates.split(",")]
         with vg.Context(0) as ctx:
-            print(json.dumps(run_served(vg, ctx, args.serve_jobs, rates,
+            print(json.dumps(run_served(vg, ctx, args.serve_jobs, rates, graph=args.serve_graph,
                                         budget=args.serve_budget)), flush=True)
         return
     temps = [3.5, 4.5115, 6.0]
@@ -821,7 +824,8 @@ def main():
                              kind=vg.BCC)
         out["wolff"] = run_wolff(vg, ctx, 32, 64, 64, args.wolff_steps)
         rates = [float(r) for r in args.serve_rates.split(",")]
-        out["served"] = run_served(vg, ctx, args.serve_jobs, rates, budget=args.serve_budget)
+        out["served"] = run_served(vg, ctx, args.serve_jobs, rates, budget=args.serve_budget,
+                                   graph=args.serve_graph)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -61,6 +61,26 @@ def test_alone_equals_in_mix(vg, ctx):
             assert np.array_equal(a, b), (budget, k)
 
 
+def test_graph_replay_equals_plain_quanta(vg, ctx):
+    """With use_graph the server replays each quantum from a captured
+    hipGraph while the job set is unchanged (recaptured when it changes or
+    the quantum length differs): every job's results equal plain launches."""
+    jobs = _jobs(vg)
+    runs = []
+    for graph in (False, True):
+        with vg.Server(ctx, memory_bytes=400 << 10, max_jobs=5, quantum_sweeps=48,
+                       use_graph=graph) as srv:
+            ids = [srv.submit(d, stages) for d, stages in jobs]
+            out = []
+            for jid in ids:
+                st = srv.wait(jid, 120.0)
+                assert st.state == vg.JOB_DONE, st.state
+                out.append(srv.results(jid))
+            runs.append(out)
+    for k, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a, b), k
+
+
 def test_server_equals_batch_program(vg, ctx):
     """The server's stage results equal the same program driven through a
     one-chain batch with the StatSensor formulas, accumulated in sweep order."""
