@@ -376,15 +376,18 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
     kind = vg.BCC if bcc else vg.FCC
     per_cell, passes, name = (2, 2, "bcc") if bcc else (4, 4, "fcc")
     out = {}
-    for spin, side, temps in (("ising", cells, [5.0, 6.35, 8.0] if bcc else [8.0, 9.8, 12.0]),
-                              ("heisenberg", heis_cells,
-                               [1.0, 2.1, 3.5] if bcc else [1.5, 3.2, 5.0])):
-        ising = spin == "ising"
+    spins = [("ising", cells, [5.0, 6.35, 8.0] if bcc else [8.0, 9.8, 12.0]),
+             ("heisenberg", heis_cells, [1.0, 2.1, 3.5] if bcc else [1.5, 3.2, 5.0])]
+    if bcc:  # the Ising chains again, stored one bit per site
+        spins.append(("ising_bits", cells, [5.0, 6.35, 8.0]))
+    for spin, side, temps in spins:
+        ising = spin != "heisenberg"
         H = (vg.hamiltonian(exchange=1.0) if ising
              else vg.hamiltonian(exchange=1.0, zeeman=True, anisotropy=0.1))
         L = vg.lattice(kind, side, side, side)
-        descs = [vg.chain(L, vg.ISING if ising else vg.HEISENBERG,
-                          vg.FLIP if ising else vg.RANDOM, H, 7) for _ in temps]
+        kind_spin = {"ising": vg.ISING, "ising_bits": vg.ISING_BITS, "heisenberg": vg.HEISENBERG}
+        descs = [vg.chain(L, kind_spin[spin], vg.FLIP if ising else vg.RANDOM, H, 7)
+                 for _ in temps]
         n = per_cell * side ** 3
         with vg.Batch(ctx, descs) as b:
             b.randomize()
@@ -401,7 +404,7 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
         value = n * len(temps) * sweeps / wall
         per_launch = n * len(temps) // passes
         avg_ms = dev_ms / (passes * sweeps)
-        bpa = 2 if ising else 24
+        bpa = {"ising": 2, "ising_bits": 0.25, "heisenberg": 24}[spin]
         achieved = bpa * per_launch / (avg_ms * 1e-3) / 1e9
         log(f"{name} {spin}: {value:.4e} attempts/s, pass {avg_ms * 1e3:.1f} us -> "
             f"{achieved:.0f} GB/s")
@@ -414,7 +417,8 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
             "ms_per_step": wall * 1e3 / sweeps,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "kernel": f"{'ising' if ising else 'heis'}_{name}_pass",
+                         "kernel": (f"{'ising' if ising else 'heis'}_{name}_pass"
+                                    + ("<bits>" if spin == "ising_bits" else "")),
                          "algorithmic_bytes_per_launch": bpa * per_launch,
                          "mean_launch_us": avg_ms * 1e3},
             "observables_last_sweep": {f"T={T}": {"E/N": float(e[i, -1]) / n,

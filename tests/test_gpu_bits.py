@@ -1,6 +1,7 @@
-"""Bit-packed Ising chains (1 bit per site): the same trajectories as the
-int8 chains and the CPU oracle, bit for bit, over sizes whose rows end inside
-a 16-position segment, open and periodic axes, both proposals and fields."""
+"""Bit-packed Ising chains (1 bit per site) on sc and bcc lattices: the same
+trajectories as the int8 chains and the CPU oracle, bit for bit, over sizes
+whose rows end inside a 16-position segment, open and periodic axes, both
+proposals and fields."""
 import numpy as np
 import pytest
 
@@ -19,8 +20,8 @@ CASES = [
 ]
 
 
-def _desc(vg, dims, pbc, spin, proposal, terms, seed):
-    L = vg.lattice(vg.SC, *dims, pbc=tuple(bool(p) for p in pbc))
+def _desc(vg, dims, pbc, spin, proposal, terms, seed, kind=None):
+    L = vg.lattice(vg.SC if kind is None else kind, *dims, pbc=tuple(bool(p) for p in pbc))
     kw = {"zeeman": "zeeman" in terms}
     if "exchange" in terms:
         kw["exchange"] = 1.0
@@ -54,6 +55,54 @@ def test_bits_match_oracle_and_int8(vg, orc, ctx, dims, pbc, T, h, proposal, ter
         assert np.array_equal(b.get_ising(1), spins)
 
 
+BCC_CASES = [
+    # dims (cells), pbc, T, h, proposal, terms
+    ((16, 16, 16), (1, 1, 1), 6.35, 0.0, "flip", ("exchange",)),
+    ((17, 4, 3), (0, 1, 0), 4.0, 0.2, "flip", ("exchange", "zeeman")),
+    ((33, 5, 7), (1, 1, 1), 5.0, 0.0, "random", ("exchange", "zeeman")),
+    ((40, 2, 2), (1, 0, 1), 3.0, -0.1, "flip", ("exchange", "zeeman")),
+    ((5, 3, 13), (0, 0, 0), 30.0, 0.5, "random", ("exchange", "zeeman")),
+]
+
+
+@pytest.mark.parametrize("dims,pbc,T,h,proposal,terms", BCC_CASES)
+def test_bcc_bits_match_oracle_and_int8(vg, orc, ctx, dims, pbc, T, h, proposal, terms):
+    hd = (0.0, 0.0, 1.0 if h >= 0 else -1.0)
+    OL = orc.lattice("bcc", *dims, pbc=pbc)
+    OM = orc.model(spin="ising", proposal=proposal, terms=terms, temperature=T, field=abs(h),
+                   field_dir=hd)
+    spins = orc.ising_init(OL, 17)
+    descs = [_desc(vg, dims, pbc, vg.ISING_BITS, proposal, terms, 17, vg.BCC),
+             _desc(vg, dims, pbc, vg.ISING, proposal, terms, 17, vg.BCC)]
+    with vg.Batch(ctx, descs) as b:
+        b.randomize()
+        assert np.array_equal(b.get_ising(0), spins)
+        for i in range(2):
+            b.set_thermostat(i, T, abs(h), hd)
+        for tile, n in [(4, 2), (1, 3), (8, 5)]:
+            b.set_tile(tile)
+            got = b.run_sums(n)
+            want = orc.ising_sweeps(OL, OM, 17, b.sweep_count(0) - n, n, spins)
+            assert np.array_equal(got[0], want) and np.array_equal(got[1], want), tile
+            assert np.array_equal(b.get_ising(0), spins), tile
+            assert np.array_equal(b.get_ising(1), spins), tile
+
+
+def test_bcc_bits_set_get_and_refused_on_fcc(vg, ctx):
+    """A bcc state set in the natural site order (2 * cell + sublattice)
+    comes back unchanged from the bit planes; bit-packed fcc is refused."""
+    d = _desc(vg, (21, 6, 5), (1, 1, 0), vg.ISING_BITS, "flip", ("exchange",), 5, vg.BCC)
+    rng = np.random.default_rng(8)
+    s = rng.choice(np.array([-1, 1], dtype=np.int8), size=2 * 21 * 6 * 5)
+    with vg.Batch(ctx, [d]) as b:
+        b.set_ising(0, s)
+        assert np.array_equal(b.get_ising(0), s)
+    with pytest.raises(vg.VegasError) as e:
+        vg.Batch(ctx, [_desc(vg, (4, 4, 4), (1, 1, 1), vg.ISING_BITS, "flip", ("exchange",), 5,
+                             vg.FCC)])
+    assert e.value.kind == "NotImplemented"
+
+
 def test_bits_set_get_tiles_graph(vg, ctx):
     d = _desc(vg, (34, 6, 13), (1, 1, 0), vg.ISING_BITS, "flip", ("exchange", "zeeman"), 5)
     rng = np.random.default_rng(3)
@@ -79,21 +128,22 @@ def test_autotune_restores_every_group(vg, orc, ctx):
     descs = [vg.chain(vg.lattice(vg.SC, 32, 32, 32), vg.ISING, vg.FLIP, H, 1),
              vg.chain(vg.lattice(vg.SC, 32, 32, 32), vg.ISING_BITS, vg.FLIP, H, 2),
              vg.chain(vg.lattice(vg.SC, 8, 8, 8), vg.HEISENBERG, vg.RANDOM, Hh, 3),
-             vg.chain(vg.lattice(vg.BCC, 6, 6, 6), vg.ISING, vg.FLIP, H, 4)]
+             vg.chain(vg.lattice(vg.BCC, 6, 6, 6), vg.ISING, vg.FLIP, H, 4),
+             vg.chain(vg.lattice(vg.BCC, 18, 6, 6), vg.ISING_BITS, vg.FLIP, H, 5)]
     runs = []
     for tune in (False, True):
         with vg.Batch(ctx, descs) as b:
             b.randomize()
-            for i in range(4):
+            for i in range(5):
                 b.set_thermostat(i, 3.0 if i != 2 else 1.2, 0.05)
             b.run(3)
             if tune:
                 assert b.autotune(3) in (1, 2, 4, 8)
             runs.append((b.run_sums(5), b.get_ising(0), b.get_ising(1), b.get_heisenberg(2),
-                         b.get_ising(3), [b.sweep_count(i) for i in range(4)]))
+                         b.get_ising(3), b.get_ising(4), [b.sweep_count(i) for i in range(5)]))
     for a, c in zip(runs[0], runs[1]):
         assert np.array_equal(np.asarray(a), np.asarray(c))
-    assert runs[1][-1] == [8, 8, 8, 8]
+    assert runs[1][-1] == [8, 8, 8, 8, 8]
 
 
 def test_bits_memory_is_an_eighth(vg, ctx):

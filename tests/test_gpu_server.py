@@ -27,6 +27,8 @@ def _jobs(vg):
     J.append((d, vg.cool_down(2.5, 1.5, 0.5, relax=0, steps=70)))
     d = vg.chain(vg.lattice(vg.SC, 12, 12, 12), vg.ISING, vg.RANDOM, ising, 302)
     J.append((d, vg.cool_down(5.0, 4.0, 0.5, relax=33, steps=101, field=0.1)))
+    d = vg.chain(vg.lattice(vg.BCC, 20, 6, 6), vg.ISING_BITS, vg.FLIP, ising, 303)
+    J.append((d, vg.relax(steps=90, temperature=5.0)))
     return J
 
 

@@ -45,8 +45,8 @@ int validate_desc(const vg_chain_desc& d) {
         return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
     if (d.spin != VG_SPIN_ISING && d.spin != VG_SPIN_HEISENBERG && d.spin != VG_SPIN_ISING_BITS)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown spin kind %d", d.spin);
-    if (d.spin == VG_SPIN_ISING_BITS && L.kind != VG_LATTICE_SC)
-        return fail(VG_ERR_NOT_IMPLEMENTED, "bit-packed Ising chains run on sc lattices");
+    if (d.spin == VG_SPIN_ISING_BITS && L.kind == VG_LATTICE_FCC)
+        return fail(VG_ERR_NOT_IMPLEMENTED, "bit-packed Ising chains run on sc and bcc lattices");
     if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM &&
         d.proposal != VG_PROPOSAL_WOLFF)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown proposal %d", d.proposal);
@@ -192,7 +192,7 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
     }
     const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
     const uint32_t align = is_ising(g) ? 64u : 16u;  // 64-byte aligned rows
-    if (g == kIsingBits)  // bytes: whole 16-position segments, 16-byte aligned rows
+    if (is_bits(g))  // bytes: whole 16-position segments, 16-byte aligned rows
         d.pitch = (uint32_t)round_up((d.w + 15) / 16 * 2, 16);
     else
         d.pitch = (uint32_t)round_up(d.w, align);
@@ -224,7 +224,7 @@ void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev*
                        hipStream_t st) {
     const uint64_t n = (uint64_t)d.ncol * d.plane;  // one thread per plane byte
     const dim3 grid((uint32_t)((n + 255) / 256));
-    if (g == kIsingBits)
+    if (is_bits(g))
         hipLaunchKernelGGL(vg::ising_init_bits, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
     else
         hipLaunchKernelGGL(vg::ising_init, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
@@ -331,25 +331,26 @@ void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     }
 }
 
+template <bool BITS>
 void launch_bcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     const dim3 grid(G.grid), block(vg::kBlock);
     uint8_t* arena = (uint8_t*)G.arena;
     int32_t* part = (int32_t*)G.partials;
     switch (G.tile) {
         case 1:
-            hipLaunchKernelGGL(vg::ising_bcc_pass<1>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_bcc_pass<1, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
         case 2:
-            hipLaunchKernelGGL(vg::ising_bcc_pass<2>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_bcc_pass<2, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
         case 8:
-            hipLaunchKernelGGL(vg::ising_bcc_pass<8>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_bcc_pass<8, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
         default:
-            hipLaunchKernelGGL(vg::ising_bcc_pass<4>, grid, block, 0, st, G.d_chains,
+            hipLaunchKernelGGL((vg::ising_bcc_pass<4, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
             break;
     }
@@ -410,7 +411,9 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                 else if (g == kIsingBits)
                     launch_ising_pass<true>(G, st, c, s);
                 else if (g == kIsingBcc)
-                    launch_bcc_pass(G, st, c, s);
+                    launch_bcc_pass<false>(G, st, c, s);
+                else if (g == kIsingBccBits)
+                    launch_bcc_pass<true>(G, st, c, s);
                 else if (g == kIsingFcc)
                     launch_fcc_pass(G, st, c, s, true);
                 else if (g == kHeisSc)
@@ -925,7 +928,7 @@ int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
     VG_HIP(hipMalloc(&tmp, ch.n_sites));
     hipError_t e = hipMemcpyAsync(tmp, spins, ch.n_sites, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        if (ch.group == kIsingBits) {
+        if (is_bits(ch.group)) {
             const uint64_t nb = 2ull * G.host[ch.index].plane;
             hipLaunchKernelGGL(vg::ising_scatter_bits, dim3((uint32_t)((nb + 255) / 256)),
                                dim3(256), 0, st, G.d_chains + ch.index, (uint8_t*)G.arena, tmp);
@@ -961,7 +964,7 @@ int vg_batch_get_ising(vg_batch* batch, uint32_t chain, int8_t* spins) {
     }
     int8_t* tmp = nullptr;
     VG_HIP(hipMalloc(&tmp, ch.n_sites));
-    if (ch.group == kIsingBits)
+    if (is_bits(ch.group))
         hipLaunchKernelGGL(vg::ising_gather_bits, dim3((uint32_t)((ch.n_sites + 255) / 256)),
                            dim3(256), 0, st, G.d_chains + ch.index, (const uint8_t*)G.arena, tmp,
                            ch.n_sites);
@@ -1106,7 +1109,7 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
     if (tile != 1 && tile != 2 && tile != 4 && tile != 8)
         return fail(VG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8 (got %u)", tile);
     VG_HIP(hipSetDevice(batch->ctx->device));
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc}) {
         if (batch->groups[g].members.empty()) continue;
         int rc = layout_work(batch, g, tile);
         if (rc) return rc;
@@ -1115,9 +1118,10 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
 }
 
 // The group whose tile vg_batch_get_tile and vg_batch_autotune report: the
-// first of sc int8, sc bit-packed, bcc and fcc that has chains.
+// first of sc int8, sc bit-packed, bcc int8, bcc bit-packed and fcc that has
+// chains.
 static int tile_group(const vg_batch* batch) {
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc})
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc})
         if (!batch->groups[g].members.empty()) return g;
     return kIsingSc;
 }
@@ -1133,7 +1137,7 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     uint32_t tile_now = 0;
     vg_batch_get_tile(batch, &tile_now);
     bool any = false;
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc})
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc})
         any = any || !batch->groups[g].members.empty();
     if (!any || trial_sweeps == 0) {
         if (chosen) *chosen = tile_now;
@@ -1174,7 +1178,7 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     int rc = VG_OK;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(VG_ERR_DEVICE, "autotune backup: %s", hipGetErrorString(e));
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingFcc}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc}) {
         Group& G = batch->groups[g];
         if (G.members.empty() || rc != VG_OK) continue;
         const uint32_t start_tile = G.tile;

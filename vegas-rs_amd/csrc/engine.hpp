@@ -31,7 +31,7 @@ __global__ void ising_scatter(const ChainDev*, uint8_t*, const int8_t*, uint64_t
 __global__ void ising_gather(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
 __global__ void heis_sc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                              uint32_t, uint32_t, uint32_t);
-template <uint32_t K>
+template <uint32_t K, bool BITS>
 __global__ void ising_bcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
                                uint32_t, uint32_t, uint32_t);
 __global__ void heis_bcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
@@ -88,16 +88,20 @@ enum GroupKind {
     kIsingFcc = 5,
     kHeisFcc = 6,
     kIsingWolff = 7,  // Wolff cluster steps, any lattice, natural site order
-    kNumGroups = 8
+    kIsingBccBits = 8,  // bcc, 1 bit per site
+    kNumGroups = 9
 };
 
 inline bool is_ising(int g) {
     return g == kIsingSc || g == kIsingBcc || g == kIsingBits || g == kIsingFcc ||
-           g == kIsingWolff;
+           g == kIsingWolff || g == kIsingBccBits;
 }
+// Groups whose planes hold one bit per site.
+inline bool is_bits(int g) { return g == kIsingBits || g == kIsingBccBits; }
 // Groups whose pass marches a tile of z planes per work item.
 inline bool is_zmarch(int g) {
-    return g == kIsingSc || g == kIsingBits || g == kIsingBcc || g == kIsingFcc;
+    return g == kIsingSc || g == kIsingBits || g == kIsingBcc || g == kIsingFcc ||
+           g == kIsingBccBits;
 }
 // Colours (planes, passes per sweep) of a group's chains; Wolff chains
 // have no colour passes.
@@ -110,7 +114,7 @@ inline int group_of(const vg_chain_desc& d) {
     if (d.proposal == VG_PROPOSAL_WOLFF) return kIsingWolff;
     if (d.spin == VG_SPIN_HEISENBERG)
         return k == VG_LATTICE_FCC ? kHeisFcc : (k == VG_LATTICE_BCC ? kHeisBcc : kHeisSc);
-    if (d.spin == VG_SPIN_ISING_BITS) return kIsingBits;
+    if (d.spin == VG_SPIN_ISING_BITS) return k == VG_LATTICE_BCC ? kIsingBccBits : kIsingBits;
     return k == VG_LATTICE_FCC ? kIsingFcc : (k == VG_LATTICE_BCC ? kIsingBcc : kIsingSc);
 }
 
@@ -128,7 +132,7 @@ struct Group {
     ChainDev* h_staging2 = nullptr;  // second one, for the block after (batches)
     uint32_t* d_block_chain = nullptr;
     uint32_t block_chain_cap = 0;
-    uint32_t tile = 4;  // Ising sc/bcc/fcc: z planes per work item (template parameter)
+    uint32_t tile = 4;  // Ising sc/bcc/fcc (int8, bits): z planes per work item (template parameter)
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
     void* arena = nullptr;

@@ -24,6 +24,8 @@
 // low 24 bits drawn only on a high-byte tie) is below the class threshold.
 // As in the sc pass, a row with ties leaves a record in the wave's LDS queue
 // and one lane per record settles them once the wave has finished its planes.
+// BITS: the planes hold one bit per site (ising_swar.hpp's storage forms);
+// a segment is a 16-bit load expanded to the same 16 bytes.
 #include "ising_swar.hpp"
 
 namespace vg {
@@ -34,7 +36,7 @@ constexpr uint32_t kTieRows = 64;  // tie records per wave (one per row with tie
 constexpr uint32_t kTieWords = 10;
 }
 
-template <uint32_t K>
+template <uint32_t K, bool BITS>
 __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict__ chains,
                                                       const uint32_t* __restrict__ block_chain,
                                                       uint8_t* __restrict__ arena,
@@ -111,13 +113,14 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
         }
         // The pair of rows (y, zz), (y', zz): byte sums at the segment (q) and
         // at xpos (e); absent rows read zeros.
+        const uint32_t s0 = seg_off<BITS>(i0);
         auto load_pair = [&](uint32_t zz, bool zok, u32x4& q, uint32_t& e) {
             const uint32_t r0 = (y + ly * zz) * pitch, r1 = (y2 + ly * zz) * pitch;
-            const u32x4 a = bload16(ro, zok ? r0 + i0 : kNoRow);
-            const u32x4 c = bload16(ro, zok && yok ? r1 + i0 : kNoRow);
+            const u32x4 a = load_seg<BITS>(ro, zok ? r0 + s0 : kNoRow);
+            const u32x4 c = load_seg<BITS>(ro, zok && yok ? r1 + s0 : kNoRow);
             q = a + c;
-            e = bload8(ro, zok && x_ok ? r0 + xpos : kNoRow) +
-                bload8(ro, zok && x_ok && yok ? r1 + xpos : kNoRow);
+            e = load_pos<BITS>(ro, r0, xpos, zok && x_ok) +
+                load_pos<BITS>(ro, r1, xpos, zok && x_ok && yok);
         };
 
         const uint32_t z0 = zc * K;
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
             lo_ok = true;
             load_pair(z0, true, q_lo, e_lo);
         }
-        uint32_t off = (y + ly * z0) * pitch + i0;  // own row (y, z), this segment
+        uint32_t off = (y + ly * z0) * pitch + s0;  // own row (y, z), this segment
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             if (k >= nk) break;
@@ -152,7 +155,7 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
             u32x4 q_hi;
             uint32_t e_hi;
             load_pair(zu, hi_ok, q_hi, e_hi);
-            const u32x4 o4 = bload16(rw_own, off);
+            const u32x4 o4 = load_seg<BITS>(rw_own, off);
             // present rows of the four; absent slots 8 - 2 nr (8 - nr at gx)
             const uint32_t nr = (lo_ok ? 1u : 0u) + (lo_ok && yok ? 1u : 0u) + (hi_ok ? 1u : 0u) +
                                 (hi_ok && yok ? 1u : 0u);
@@ -221,12 +224,17 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                 n[kk] = (n[kk] & ~(1u << sh)) | ((bj ^ f) << sh);
             }
             if ((n[0] ^ o[0]) | (n[1] ^ o[1]) | (n[2] ^ o[2]) | (n[3] ^ o[3])) {
-                u32x4 nv;
-                nv.x = n[0];
-                nv.y = n[1];
-                nv.z = n[2];
-                nv.w = n[3];
-                __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+                if (BITS) {
+                    __builtin_amdgcn_raw_buffer_store_b16(
+                        (unsigned short)pack16(n[0], n[1], n[2], n[3]), rw_own, off, 0, 0);
+                } else {
+                    u32x4 nv;
+                    nv.x = n[0];
+                    nv.y = n[1];
+                    nv.z = n[2];
+                    nv.w = n[3];
+                    __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+                }
             }
             uint32_t nc = 0, cs = 0;
             int32_t upc = 0;
@@ -266,7 +274,7 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                         *reinterpret_cast<uint4*>(rec + 6) = make_uint4(n[0], n[1], n[2], n[3]);
                     } else {
                         // Queue full (practically never): resolve in this lane.
-                        uint32_t tm = tmask;
+                        uint32_t tm = tmask, flips = 0;
                         while (tm) {
                             const uint32_t j = __builtin_ctz(tm);
                             tm &= tm - 1u;
@@ -276,12 +284,17 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                                 2u * ((cnt[kk] >> sh) & 0xFFu) + (j == gx ? a_edge : a_in);
                             if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t,
                                            colour, k0, k1)) {
-                                own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                if (!BITS) own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                flips |= 1u << j;
                                 const int32_t ds = 2 - 4 * (int32_t)bj;
                                 m_part += ds;
                                 if (colour == 1) b_part += ds * ((int32_t)S - 8);
                             }
                         }
+                        if (BITS && flips)
+                            __builtin_amdgcn_raw_buffer_store_b16(
+                                (unsigned short)(pack16(n[0], n[1], n[2], n[3]) ^ flips), rw_own,
+                                off, 0, 0);
                     }
                 }
                 nrec += __builtin_popcountll(has);
@@ -315,7 +328,7 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
             const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
             const uint32_t zz = (r2 >> 1) * K + k;
             const uint32_t row = yy + ly * zz;
-            uint32_t tm = rec[1];
+            uint32_t tm = rec[1], flips = 0;
             while (tm) {
                 const uint32_t j = __builtin_ctz(tm);
                 tm &= tm - 1u;
@@ -325,12 +338,19 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                 const uint32_t S = 2u * c + (j == gx ? a_edge : a_in);
                 const uint32_t pos = sg * kIsingSeg + j;
                 if (tie_accept(lut, bj ? always1 : always0, bj, S, pos, row, t, colour, k0, k1)) {
-                    own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    if (!BITS) own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    flips |= 1u << j;
                     const int32_t ds = 2 - 4 * (int32_t)bj;
                     m_part += ds;
                     if (colour == 1) b_part += ds * ((int32_t)S - 8);
                 }
             }
+            // Bit-packed: the owner stored the row's 16 bits with the tied
+            // sites unchanged; this lane rewrites them with the accepted ones.
+            if (BITS && flips)
+                __builtin_amdgcn_raw_buffer_store_b16(
+                    (unsigned short)(pack16(rec[6], rec[7], rec[8], rec[9]) ^ flips), rw_own,
+                    row * pitch + seg_off<BITS>(sg * kIsingSeg), 0, 0);
         }
     }
 
@@ -344,13 +364,29 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
     }
 }
 
-template __global__ void ising_bcc_pass<1>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                           uint32_t, uint32_t, uint32_t, uint32_t);
-template __global__ void ising_bcc_pass<2>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                           uint32_t, uint32_t, uint32_t, uint32_t);
-template __global__ void ising_bcc_pass<4>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                           uint32_t, uint32_t, uint32_t, uint32_t);
-template __global__ void ising_bcc_pass<8>(const ChainDev*, const uint32_t*, uint8_t*, int32_t*,
-                                           uint32_t, uint32_t, uint32_t, uint32_t);
+template __global__ void ising_bcc_pass<1, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
+template __global__ void ising_bcc_pass<2, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
+template __global__ void ising_bcc_pass<4, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
+template __global__ void ising_bcc_pass<8, false>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
+template __global__ void ising_bcc_pass<1, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
+template __global__ void ising_bcc_pass<2, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
+template __global__ void ising_bcc_pass<4, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
+template __global__ void ising_bcc_pass<8, true>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t);
 
 }  // namespace vg

@@ -41,42 +41,6 @@ __device__ __forceinline__ uint32_t pick4(uint32_t k, uint32_t a, uint32_t b, ui
     return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
 }
 
-// The two storage forms of a colour plane.  Int8: one byte per position,
-// row segments of 16 positions are 16-byte loads.  Bit-packed: bit (i & 7)
-// of byte i >> 3 of the row, a segment is one 16-bit load expanded to the
-// same 16 bytes in registers, so everything after the loads is shared.
-template <bool BITS>
-__device__ __forceinline__ uint32_t seg_off(uint32_t i) {
-    return BITS ? i >> 3 : i;
-}
-
-template <bool BITS>
-__device__ __forceinline__ u32x4 load_seg(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    if (!BITS) return bload16(r, off);
-    const uint32_t v = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
-    u32x4 e;
-    e.x = nibble_bytes(v);
-    e.y = nibble_bytes(v >> 4);
-    e.z = nibble_bytes(v >> 8);
-    e.w = nibble_bytes(v >> 12);
-    return e;
-}
-
-// Spin bit of position i of the row starting at byte offset roff (kNoRow:
-// absent, reads 0).
-template <bool BITS>
-__device__ __forceinline__ uint32_t load_pos(__amdgpu_buffer_rsrc_t r, uint32_t roff, uint32_t i,
-                                             bool present) {
-    const uint32_t v = bload8(r, present ? roff + seg_off<BITS>(i) : kNoRow);
-    return BITS ? (v >> (i & 7u)) & 1u : v;
-}
-
-// 16 bytes of 0/1 -> 16 bits (byte j -> bit j).
-__device__ __forceinline__ uint32_t pack16(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
-    return (((a & kOne) * 0x10204080u) >> 28) | ((((b & kOne) * 0x10204080u) >> 28) << 4) |
-           ((((c & kOne) * 0x10204080u) >> 28) << 8) | ((((d & kOne) * 0x10204080u) >> 28) << 12);
-}
-
 template <uint32_t K, bool BITS>
 __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict__ chains,
                                                      const uint32_t* __restrict__ block_chain,

@@ -77,10 +77,16 @@ __global__ void ising_gather(const ChainDev* __restrict__ chain, const uint8_t* 
 
 // Bit-packed Ising planes (sc): position i of a row is bit (i & 7) of byte
 // i >> 3.  One thread per plane byte; bits past w stay zero.
-__device__ __forceinline__ uint64_t sc_site(const ChainDev& C, uint32_t c, uint32_t row, uint32_t i) {
+// Natural site id of position i of row `row` of colour c: sc x + lx*(y +
+// ly*z) with x = 2i + ((c+y+z)&1); bcc 2*(cell) + c with the cell's x = i.
+__device__ __forceinline__ uint64_t plane_site(const ChainDev& C, uint32_t c, uint32_t row,
+                                               uint32_t i) {
     const uint32_t y = row % C.ly, z = row / C.ly;
-    const uint32_t x = 2u * i + ((c + y + z) & 1u);
-    return (uint64_t)x + (uint64_t)C.lx * ((uint64_t)y + (uint64_t)C.ly * z);
+    if (C.kind == 0) {
+        const uint32_t x = 2u * i + ((c + y + z) & 1u);
+        return (uint64_t)x + (uint64_t)C.lx * ((uint64_t)y + (uint64_t)C.ly * z);
+    }
+    return 2u * ((uint64_t)i + (uint64_t)C.lx * ((uint64_t)y + (uint64_t)C.ly * z)) + c;
 }
 
 // The same draws as ising_init: position i is Up iff word (i & 3) of
@@ -120,7 +126,7 @@ __global__ void ising_scatter_bits(const ChainDev* __restrict__ chain, uint8_t* 
     const uint32_t i0 = 8u * (uint32_t)(e % C.pitch);
     uint32_t v = 0;
     for (uint32_t k = 0; k < 8 && i0 + k < C.w; ++k)
-        if (natural[sc_site(C, c, row, i0 + k)] > 0) v |= 1u << k;
+        if (natural[plane_site(C, c, row, i0 + k)] > 0) v |= 1u << k;
     arena[C.off[c] + e] = (uint8_t)v;
 }
 
