@@ -369,7 +369,8 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
     fcc cells (4 sites each; J = 1, h = 0; T = 8, 9.8, 12 around T_c ~ 9.8)
     and 3 Heisenberg chains of heis_cells^3 cells (J = 1, K = 0.1 z, B = 0.05
     z, uniform-sphere proposals; T = 1.5, 3.2, 5), random starts from seed 7.
-    Attempts/s and each pass kernel's roofline (2 B / 24 B per attempt).
+    The Ising chains run again stored one bit per site.  Attempts/s and each
+    pass kernel's roofline (2 B / 0.25 B / 24 B per attempt).
     With kind = vg.BCC the same on bcc lattices (2 sites per cell, two colour
     passes; Ising T = 5, 6.35, 8 around T_c ~ 6.35, Heisenberg T = 1, 2.1, 3.5)."""
     bcc = kind == vg.BCC
@@ -378,8 +379,8 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
     out = {}
     spins = [("ising", cells, [5.0, 6.35, 8.0] if bcc else [8.0, 9.8, 12.0]),
              ("heisenberg", heis_cells, [1.0, 2.1, 3.5] if bcc else [1.5, 3.2, 5.0])]
-    if bcc:  # the Ising chains again, stored one bit per site
-        spins.append(("ising_bits", cells, [5.0, 6.35, 8.0]))
+    # the Ising chains again, stored one bit per site
+    spins.append(("ising_bits", cells, spins[0][2]))
     for spin, side, temps in spins:
         ising = spin != "heisenberg"
         H = (vg.hamiltonian(exchange=1.0) if ising

@@ -117,7 +117,7 @@ typedef struct vg_hamiltonian {
 /* Chain = lattice + spin kind + proposal + Hamiltonian + seed.          */
 /* ------------------------------------------------------------------ */
 #define VG_SPIN_ISING 0          /* IsingSpin (state.rs:61-130), int8 arena */
-#define VG_SPIN_ISING_BITS 1     /* IsingSpin, 1 bit per site arena (sc and bcc lattices) */
+#define VG_SPIN_ISING_BITS 1     /* IsingSpin, 1 bit per site arena (every lattice) */
 #define VG_SPIN_HEISENBERG 2     /* HeisenbergSpin (state.rs:134-192), fp32 xyz */
 
 #define VG_PROPOSAL_FLIP 0       /* MetropolisFlipIntegrator (integrator.rs:100-138) */
@@ -207,14 +207,14 @@ int vg_batch_set_graph(vg_batch* batch, int enabled);
  * loops); for latency-sensitive callers and timing.  A layout change (tile,
  * autotune) drops the captured graphs. */
 int vg_batch_prepare(vg_batch* batch, uint64_t sweeps);
-/* Tile of the Ising sc and bcc (int8 and bit-packed) and fcc passes: z
+/* Tile of the Ising sc, bcc and fcc passes (int8 and bit-packed): z
  * planes marched by one work item (1, 2, 4 or 8; default 4), set for all
- * five groups; get_tile reports the first of them that has chains.  Results
+ * six groups; get_tile reports the first of them that has chains.  Results
  * do not depend on it. */
 int vg_batch_set_tile(vg_batch* batch, uint32_t tile);
 int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile);
 /* Times `trial_sweeps` sweeps with every tile and keeps the fastest, for
- * each of the five groups.  The chains' spins and sweep counters are
+ * each of the six groups.  The chains' spins and sweep counters are
  * restored after every trial, so tuning does not change any trajectory. */
 int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen_tile);
 /* Last run's device time in milliseconds (hipEvent pair around the sweeps). */

@@ -1,4 +1,4 @@
-"""Bit-packed Ising chains (1 bit per site) on sc and bcc lattices: the same
+"""Bit-packed Ising chains (1 bit per site) on sc, bcc and fcc lattices: the same
 trajectories as the int8 chains and the CPU oracle, bit for bit, over sizes
 whose rows end inside a 16-position segment, open and periodic axes, both
 proposals and fields."""
@@ -65,21 +65,34 @@ BCC_CASES = [
 ]
 
 
-@pytest.mark.parametrize("dims,pbc,T,h,proposal,terms", BCC_CASES)
-def test_bcc_bits_match_oracle_and_int8(vg, orc, ctx, dims, pbc, T, h, proposal, terms):
+FCC_CASES = [
+    # dims (cells), pbc, T, h, proposal, terms
+    ((16, 16, 16), (1, 1, 1), 9.8, 0.0, "flip", ("exchange",)),
+    ((17, 4, 3), (0, 1, 0), 6.0, 0.2, "flip", ("exchange", "zeeman")),
+    ((33, 5, 7), (1, 1, 1), 8.0, 0.0, "random", ("exchange", "zeeman")),
+    ((40, 2, 2), (1, 0, 1), 4.0, -0.1, "flip", ("exchange", "zeeman")),
+    ((5, 3, 13), (0, 0, 0), 30.0, 0.5, "random", ("exchange", "zeeman")),
+    ((1, 5, 5), (1, 1, 1), 9.0, 0.0, "flip", ("exchange",)),
+    ((32, 8, 8), (1, 1, 1), 2.0, 0.0, "flip", ("exchange",)),
+]
+
+
+def _bits_vs_int8(vg, orc, ctx, lattice, kind, dims, pbc, T, h, proposal, terms):
+    """The bit-packed and int8 chains of one lattice against the oracle, over
+    every tile."""
     hd = (0.0, 0.0, 1.0 if h >= 0 else -1.0)
-    OL = orc.lattice("bcc", *dims, pbc=pbc)
+    OL = orc.lattice(lattice, *dims, pbc=pbc)
     OM = orc.model(spin="ising", proposal=proposal, terms=terms, temperature=T, field=abs(h),
                    field_dir=hd)
     spins = orc.ising_init(OL, 17)
-    descs = [_desc(vg, dims, pbc, vg.ISING_BITS, proposal, terms, 17, vg.BCC),
-             _desc(vg, dims, pbc, vg.ISING, proposal, terms, 17, vg.BCC)]
+    descs = [_desc(vg, dims, pbc, vg.ISING_BITS, proposal, terms, 17, kind),
+             _desc(vg, dims, pbc, vg.ISING, proposal, terms, 17, kind)]
     with vg.Batch(ctx, descs) as b:
         b.randomize()
         assert np.array_equal(b.get_ising(0), spins)
         for i in range(2):
             b.set_thermostat(i, T, abs(h), hd)
-        for tile, n in [(4, 2), (1, 3), (8, 5)]:
+        for tile, n in [(4, 2), (1, 3), (2, 2), (8, 4)]:
             b.set_tile(tile)
             got = b.run_sums(n)
             want = orc.ising_sweeps(OL, OM, 17, b.sweep_count(0) - n, n, spins)
@@ -88,19 +101,46 @@ def test_bcc_bits_match_oracle_and_int8(vg, orc, ctx, dims, pbc, T, h, proposal,
             assert np.array_equal(b.get_ising(1), spins), tile
 
 
-def test_bcc_bits_set_get_and_refused_on_fcc(vg, ctx):
-    """A bcc state set in the natural site order (2 * cell + sublattice)
-    comes back unchanged from the bit planes; bit-packed fcc is refused."""
-    d = _desc(vg, (21, 6, 5), (1, 1, 0), vg.ISING_BITS, "flip", ("exchange",), 5, vg.BCC)
+@pytest.mark.parametrize("dims,pbc,T,h,proposal,terms", BCC_CASES)
+def test_bcc_bits_match_oracle_and_int8(vg, orc, ctx, dims, pbc, T, h, proposal, terms):
+    _bits_vs_int8(vg, orc, ctx, "bcc", vg.BCC, dims, pbc, T, h, proposal, terms)
+
+
+@pytest.mark.parametrize("dims,pbc,T,h,proposal,terms", FCC_CASES)
+def test_fcc_bits_match_oracle_and_int8(vg, orc, ctx, dims, pbc, T, h, proposal, terms):
+    _bits_vs_int8(vg, orc, ctx, "fcc", vg.FCC, dims, pbc, T, h, proposal, terms)
+
+
+@pytest.mark.parametrize("lattice,T,n", [("bcc", 6.35, 30), ("fcc", 9.8, 20)])
+def test_bits_tie_queue_with_lanes_that_stop_early(vg, orc, ctx, lattice, T, n):
+    """32x6x13 cells, open z, tile 8: 12 lanes of each pass's wave march 8
+    planes and 12 the last 5; the tie records of the bit-packed passes
+    (settled by rewriting the owner's 16-bit word) are all applied."""
+    d = _desc(vg, (32, 6, 13), (1, 1, 0), vg.ISING_BITS, "flip", ("exchange",), 13,
+              getattr(vg, lattice.upper()))
+    OL = orc.lattice(lattice, 32, 6, 13, pbc=(1, 1, 0))
+    OM = orc.model(spin="ising", proposal="flip", terms=("exchange",), temperature=T)
+    spins = orc.ising_init(OL, 13)
+    want = orc.ising_sweeps(OL, OM, 13, 0, n, spins)
+    with vg.Batch(ctx, [d]) as b:
+        b.set_tile(8)
+        b.randomize()
+        b.set_thermostat(0, T)
+        assert np.array_equal(b.run_sums(n)[0], want)
+        assert np.array_equal(b.get_ising(0), spins)
+
+
+@pytest.mark.parametrize("kind,ncol", [("BCC", 2), ("FCC", 4)])
+def test_bits_set_get_natural_order(vg, ctx, kind, ncol):
+    """A bcc or fcc state set in the natural site order (ncol * cell +
+    sublattice) comes back unchanged from the bit planes."""
+    d = _desc(vg, (21, 6, 5), (1, 1, 0), vg.ISING_BITS, "flip", ("exchange",), 5,
+              getattr(vg, kind))
     rng = np.random.default_rng(8)
-    s = rng.choice(np.array([-1, 1], dtype=np.int8), size=2 * 21 * 6 * 5)
+    s = rng.choice(np.array([-1, 1], dtype=np.int8), size=ncol * 21 * 6 * 5)
     with vg.Batch(ctx, [d]) as b:
         b.set_ising(0, s)
         assert np.array_equal(b.get_ising(0), s)
-    with pytest.raises(vg.VegasError) as e:
-        vg.Batch(ctx, [_desc(vg, (4, 4, 4), (1, 1, 1), vg.ISING_BITS, "flip", ("exchange",), 5,
-                             vg.FCC)])
-    assert e.value.kind == "NotImplemented"
 
 
 def test_bits_set_get_tiles_graph(vg, ctx):
@@ -129,21 +169,23 @@ def test_autotune_restores_every_group(vg, orc, ctx):
              vg.chain(vg.lattice(vg.SC, 32, 32, 32), vg.ISING_BITS, vg.FLIP, H, 2),
              vg.chain(vg.lattice(vg.SC, 8, 8, 8), vg.HEISENBERG, vg.RANDOM, Hh, 3),
              vg.chain(vg.lattice(vg.BCC, 6, 6, 6), vg.ISING, vg.FLIP, H, 4),
-             vg.chain(vg.lattice(vg.BCC, 18, 6, 6), vg.ISING_BITS, vg.FLIP, H, 5)]
+             vg.chain(vg.lattice(vg.BCC, 18, 6, 6), vg.ISING_BITS, vg.FLIP, H, 5),
+             vg.chain(vg.lattice(vg.FCC, 18, 6, 6), vg.ISING_BITS, vg.FLIP, H, 6)]
     runs = []
     for tune in (False, True):
         with vg.Batch(ctx, descs) as b:
             b.randomize()
-            for i in range(5):
-                b.set_thermostat(i, 3.0 if i != 2 else 1.2, 0.05)
+            for i in range(6):
+                b.set_thermostat(i, (3.0, 3.0, 1.2, 3.0, 3.0, 7.0)[i], 0.05)
             b.run(3)
             if tune:
                 assert b.autotune(3) in (1, 2, 4, 8)
             runs.append((b.run_sums(5), b.get_ising(0), b.get_ising(1), b.get_heisenberg(2),
-                         b.get_ising(3), b.get_ising(4), [b.sweep_count(i) for i in range(5)]))
+                         b.get_ising(3), b.get_ising(4), b.get_ising(5),
+                         [b.sweep_count(i) for i in range(6)]))
     for a, c in zip(runs[0], runs[1]):
         assert np.array_equal(np.asarray(a), np.asarray(c))
-    assert runs[1][-1] == [8, 8, 8, 8, 8]
+    assert runs[1][-1] == [8] * 6
 
 
 def test_bits_memory_is_an_eighth(vg, ctx):

@@ -45,8 +45,6 @@ int validate_desc(const vg_chain_desc& d) {
         return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
     if (d.spin != VG_SPIN_ISING && d.spin != VG_SPIN_HEISENBERG && d.spin != VG_SPIN_ISING_BITS)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown spin kind %d", d.spin);
-    if (d.spin == VG_SPIN_ISING_BITS && L.kind == VG_LATTICE_FCC)
-        return fail(VG_ERR_NOT_IMPLEMENTED, "bit-packed Ising chains run on sc and bcc lattices");
     if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM &&
         d.proposal != VG_PROPOSAL_WOLFF)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown proposal %d", d.proposal);
@@ -356,34 +354,42 @@ void launch_bcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     }
 }
 
-template <uint32_t A, uint32_t K>
+template <uint32_t A, uint32_t K, bool BITS>
 void launch_ising_fcc(const Group& G, hipStream_t st, uint32_t s) {
-    hipLaunchKernelGGL((vg::ising_fcc_pass<A, K>), dim3(G.grid), dim3(vg::kBlock), 0, st,
+    hipLaunchKernelGGL((vg::ising_fcc_pass<A, K, BITS>), dim3(G.grid), dim3(vg::kBlock), 0, st,
                        G.d_chains, G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials, s,
                        G.total_blocks, G.grid);
 }
 
+template <uint32_t A, bool BITS>
+void launch_ising_fcc_tile(const Group& G, hipStream_t st, uint32_t s) {
+    switch (G.tile) {
+        case 1: launch_ising_fcc<A, 1, BITS>(G, st, s); break;
+        case 2: launch_ising_fcc<A, 2, BITS>(G, st, s); break;
+        case 8: launch_ising_fcc<A, 8, BITS>(G, st, s); break;
+        default: launch_ising_fcc<A, 4, BITS>(G, st, s); break;
+    }
+}
+
+// kind: 0 Ising int8, 1 Ising bit-packed, 2 Heisenberg.
 template <uint32_t A>
-void launch_fcc_colour(const Group& G, hipStream_t st, uint32_t s, bool ising) {
+void launch_fcc_colour(const Group& G, hipStream_t st, uint32_t s, int kind) {
     const dim3 grid(G.grid), block(vg::kBlock);
-    if (ising) {
-        switch (G.tile) {
-            case 1: launch_ising_fcc<A, 1>(G, st, s); break;
-            case 2: launch_ising_fcc<A, 2>(G, st, s); break;
-            case 8: launch_ising_fcc<A, 8>(G, st, s); break;
-            default: launch_ising_fcc<A, 4>(G, st, s); break;
-        }
-    } else
+    if (kind == 0)
+        launch_ising_fcc_tile<A, false>(G, st, s);
+    else if (kind == 1)
+        launch_ising_fcc_tile<A, true>(G, st, s);
+    else
         hipLaunchKernelGGL(vg::heis_fcc_pass<A>, grid, block, 0, st, G.d_chains, G.d_block_chain,
                            (float*)G.arena, (int64_t*)G.partials, s, G.total_blocks, G.grid);
 }
 
-void launch_fcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s, bool ising) {
+void launch_fcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s, int kind) {
     switch (c) {
-        case 0: launch_fcc_colour<0>(G, st, s, ising); break;
-        case 1: launch_fcc_colour<1>(G, st, s, ising); break;
-        case 2: launch_fcc_colour<2>(G, st, s, ising); break;
-        default: launch_fcc_colour<3>(G, st, s, ising); break;
+        case 0: launch_fcc_colour<0>(G, st, s, kind); break;
+        case 1: launch_fcc_colour<1>(G, st, s, kind); break;
+        case 2: launch_fcc_colour<2>(G, st, s, kind); break;
+        default: launch_fcc_colour<3>(G, st, s, kind); break;
     }
 }
 
@@ -415,7 +421,9 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                 else if (g == kIsingBccBits)
                     launch_bcc_pass<true>(G, st, c, s);
                 else if (g == kIsingFcc)
-                    launch_fcc_pass(G, st, c, s, true);
+                    launch_fcc_pass(G, st, c, s, 0);
+                else if (g == kIsingFccBits)
+                    launch_fcc_pass(G, st, c, s, 1);
                 else if (g == kHeisSc)
                     hipLaunchKernelGGL(vg::heis_sc_pass, grid, block, 0, st, G.d_chains,
                                        G.d_block_chain, (float*)G.arena, (int64_t*)G.partials,
@@ -425,7 +433,7 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                                        G.d_block_chain, (float*)G.arena, (int64_t*)G.partials,
                                        c, s, G.total_blocks, G.grid);
                 else
-                    launch_fcc_pass(G, st, c, s, false);
+                    launch_fcc_pass(G, st, c, s, 2);
                 if (timed) {
                     VG_HIP(hipEventRecord(B->pass_events[B->pass_events_used + 1], st));
                     B->pass_events_used += 2;
@@ -929,7 +937,7 @@ int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
     hipError_t e = hipMemcpyAsync(tmp, spins, ch.n_sites, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
         if (is_bits(ch.group)) {
-            const uint64_t nb = 2ull * G.host[ch.index].plane;
+            const uint64_t nb = (uint64_t)G.host[ch.index].ncol * G.host[ch.index].plane;
             hipLaunchKernelGGL(vg::ising_scatter_bits, dim3((uint32_t)((nb + 255) / 256)),
                                dim3(256), 0, st, G.d_chains + ch.index, (uint8_t*)G.arena, tmp);
         } else {
@@ -1109,7 +1117,7 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
     if (tile != 1 && tile != 2 && tile != 4 && tile != 8)
         return fail(VG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8 (got %u)", tile);
     VG_HIP(hipSetDevice(batch->ctx->device));
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc, kIsingFccBits}) {
         if (batch->groups[g].members.empty()) continue;
         int rc = layout_work(batch, g, tile);
         if (rc) return rc;
@@ -1118,10 +1126,10 @@ int vg_batch_set_tile(vg_batch* batch, uint32_t tile) {
 }
 
 // The group whose tile vg_batch_get_tile and vg_batch_autotune report: the
-// first of sc int8, sc bit-packed, bcc int8, bcc bit-packed and fcc that has
-// chains.
+// first of the z-marching Ising groups (sc, bcc, fcc; int8, then bit-packed)
+// that has chains.
 static int tile_group(const vg_batch* batch) {
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc})
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc, kIsingFccBits})
         if (!batch->groups[g].members.empty()) return g;
     return kIsingSc;
 }
@@ -1137,7 +1145,7 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     uint32_t tile_now = 0;
     vg_batch_get_tile(batch, &tile_now);
     bool any = false;
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc})
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc, kIsingFccBits})
         any = any || !batch->groups[g].members.empty();
     if (!any || trial_sweeps == 0) {
         if (chosen) *chosen = tile_now;
@@ -1178,7 +1186,7 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     int rc = VG_OK;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(VG_ERR_DEVICE, "autotune backup: %s", hipGetErrorString(e));
-    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc}) {
+    for (int g : {kIsingSc, kIsingBits, kIsingBcc, kIsingBccBits, kIsingFcc, kIsingFccBits}) {
         Group& G = batch->groups[g];
         if (G.members.empty() || rc != VG_OK) continue;
         const uint32_t start_tile = G.tile;

@@ -40,7 +40,7 @@ __global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t,
                             int64_t*);
 // fcc passes: one instantiation per colour A (sublattice), args (chains,
 // block -> chain map, arena, partials, sweep, total blocks, grid).
-template <uint32_t A, uint32_t K>
+template <uint32_t A, uint32_t K, bool BITS>
 __global__ void ising_fcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
                                uint32_t, uint32_t);
 template <uint32_t A>
@@ -89,24 +89,28 @@ enum GroupKind {
     kHeisFcc = 6,
     kIsingWolff = 7,  // Wolff cluster steps, any lattice, natural site order
     kIsingBccBits = 8,  // bcc, 1 bit per site
-    kNumGroups = 9
+    kIsingFccBits = 9,  // fcc, 1 bit per site
+    kNumGroups = 10
 };
 
 inline bool is_ising(int g) {
     return g == kIsingSc || g == kIsingBcc || g == kIsingBits || g == kIsingFcc ||
-           g == kIsingWolff || g == kIsingBccBits;
+           g == kIsingWolff || g == kIsingBccBits || g == kIsingFccBits;
 }
 // Groups whose planes hold one bit per site.
-inline bool is_bits(int g) { return g == kIsingBits || g == kIsingBccBits; }
+inline bool is_bits(int g) {
+    return g == kIsingBits || g == kIsingBccBits || g == kIsingFccBits;
+}
 // Groups whose pass marches a tile of z planes per work item.
 inline bool is_zmarch(int g) {
     return g == kIsingSc || g == kIsingBits || g == kIsingBcc || g == kIsingFcc ||
-           g == kIsingBccBits;
+           g == kIsingBccBits || g == kIsingFccBits;
 }
 // Colours (planes, passes per sweep) of a group's chains; Wolff chains
 // have no colour passes.
 inline uint32_t colours_of(int g) {
-    return g == kIsingWolff ? 0u : (g == kIsingFcc || g == kHeisFcc ? 4u : 2u);
+    return g == kIsingWolff ? 0u
+                            : (g == kIsingFcc || g == kHeisFcc || g == kIsingFccBits ? 4u : 2u);
 }
 
 inline int group_of(const vg_chain_desc& d) {
@@ -114,7 +118,8 @@ inline int group_of(const vg_chain_desc& d) {
     if (d.proposal == VG_PROPOSAL_WOLFF) return kIsingWolff;
     if (d.spin == VG_SPIN_HEISENBERG)
         return k == VG_LATTICE_FCC ? kHeisFcc : (k == VG_LATTICE_BCC ? kHeisBcc : kHeisSc);
-    if (d.spin == VG_SPIN_ISING_BITS) return k == VG_LATTICE_BCC ? kIsingBccBits : kIsingBits;
+    if (d.spin == VG_SPIN_ISING_BITS)
+        return k == VG_LATTICE_FCC ? kIsingFccBits : (k == VG_LATTICE_BCC ? kIsingBccBits : kIsingBits);
     return k == VG_LATTICE_FCC ? kIsingFcc : (k == VG_LATTICE_BCC ? kIsingBcc : kIsingSc);
 }
 

@@ -25,7 +25,8 @@
 // high-byte tie; a row with ties leaves a record in the wave's LDS queue,
 // settled once the wave has finished its planes.  The pass of colour a adds
 // the bonds to the lower colours b < a, so over a sweep every bond counts
-// once.
+// once.  BITS: the planes hold one bit per site (ising_swar.hpp's storage
+// forms); a segment is a 16-bit load expanded to the same 16 bytes.
 #include "fcc_common.hpp"
 #include "ising_swar.hpp"
 
@@ -37,7 +38,7 @@ constexpr uint32_t kTieRows = 64;  // tie records per wave (one per row with tie
 constexpr uint32_t kTieWords = 14;
 }  // namespace
 
-template <uint32_t A, uint32_t K>
+template <uint32_t A, uint32_t K, bool BITS>
 __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict__ chains,
                                                       const uint32_t* __restrict__ block_chain,
                                                       uint8_t* __restrict__ arena,
@@ -78,6 +79,7 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
         const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
         const uint32_t zstep = ly * pitch;
         const uint32_t i0 = seg * kIsingSeg;
+        const uint32_t s0 = seg_off<BITS>(i0);  // the segment's byte offset in a row
         const uint32_t nvalid = min(kIsingSeg, w - i0);
         const uint32_t pw = w - i0;  // positions left in the row (1..16 in the last segment)
         const uint32_t vbits = nvalid >= kIsingSeg ? 0xFFFFu : (1u << nvalid) - 1u;
@@ -156,24 +158,24 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
             }
             const uint32_t r0 = (y + ly * zl) * pitch;
             if (diff & 2u) {  // y, z
-                const u32x4 a = bload16(ro, lo_ok[ob] ? r0 + i0 : kNoRow);
-                const u32x4 c =
-                    bload16(ro, lo_ok[ob] && yok[ob] ? (y2[ob] + ly * zl) * pitch + i0 : kNoRow);
+                const u32x4 a = load_seg<BITS>(ro, lo_ok[ob] ? r0 + s0 : kNoRow);
+                const u32x4 c = load_seg<BITS>(
+                    ro, lo_ok[ob] && yok[ob] ? (y2[ob] + ly * zl) * pitch + s0 : kNoRow);
                 lo_q[ob] = a + c;
                 lo_e[ob] = 0;
             } else {  // x, z
-                lo_q[ob] = bload16(ro, lo_ok[ob] ? r0 + i0 : kNoRow);
-                lo_e[ob] = bload8(ro, lo_ok[ob] && xe_ok ? r0 + xe : kNoRow);
+                lo_q[ob] = load_seg<BITS>(ro, lo_ok[ob] ? r0 + s0 : kNoRow);
+                lo_e[ob] = load_pos<BITS>(ro, r0, xe, lo_ok[ob] && xe_ok);
             }
         }
 
-        uint32_t off = (y + ly * z0) * pitch + i0;  // own row (y, z), this segment
+        uint32_t off = (y + ly * z0) * pitch + s0;  // own row (y, z), this segment
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             if (k >= nk) break;
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
-            const u32x4 o4 = bload16(rw_own, off);
+            const u32x4 o4 = load_seg<BITS>(rw_own, off);
             uint32_t cnt[4] = {0u, 0u, 0u, 0u}, low[4] = {0u, 0u, 0u, 0u};
             uint32_t pres = 0, pres_low = 0, miss = 0, miss_low = 0;
 #pragma unroll
@@ -196,9 +198,9 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                     }
                     const uint32_t r0 = (y + ly * zu) * pitch;
                     if (diff & 2u) {  // y, z: four rows at x
-                        const u32x4 a = bload16(ro, hi_ok ? r0 + i0 : kNoRow);
-                        const u32x4 c = bload16(
-                            ro, hi_ok && yok[ob] ? (y2[ob] + ly * zu) * pitch + i0 : kNoRow);
+                        const u32x4 a = load_seg<BITS>(ro, hi_ok ? r0 + s0 : kNoRow);
+                        const u32x4 c = load_seg<BITS>(
+                            ro, hi_ok && yok[ob] ? (y2[ob] + ly * zu) * pitch + s0 : kNoRow);
                         const u32x4 hq = a + c;
                         const u32x4 P = lo_q[ob] + hq;
                         c4[0] = P.x;
@@ -209,8 +211,8 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                              (hi_ok ? 1u : 0u) + (hi_ok && yok[ob] ? 1u : 0u);
                         lo_q[ob] = hq;
                     } else {  // x, z: two rows, each at x and at the x offset
-                        const u32x4 hq = bload16(ro, hi_ok ? r0 + i0 : kNoRow);
-                        const uint32_t he = bload8(ro, hi_ok && xe_ok ? r0 + xe : kNoRow);
+                        const u32x4 hq = load_seg<BITS>(ro, hi_ok ? r0 + s0 : kNoRow);
+                        const uint32_t he = load_pos<BITS>(ro, r0, xe, hi_ok && xe_ok);
                         xshift(lo_q[ob] + hq, lo_e[ob] + he, c4);
                         const uint32_t nr = (lo_ok[ob] ? 1u : 0u) + (hi_ok ? 1u : 0u);
                         pr = 2u * nr;
@@ -222,10 +224,10 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                 } else {
                     // x, y: rows (y, z) and (y', z), each at x and at the x offset
                     const uint32_t r1 = (y2[ob] + ly * z) * pitch;
-                    const u32x4 v0 = bload16(ro, off);
-                    const u32x4 v1 = bload16(ro, yok[ob] ? r1 + i0 : kNoRow);
-                    const uint32_t e = bload8(ro, xe_ok ? row * pitch + xe : kNoRow) +
-                                       bload8(ro, xe_ok && yok[ob] ? r1 + xe : kNoRow);
+                    const u32x4 v0 = load_seg<BITS>(ro, off);
+                    const u32x4 v1 = load_seg<BITS>(ro, yok[ob] ? r1 + s0 : kNoRow);
+                    const uint32_t e = load_pos<BITS>(ro, row * pitch, xe, xe_ok) +
+                                       load_pos<BITS>(ro, r1, xe, xe_ok && yok[ob]);
                     xshift(v0 + v1, e, c4);
                     const uint32_t nr = yok[ob] ? 2u : 1u;
                     pr = 2u * nr;
@@ -280,12 +282,17 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                 n[kk] = (n[kk] & ~(1u << sh)) | ((bj ^ f) << sh);
             }
             if ((n[0] ^ o[0]) | (n[1] ^ o[1]) | (n[2] ^ o[2]) | (n[3] ^ o[3])) {
-                u32x4 nv;
-                nv.x = n[0];
-                nv.y = n[1];
-                nv.z = n[2];
-                nv.w = n[3];
-                __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+                if (BITS) {
+                    __builtin_amdgcn_raw_buffer_store_b16(
+                        (unsigned short)pack16(n[0], n[1], n[2], n[3]), rw_own, off, 0, 0);
+                } else {
+                    u32x4 nv;
+                    nv.x = n[0];
+                    nv.y = n[1];
+                    nv.z = n[2];
+                    nv.w = n[3];
+                    __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+                }
             }
             uint32_t nl = 0, ls = 0;
             int32_t upc = 0;
@@ -327,7 +334,7 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                         *reinterpret_cast<uint4*>(rec + 10) = make_uint4(low[0], low[1], low[2], low[3]);
                     } else {
                         // Queue full (practically never): resolve in this lane.
-                        uint32_t tm = tmask;
+                        uint32_t tm = tmask, flips = 0;
                         while (tm) {
                             const uint32_t j = __builtin_ctz(tm);
                             tm &= tm - 1u;
@@ -337,7 +344,8 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                                 2u * ((cnt[kk] >> sh) & 0xFFu) + (j == gx ? a_edge : a_in);
                             if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t, A,
                                            k0, k1)) {
-                                own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                if (!BITS) own_plane[off + j] = (uint8_t)(bj ^ 1u);
+                                flips |= 1u << j;
                                 const int32_t ds = 2 - 4 * (int32_t)bj;
                                 m_part += ds;
                                 if (A > 0) {
@@ -347,6 +355,10 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                                 }
                             }
                         }
+                        if (BITS && flips)
+                            __builtin_amdgcn_raw_buffer_store_b16(
+                                (unsigned short)(pack16(n[0], n[1], n[2], n[3]) ^ flips), rw_own,
+                                off, 0, 0);
                     }
                 }
                 nrec += __builtin_popcountll(has);
@@ -378,7 +390,7 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
             const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
             const uint32_t zz = (r2 >> 1) * K + k;
             const uint32_t row = yy + ly * zz;
-            uint32_t tm = rec[1];
+            uint32_t tm = rec[1], flips = 0;
             while (tm) {
                 const uint32_t j = __builtin_ctz(tm);
                 tm &= tm - 1u;
@@ -388,7 +400,8 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                 const uint32_t S = 2u * c + (j == gx ? a_edge : a_in);
                 const uint32_t pos = sg * kIsingSeg + j;
                 if (tie_accept(lut, bj ? always1 : always0, bj, S, pos, row, t, A, k0, k1)) {
-                    own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    if (!BITS) own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
+                    flips |= 1u << j;
                     const int32_t ds = 2 - 4 * (int32_t)bj;
                     m_part += ds;
                     if (A > 0) {
@@ -398,6 +411,12 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                     }
                 }
             }
+            // Bit-packed: the owner stored the row's 16 bits with the tied
+            // sites unchanged; this lane rewrites them with the accepted ones.
+            if (BITS && flips)
+                __builtin_amdgcn_raw_buffer_store_b16(
+                    (unsigned short)(pack16(rec[6], rec[7], rec[8], rec[9]) ^ flips), rw_own,
+                    row * pitch + seg_off<BITS>(sg * kIsingSeg), 0, 0);
         }
     }
 
@@ -411,9 +430,13 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
     }
 }
 
-#define VG_FCC_PASS(A, K)                                                                   \
-    template __global__ void ising_fcc_pass<A, K>(const ChainDev*, const uint32_t*, uint8_t*, \
-                                                  int32_t*, uint32_t, uint32_t, uint32_t);
+#define VG_FCC_PASS(A, K)                                                                    \
+    template __global__ void ising_fcc_pass<A, K, false>(const ChainDev*, const uint32_t*,     \
+                                                         uint8_t*, int32_t*, uint32_t, uint32_t, \
+                                                         uint32_t);                              \
+    template __global__ void ising_fcc_pass<A, K, true>(const ChainDev*, const uint32_t*,      \
+                                                        uint8_t*, int32_t*, uint32_t, uint32_t,  \
+                                                        uint32_t);
 VG_FCC_PASS(0, 1)
 VG_FCC_PASS(0, 2)
 VG_FCC_PASS(0, 4)

@@ -75,10 +75,12 @@ __global__ void ising_gather(const ChainDev* __restrict__ chain, const uint8_t* 
     natural[n] = arena[C.off[p.c] + (uint64_t)p.row * C.pitch + p.i] ? 1 : -1;
 }
 
-// Bit-packed Ising planes (sc): position i of a row is bit (i & 7) of byte
-// i >> 3.  One thread per plane byte; bits past w stay zero.
+// Bit-packed Ising planes: position i of a row is bit (i & 7) of byte i >> 3.
+// One thread per plane byte; bits past w stay zero.
+
 // Natural site id of position i of row `row` of colour c: sc x + lx*(y +
-// ly*z) with x = 2i + ((c+y+z)&1); bcc 2*(cell) + c with the cell's x = i.
+// ly*z) with x = 2i + ((c+y+z)&1); bcc and fcc ncol*(cell) + c with the
+// cell's x = i.
 __device__ __forceinline__ uint64_t plane_site(const ChainDev& C, uint32_t c, uint32_t row,
                                                uint32_t i) {
     const uint32_t y = row % C.ly, z = row / C.ly;
@@ -86,7 +88,8 @@ __device__ __forceinline__ uint64_t plane_site(const ChainDev& C, uint32_t c, ui
         const uint32_t x = 2u * i + ((c + y + z) & 1u);
         return (uint64_t)x + (uint64_t)C.lx * ((uint64_t)y + (uint64_t)C.ly * z);
     }
-    return 2u * ((uint64_t)i + (uint64_t)C.lx * ((uint64_t)y + (uint64_t)C.ly * z)) + c;
+    return (uint64_t)C.ncol * ((uint64_t)i + (uint64_t)C.lx * ((uint64_t)y + (uint64_t)C.ly * z)) +
+           c;
 }
 
 // The same draws as ising_init: position i is Up iff word (i & 3) of
@@ -95,8 +98,8 @@ __global__ void ising_init_bits(const ChainDev* __restrict__ chain, uint8_t* __r
     const ChainDev& C = *chain;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t per_plane = (uint64_t)C.rows * C.pitch;
-    if (idx >= 2 * per_plane) return;
-    const uint32_t c = idx >= per_plane ? 1u : 0u;
+    if (idx >= C.ncol * per_plane) return;
+    const uint32_t c = (uint32_t)(idx / per_plane);
     const uint64_t e = idx - c * per_plane;
     const uint32_t row = (uint32_t)(e / C.pitch);
     const uint32_t i0 = 8u * (uint32_t)(e % C.pitch);
@@ -119,8 +122,8 @@ __global__ void ising_scatter_bits(const ChainDev* __restrict__ chain, uint8_t* 
     const ChainDev& C = *chain;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t per_plane = (uint64_t)C.rows * C.pitch;
-    if (idx >= 2 * per_plane) return;
-    const uint32_t c = idx >= per_plane ? 1u : 0u;
+    if (idx >= C.ncol * per_plane) return;
+    const uint32_t c = (uint32_t)(idx / per_plane);
     const uint64_t e = idx - c * per_plane;
     const uint32_t row = (uint32_t)(e / C.pitch);
     const uint32_t i0 = 8u * (uint32_t)(e % C.pitch);
