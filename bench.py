@@ -621,7 +621,7 @@ def main():
     ap.add_argument("--side", type=int, default=256, help="lattice side (256 for the headline)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="plain launches instead of replaying sweep blocks from a hipGraph")
-    ap.add_argument("--cpu-sweeps", type=int, default=3,
+    ap.add_argument("--cpu-sweeps", type=int, default=5,
                     help="timed sweeps of the CPU baseline (after one warm-up sweep)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-sweeps", type=int, default=100)
