@@ -20,7 +20,7 @@
 namespace vg {
 template <uint32_t K, bool BITS>
 __global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
-                              uint32_t, uint32_t, uint32_t);
+                              uint32_t, uint32_t, uint32_t, uint32_t);
 __global__ void ising_init_bits(const ChainDev*, uint8_t*);
 __global__ void ising_scatter_bits(const ChainDev*, uint8_t*, const int8_t*);
 __global__ void ising_gather_bits(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
@@ -140,6 +140,7 @@ struct Group {
     uint32_t tile = 4;  // Ising sc/bcc/fcc (int8, bits): z planes per work item (template parameter)
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
+    uint32_t prio = 0;  // sc Ising passes: every workgroup resident at once (wave priorities on)
     void* arena = nullptr;
     uint64_t arena_bytes = 0;
     void* partials = nullptr;  // [sweep][colour][block][quantity]

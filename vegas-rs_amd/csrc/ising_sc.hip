@@ -47,7 +47,8 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                      uint8_t* __restrict__ arena,
                                                      int32_t* __restrict__ partials,
                                                      uint32_t colour, uint32_t s,
-                                                     uint32_t total_blocks, uint32_t grid) {
+                                                     uint32_t total_blocks, uint32_t grid,
+                                                     uint32_t prio) {
     __shared__ uint32_t lut[2 * kThrS];
     __shared__ uint32_t tab[20];  // t8, [g][b][2]
     // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4]}
@@ -117,6 +118,12 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             if (k >= nk) break;
+            // The SIMD arbiter favours older waves; when every wave of the
+            // pass is resident at once, a wave's priority falls as it
+            // marches, so the waves of a SIMD progress together and the pass
+            // does not end in a tail of a few young waves.  (With a second
+            // round of waves, staggered ends let it start early instead.)
+            if (prio) set_prio(((K - 1u - k) * 4u) / K);
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
             const bool zp_ok = z + 1 < lz || pz;
@@ -374,28 +381,28 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
 
 template __global__ void ising_sc_pass<1, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 template __global__ void ising_sc_pass<2, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 template __global__ void ising_sc_pass<4, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 template __global__ void ising_sc_pass<8, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 template __global__ void ising_sc_pass<1, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 template __global__ void ising_sc_pass<2, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 template __global__ void ising_sc_pass<4, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 template __global__ void ising_sc_pass<8, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t);
+                                                  uint32_t, uint32_t);
 
 // Sums the pass partials of each (chain, sweep) over its ncol colour passes
 // into its record: record = {bond, 0, 0, mag, 0} (int64).  A pass adds the

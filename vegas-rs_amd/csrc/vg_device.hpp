@@ -229,6 +229,17 @@ __device__ __forceinline__ int32_t wave_sum(int32_t v) {
     return v;
 }
 
+// Issue priority of the calling wave, 0 (lowest) .. 3; p must fold to a
+// constant.
+__device__ __forceinline__ void set_prio(uint32_t p) {
+    switch (p) {
+        case 0: __builtin_amdgcn_s_setprio(0); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        default: __builtin_amdgcn_s_setprio(3); break;
+    }
+}
+
 // Maximum over the 64 lanes of a wave (every lane must call it).
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 #pragma unroll
