@@ -240,17 +240,25 @@ int upload_descriptors(vg_batch* B, int g, int slot) {
 
 constexpr uint32_t kDefaultTile = 4;
 
-// Whether `grid` workgroups of the sc Ising pass with this tile are all
-// resident at once on the current device.
-bool all_resident(int g, uint32_t tile, uint32_t grid) {
-    const bool bits = g == kIsingBits;
-    const void* fn;
-    switch (tile) {
-        case 1: fn = bits ? (const void*)&vg::ising_sc_pass<1, true> : (const void*)&vg::ising_sc_pass<1, false>; break;
-        case 2: fn = bits ? (const void*)&vg::ising_sc_pass<2, true> : (const void*)&vg::ising_sc_pass<2, false>; break;
-        case 8: fn = bits ? (const void*)&vg::ising_sc_pass<8, true> : (const void*)&vg::ising_sc_pass<8, false>; break;
-        default: fn = bits ? (const void*)&vg::ising_sc_pass<4, true> : (const void*)&vg::ising_sc_pass<4, false>; break;
+template <uint32_t K>
+const void* zmarch_fn(int g) {
+    switch (g) {
+        case kIsingBits: return (const void*)&vg::ising_sc_pass<K, true>;
+        case kIsingBcc: return (const void*)&vg::ising_bcc_pass<K, false>;
+        case kIsingBccBits: return (const void*)&vg::ising_bcc_pass<K, true>;
+        case kIsingFcc: return (const void*)&vg::ising_fcc_pass<0, K, false>;
+        case kIsingFccBits: return (const void*)&vg::ising_fcc_pass<0, K, true>;
+        default: return (const void*)&vg::ising_sc_pass<K, false>;
     }
+}
+
+// Whether `grid` workgroups of group g's z-marching Ising pass with this tile
+// are all resident at once on the current device.
+bool all_resident(int g, uint32_t tile, uint32_t grid) {
+    const void* fn = tile == 1   ? zmarch_fn<1>(g)
+                     : tile == 2 ? zmarch_fn<2>(g)
+                     : tile == 8 ? zmarch_fn<8>(g)
+                                 : zmarch_fn<4>(g);
     int nb = 0, dev = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, vg::kBlock, 0) != hipSuccess ||
         hipGetDevice(&dev) != hipSuccess ||
@@ -295,7 +303,7 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
     G.tile = tile;
     G.total_blocks = (uint32_t)block_chain.size();
     G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
-    if (g == kIsingSc || g == kIsingBits) G.prio = all_resident(g, tile, G.grid) ? 1u : 0u;
+    if (is_zmarch(g)) G.prio = all_resident(g, tile, G.grid) ? 1u : 0u;
     G.partial_bytes_per_sweep = std::max<uint64_t>(
         256, (uint64_t)colours_of(g) * G.total_blocks * quantities(g) * quantity_bytes(g));
     for (auto& kv : B->graphs) quiet(hipGraphExecDestroy(kv.second));
@@ -361,19 +369,23 @@ void launch_bcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     switch (G.tile) {
         case 1:
             hipLaunchKernelGGL((vg::ising_bcc_pass<1, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
+                               G.prio);
             break;
         case 2:
             hipLaunchKernelGGL((vg::ising_bcc_pass<2, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
+                               G.prio);
             break;
         case 8:
             hipLaunchKernelGGL((vg::ising_bcc_pass<8, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
+                               G.prio);
             break;
         default:
             hipLaunchKernelGGL((vg::ising_bcc_pass<4, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid);
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
+                               G.prio);
             break;
     }
 }
@@ -382,7 +394,7 @@ template <uint32_t A, uint32_t K, bool BITS>
 void launch_ising_fcc(const Group& G, hipStream_t st, uint32_t s) {
     hipLaunchKernelGGL((vg::ising_fcc_pass<A, K, BITS>), dim3(G.grid), dim3(vg::kBlock), 0, st,
                        G.d_chains, G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials, s,
-                       G.total_blocks, G.grid);
+                       G.total_blocks, G.grid, G.prio);
 }
 
 template <uint32_t A, bool BITS>

@@ -33,16 +33,17 @@ __global__ void heis_sc_pass(const ChainDev*, const uint32_t*, float*, int64_t*,
                              uint32_t, uint32_t, uint32_t);
 template <uint32_t K, bool BITS>
 __global__ void ising_bcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
-                               uint32_t, uint32_t, uint32_t);
+                               uint32_t, uint32_t, uint32_t, uint32_t);
 __global__ void heis_bcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                               uint32_t, uint32_t, uint32_t);
 __global__ void heis_reduce(const ChainDev*, const int64_t*, uint32_t, uint32_t, uint32_t,
                             int64_t*);
 // fcc passes: one instantiation per colour A (sublattice), args (chains,
-// block -> chain map, arena, partials, sweep, total blocks, grid).
+// block -> chain map, arena, partials, sweep, total blocks, grid[, wave
+// priorities on]).
 template <uint32_t A, uint32_t K, bool BITS>
 __global__ void ising_fcc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
-                               uint32_t, uint32_t);
+                               uint32_t, uint32_t, uint32_t);
 template <uint32_t A>
 __global__ void heis_fcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                               uint32_t, uint32_t);
@@ -140,7 +141,7 @@ struct Group {
     uint32_t tile = 4;  // Ising sc/bcc/fcc (int8, bits): z planes per work item (template parameter)
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
-    uint32_t prio = 0;  // sc Ising passes: every workgroup resident at once (wave priorities on)
+    uint32_t prio = 0;  // z-marching Ising passes: every workgroup resident at once (wave priorities on)
     void* arena = nullptr;
     uint64_t arena_bytes = 0;
     void* partials = nullptr;  // [sweep][colour][block][quantity]

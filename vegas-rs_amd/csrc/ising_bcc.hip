@@ -42,7 +42,8 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
                                                       uint8_t* __restrict__ arena,
                                                       int32_t* __restrict__ partials,
                                                       uint32_t colour, uint32_t s,
-                                                      uint32_t total_blocks, uint32_t grid) {
+                                                      uint32_t total_blocks, uint32_t grid,
+                                                      uint32_t prio) {
     __shared__ uint32_t lut[2 * kThrS];
     __shared__ uint32_t tab[9 * 8];  // class tables for 0..8 absent slots
     // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4]}
@@ -140,6 +141,8 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             if (k >= nk) break;
+            // Falling priority while every wave is resident (as in the sc pass).
+            if (prio) set_prio(((K - 1u - k) * 4u) / K);
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
             // the upper pair: zu = z (colour 0) or z+1 (colour 1)
@@ -366,27 +369,27 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
 
 template __global__ void ising_bcc_pass<1, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 template __global__ void ising_bcc_pass<2, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 template __global__ void ising_bcc_pass<4, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 template __global__ void ising_bcc_pass<8, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 template __global__ void ising_bcc_pass<1, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 template __global__ void ising_bcc_pass<2, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 template __global__ void ising_bcc_pass<4, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 template __global__ void ising_bcc_pass<8, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                      int32_t*, uint32_t, uint32_t, uint32_t,
-                                                     uint32_t);
+                                                     uint32_t, uint32_t);
 
 }  // namespace vg

@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                                                       const uint32_t* __restrict__ block_chain,
                                                       uint8_t* __restrict__ arena,
                                                       int32_t* __restrict__ partials, uint32_t s,
-                                                      uint32_t total_blocks, uint32_t grid) {
+                                                      uint32_t total_blocks, uint32_t grid,
+                                                      uint32_t prio) {
     __shared__ uint32_t lut[2 * kThrS];
     __shared__ uint32_t tab[13 * 8];  // class tables for 0..12 absent slots
     // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4], low[4]}
@@ -173,6 +174,8 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             if (k >= nk) break;
+            // Falling priority while every wave is resident (as in the sc pass).
+            if (prio) set_prio(((K - 1u - k) * 4u) / K);
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
             const u32x4 o4 = load_seg<BITS>(rw_own, off);
@@ -433,10 +436,10 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
 #define VG_FCC_PASS(A, K)                                                                    \
     template __global__ void ising_fcc_pass<A, K, false>(const ChainDev*, const uint32_t*,     \
                                                          uint8_t*, int32_t*, uint32_t, uint32_t, \
-                                                         uint32_t);                              \
+                                                         uint32_t, uint32_t);                    \
     template __global__ void ising_fcc_pass<A, K, true>(const ChainDev*, const uint32_t*,      \
                                                         uint8_t*, int32_t*, uint32_t, uint32_t,  \
-                                                        uint32_t);
+                                                        uint32_t, uint32_t);
 VG_FCC_PASS(0, 1)
 VG_FCC_PASS(0, 2)
 VG_FCC_PASS(0, 4)
