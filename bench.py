@@ -253,7 +253,7 @@ def run_bitpacked(vg, ctx, side, sweeps):
         b.randomize()
         for i, T in enumerate(temps):
             b.set_thermostat(i, T)
-        tile = b.autotune(10)
+        tile = b.autotune(20)
         b.set_tile(tile)
         b.set_graph(True)
         b.run(10)
@@ -394,7 +394,7 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
             b.randomize()
             for i, T in enumerate(temps):
                 b.set_thermostat(i, T, 0.0 if ising else 0.05)
-            b.autotune(5)  # the bcc Ising pass's z tile (a no-op for the others)
+            b.autotune(30)  # the Ising passes' z tile (a no-op for Heisenberg)
             b.set_graph(True)
             b.run(5)
             b.prepare(sweeps)
@@ -728,7 +728,7 @@ def main():
     batch.randomize()
     for i, T in enumerate(temps):
         batch.set_thermostat(i, T)
-    tile = batch.autotune(10) if args.tile == 0 else args.tile
+    tile = batch.autotune(20) if args.tile == 0 else args.tile
     batch.set_tile(tile)
     batch.set_graph(args.graph)
     log(f"bench: {len(temps)} chains sc {side}^3 int8, warmup {args.warmup}")
