@@ -218,6 +218,12 @@ int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile);
  * groups.  The chains' spins and sweep counters are
  * restored after every trial, so tuning does not change any trajectory. */
 int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen_tile);
+/* Records (0..64, default 64) the per-wave tie queue of the Ising sc, bcc
+ * and fcc passes may hold before a wave settles further tied rows in the
+ * lane that found them.  No counterpart in the reference; a test hook, so
+ * that the queue-overflow path can be checked against the oracle: results
+ * do not depend on it.  Drops captured graphs. */
+int vg_batch_set_tie_capacity(vg_batch* batch, uint32_t records);
 /* Last run's device time in milliseconds (hipEvent pair around the sweeps). */
 int vg_batch_last_run_ms(const vg_batch* batch, double* ms);
 /* Per-launch timing of the colour-pass kernels: when enabled (plain launches

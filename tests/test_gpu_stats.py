@@ -34,6 +34,10 @@ CASES = {
                       ("exchange", "zeeman", "anisotropy"), 0.1, 64, 4000, 30000),
     "heis_sc8_T2.5": ("heisenberg", (8, 8, 8), (1, 1, 1), 2.5, 0.05, "random",
                       ("exchange", "zeeman", "anisotropy"), 0.1, 64, 4000, 30000),
+    # the lowest temperature of the 128^3 configuration: ordered, easy-plane
+    # (K > 0 penalises the z axis in the reference's sign), low acceptance
+    "heis_sc8_T0.5": ("heisenberg", (8, 8, 8), (1, 1, 1), 0.5, 0.05, "random",
+                      ("exchange", "zeeman", "anisotropy"), 0.1, 64, 6000, 60000),
     "ising_bcc5_T8": ("ising", (5, 5, 5), (1, 1, 1), 8.0, 0.1, "flip",
                       ("exchange", "zeeman"), 0.0, 64, 6000, 60000),
     "ising_bcc6x4x3_open_T5": ("ising", (6, 4, 3), (1, 0, 0), 5.0, 0.0, "flip",

@@ -389,6 +389,38 @@ def test_checkerboard_heisenberg_single_spin_closed_forms(orc):
     assert z_score(mean, err, num / den, 0.0) < 4
 
 
+def test_heisenberg_exchange_dimer_closed_form(orc):
+    """Two unit spins joined by one exchange bond (sc 2x1x1, open axes):
+    with H = -J s1.s2 the bond's distribution is p(c) ~ exp(J c / T) on
+    c = s1.s2 in [-1, 1], so <s1.s2> = coth(J/T) - T/J (Langevin L(J/T)).
+    Both the checkerboard restatement (bond sum of the fixed-point sums) and
+    the sequential restatement of the reference (its Exchange-only
+    total_energy is -J s1.s2) must reproduce it; a flipped sign (-L), a
+    doubled coupling (L(2J/T)) or a missing bond (0) fail by many sigma."""
+    L = orc.lattice("sc", 2, 1, 1, pbc=(0, 0, 0))
+    assert orc.neighbours(L, 0) == [-1, 1, -1, -1, -1, -1]
+    for J, T in [(1.0, 2.0), (1.0, 1.0), (1.0, 0.5), (-0.7, 0.8)]:
+        want = _langevin(J / T)
+        M = orc.model(spin="heisenberg", proposal="random", terms=("exchange",), exchange=J,
+                      temperature=T)
+        xyz = orc.heis_init(L, 7)
+        sums = orc.heis_sweeps(L, M, 8, 0, 200000, xyz)[200:]
+        c = sums[:, 0] / 2 ** 27
+        mean, err = binned(c)
+        assert err < 0.01
+        assert z_score(mean, err, want, 0.0) < 4, (J, T, mean, err, want)
+        for wrong in (-want, _langevin(2 * J / T), 0.0):
+            assert z_score(mean, err, wrong, 0.0) > 10, (J, T, wrong)
+        seq = orc.Sequential(L, M, 9)
+        seq.set_thermostat(T)
+        seq.set_state(orc.heis_init(L, 9).astype(np.float64))
+        seq.sweeps(200, observe=False)
+        e, _ = seq.sweeps(200000)
+        seq.close()
+        mean, err = binned(-e / J)
+        assert z_score(mean, err, want, 0.0) < 4, (J, T, mean, err, want)
+
+
 def test_checkerboard_ising_zeeman_only_closed_form(orc):
     L = orc.lattice("sc", 8, 8, 8)
     h, T = 0.5, 1.3

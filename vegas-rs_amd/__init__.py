@@ -131,6 +131,7 @@ _SIGS = {
     "vg_batch_set_tile": [_vp, C.c_uint32],
     "vg_batch_get_tile": [_vp, _P(C.c_uint32)],
     "vg_batch_autotune": [_vp, C.c_uint32, _P(C.c_uint32)],
+    "vg_batch_set_tie_capacity": [_vp, C.c_uint32],
     "vg_batch_pass_stats": [_vp, C.c_int32, _P(C.c_double), _P(C.c_uint64)],
     "vg_server_create": [_vp, _P(ServerConfig), _P(_vp)],
     "vg_server_destroy": [_vp],
@@ -333,6 +334,11 @@ class Batch:
         v = C.c_uint32()
         _check(_lib.vg_batch_autotune(self.h, trial_sweeps, C.byref(v)))
         return v.value
+
+    def set_tie_capacity(self, records: int) -> None:
+        """Test hook: tie records a wave may queue (0..64) before it settles
+        further tied rows in place; results do not depend on it."""
+        _check(_lib.vg_batch_set_tie_capacity(self.h, records))
 
     def set_timing(self, enabled: bool) -> None:
         _check(_lib.vg_batch_set_timing(self.h, 1 if enabled else 0))

@@ -341,22 +341,22 @@ void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
         case 1:
             hipLaunchKernelGGL((vg::ising_sc_pass<1, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
         case 2:
             hipLaunchKernelGGL((vg::ising_sc_pass<2, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
         case 8:
             hipLaunchKernelGGL((vg::ising_sc_pass<8, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
         default:
             hipLaunchKernelGGL((vg::ising_sc_pass<4, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
     }
 }
@@ -370,22 +370,22 @@ void launch_bcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
         case 1:
             hipLaunchKernelGGL((vg::ising_bcc_pass<1, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
         case 2:
             hipLaunchKernelGGL((vg::ising_bcc_pass<2, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
         case 8:
             hipLaunchKernelGGL((vg::ising_bcc_pass<8, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
         default:
             hipLaunchKernelGGL((vg::ising_bcc_pass<4, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio);
+                               G.prio | (G.tie_cap << 8));
             break;
     }
 }
@@ -394,7 +394,7 @@ template <uint32_t A, uint32_t K, bool BITS>
 void launch_ising_fcc(const Group& G, hipStream_t st, uint32_t s) {
     hipLaunchKernelGGL((vg::ising_fcc_pass<A, K, BITS>), dim3(G.grid), dim3(vg::kBlock), 0, st,
                        G.d_chains, G.d_block_chain, (uint8_t*)G.arena, (int32_t*)G.partials, s,
-                       G.total_blocks, G.grid, G.prio);
+                       G.total_blocks, G.grid, G.prio | (G.tie_cap << 8));
 }
 
 template <uint32_t A, bool BITS>
@@ -1210,13 +1210,13 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     std::vector<uint64_t> counters(batch->chains.size());
     for (size_t i = 0; i < batch->chains.size(); ++i) counters[i] = batch->chains[i].sweeps;
     auto restore = [&]() -> int {
+        for (size_t i = 0; i < batch->chains.size(); ++i) batch->chains[i].sweeps = counters[i];
         for (int g = 0; g < kNumGroups; ++g) {
             Group& G = batch->groups[g];
             if (G.members.empty()) continue;
             VG_HIP(hipMemcpyAsync(G.arena, backup[g], G.arena_bytes, hipMemcpyDeviceToDevice, st));
         }
         VG_HIP(hipStreamSynchronize(st));
-        for (size_t i = 0; i < batch->chains.size(); ++i) batch->chains[i].sweeps = counters[i];
         return VG_OK;
     };
     int rc = VG_OK;
@@ -1248,7 +1248,13 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
             }
         }
         if (rc != VG_OK) {
+            // A trial failed part way (graph capture, memory): put the spins
+            // and sweep counters back as they were, best effort, keeping the
+            // trial's error message for the caller.
+            const std::string msg = g_error;
+            restore();
             layout_work(batch, g, start_tile);
+            g_error = msg;
             break;
         }
         rc = layout_work(batch, g, best);
@@ -1257,6 +1263,17 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
     batch->use_graph = use_graph;
     if (rc == VG_OK && chosen) *chosen = batch->groups[tile_group(batch)].tile;
     return rc;
+}
+
+int vg_batch_set_tie_capacity(vg_batch* batch, uint32_t records) {
+    if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
+    if (records > 64)
+        return fail(VG_ERR_INVALID_ARGUMENT, "tie capacity must be 0..64 (got %u)", records);
+    for (Group& G : batch->groups) G.tie_cap = records;
+    // captured graphs hold the old kernel arguments
+    for (auto& kv : batch->graphs) quiet(hipGraphExecDestroy(kv.second));
+    batch->graphs.clear();
+    return VG_OK;
 }
 
 int vg_batch_last_run_ms(const vg_batch* batch, double* ms) {

@@ -142,6 +142,7 @@ struct Group {
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
     uint32_t prio = 0;  // z-marching Ising passes: every workgroup resident at once (wave priorities on)
+    uint32_t tie_cap = 64;  // z-marching Ising passes: tie records queued per wave (at most 64)
     void* arena = nullptr;
     uint64_t arena_bytes = 0;
     void* partials = nullptr;  // [sweep][colour][block][quantity]

@@ -44,11 +44,16 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                                                       uint8_t* __restrict__ arena,
                                                       int32_t* __restrict__ partials, uint32_t s,
                                                       uint32_t total_blocks, uint32_t grid,
-                                                      uint32_t prio) {
+                                                      uint32_t opts) {
     __shared__ uint32_t lut[2 * kThrS];
     __shared__ uint32_t tab[13 * 8];  // class tables for 0..12 absent slots
     // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4], low[4]}
     __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
+    // opts: bit 0 = falling wave priorities; bits 8-15 = records the tie
+    // queue may hold (at most kTieRows; fewer only to exercise the overflow
+    // path in tests -- results do not depend on it).
+    const uint32_t prio = opts & 1u;
+    const uint32_t qcap = min((opts >> 8) & 0xFFu, kTieRows);
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
     const ChainDev& C = chains[block_chain[b]];
@@ -327,7 +332,7 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                     const uint32_t slot =
                         nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
-                    if (slot < kTieRows) {
+                    if (slot < qcap) {
                         uint32_t* rec = q_wave[slot];
                         rec[0] = lane | (k << 6) | (a_in << 9) | (a_edge << 13) |
                                  (min(gx, 31u) << 17) | (pres_low << 22) | (miss_low << 26);
@@ -336,7 +341,10 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
                         *reinterpret_cast<uint4*>(rec + 6) = make_uint4(n[0], n[1], n[2], n[3]);
                         *reinterpret_cast<uint4*>(rec + 10) = make_uint4(low[0], low[1], low[2], low[3]);
                     } else {
-                        // Queue full (practically never): resolve in this lane.
+                        // Queue full: resolve in this lane.  (A wave marching 8
+                        // planes queues ~31 records on average against 64
+                        // slots, so this happens a few times per 10^4
+                        // sweeps of sc 256^3 x 3.)
                         uint32_t tm = tmask, flips = 0;
                         while (tm) {
                             const uint32_t j = __builtin_ctz(tm);
@@ -375,7 +383,7 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
     // The wave's own row stores must land first.  A lane stops counting
     // records when it stops marching (last z chunk) or never starts, so the
     // wave's maximum is the count.
-    const uint32_t nq = min(wave_max(nrec), kTieRows);
+    const uint32_t nq = min(wave_max(nrec), qcap);
     if (nq) {
         __builtin_amdgcn_s_waitcnt(0);
         const uint32_t ly = C.ly, pitch = C.pitch;

@@ -48,11 +48,16 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                      int32_t* __restrict__ partials,
                                                      uint32_t colour, uint32_t s,
                                                      uint32_t total_blocks, uint32_t grid,
-                                                     uint32_t prio) {
+                                                     uint32_t opts) {
     __shared__ uint32_t lut[2 * kThrS];
     __shared__ uint32_t tab[20];  // t8, [g][b][2]
     // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4]}
     __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
+    // opts: bit 0 = falling wave priorities; bits 8-15 = records the tie
+    // queue may hold (at most kTieRows; fewer only to exercise the overflow
+    // path in tests -- results do not depend on it).
+    const uint32_t prio = opts & 1u;
+    const uint32_t qcap = min((opts >> 8) & 0xFFu, kTieRows);
 
     const uint32_t b = xcd_block(blockIdx.x, grid);
     if (b >= total_blocks) return;  // uniform over the workgroup
@@ -277,7 +282,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                     const uint32_t slot =
                         nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
-                    if (slot < kTieRows) {
+                    if (slot < qcap) {
                         uint32_t* rec = q_wave[slot];
                         rec[0] = lane | (k << 6) | (g << 9) | (min(gx, 31u) << 12);
                         rec[1] = tmask;
@@ -286,7 +291,10 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                         // unchanged); packed, they are the row's stored bits
                         *reinterpret_cast<uint4*>(rec + 6) = make_uint4(n[0], n[1], n[2], n[3]);
                     } else {
-                        // Queue full (practically never): resolve in this lane.
+                        // Queue full: resolve in this lane.  (A wave marching 8
+                        // planes queues ~31 records on average against 64
+                        // slots, so this happens a few times per 10^4
+                        // sweeps of sc 256^3 x 3.)
                         uint32_t tm = tmask;
                         uint32_t flips = 0;
                         while (tm) {
@@ -327,7 +335,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     // or never started (past the chain's items, or the row y = ly of an odd
     // ly) stops counting: the lanes marching longest saw every record, so
     // the wave's maximum is the count, the same for every lane.
-    const uint32_t nq = min(wave_max(nrec), kTieRows);
+    const uint32_t nq = min(wave_max(nrec), qcap);
     if (nq) {
         __builtin_amdgcn_s_waitcnt(0);
         for (uint32_t e = lane; e < nq; e += 64) {

@@ -491,13 +491,19 @@ int validate_job(const vg_job_desc& j) {
     if (j.chain.proposal == VG_PROPOSAL_WOLFF)
         return fail(VG_ERR_NOT_IMPLEMENTED, "the job server runs Metropolis chains; Wolff chains run in batches");
     if (j.n_stages == 0 || !j.stages) return fail(VG_ERR_NO_STEPS, "a job needs at least one stage");
+    // The job's sweep counter keys Philox with 56 bits, as in a batch run.
+    const uint64_t kMaxSweeps = 1ull << 56;
+    uint64_t total = 0;
     for (uint32_t k = 0; k < j.n_stages; ++k) {
         const vg_stage& s = j.stages[k];
         if (s.relax + s.measure == 0) return fail(VG_ERR_NO_STEPS, "stage %u has no sweeps", k);
         if (std::isnan(s.temperature) || std::isnan(s.field))
             return fail(VG_ERR_INVALID_ARGUMENT, "stage %u: NaN thermostat parameter", k);
-        if (s.relax + s.measure < s.relax)
-            return fail(VG_ERR_INVALID_ARGUMENT, "stage %u: sweep count overflows", k);
+        if (s.relax > kMaxSweeps || s.measure > kMaxSweeps - s.relax ||
+            s.relax + s.measure > kMaxSweeps - total)
+            return fail(VG_ERR_INVALID_ARGUMENT,
+                        "stage %u: the job's sweeps would pass 2^56 (the sweep counter's range)", k);
+        total += s.relax + s.measure;
     }
     return VG_OK;
 }
