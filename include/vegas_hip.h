@@ -74,6 +74,10 @@ typedef enum vg_status {
 #define VG_LATTICE_SC 0
 #define VG_LATTICE_BCC 1
 #define VG_LATTICE_FCC 2
+/* An arbitrary exchange matrix (vg_csr below, vg_batch_create_csr): the  */
+/* chain's sites and bonds are the matrix's rows and nonzeros; lx, ly, lz */
+/* and pbc are not used.                                                  */
+#define VG_LATTICE_GRAPH 3
 
 #define VG_PBC_X 1u
 #define VG_PBC_Y 2u
@@ -139,6 +143,24 @@ typedef struct vg_chain_desc {
 typedef struct vg_ctx vg_ctx;
 typedef struct vg_batch vg_batch;
 
+/* Exchange::new(CsMat<f64>) (src/energy.rs:170-172): the exchange of a     */
+/* VG_LATTICE_GRAPH chain as a sparse matrix J in CSR form, row i holding  */
+/* the J_ij of site i, evaluated as the reference does (energy.rs:194-213):*/
+/*   e_i = -sum_j J_ij s_i.s_j over row i; Exchange's total_energy halves   */
+/*   sum_i e_i; a Compound adds the e_i of every term.                     */
+/* The couplings live in the matrix (vg_hamiltonian.exchange is not used; */
+/* the VG_TERM_EXCHANGE bit must be set).  Duplicate entries of a row are */
+/* summed; a diagonal entry J_ii adds the constant -J_ii to e_i.  The     */
+/* sweep visits the colour classes of a greedy colouring of the matrix's  */
+/* symmetrised pattern in turn, all sites of a class at once (sites of one */
+/* class share no nonzero); vg_batch_colours reports the class count.     */
+typedef struct vg_csr {
+    uint64_t n_sites;          /* rows = columns = sites, 1 .. 2^31 - 1 */
+    const uint64_t* row_ptr;   /* n_sites + 1 offsets, row_ptr[0] = 0 */
+    const uint64_t* col;       /* row_ptr[n_sites] column indices (< n_sites) */
+    const double* value;       /* J_ij, finite */
+} vg_csr;
+
 /* Per-sweep integer sums the kernels reduce (fixed point for Heisenberg,
  * scale vg_sums_scale()).  bond = sum over bonds of s_i.s_j counted once;
  * mag = sum_i s_i; axis2 = sum_i (s_i.axis)^2. */
@@ -167,6 +189,22 @@ void vg_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t
 /* ------------------------------------------------------------------ */
 int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs,
                     vg_batch** out);
+/* vg_batch_create for batches holding VG_LATTICE_GRAPH chains (Exchange::new
+ * with an arbitrary matrix, energy.rs:170-172): exchange[i] is the matrix of
+ * chain i when its lattice kind is VG_LATTICE_GRAPH and is ignored otherwise
+ * (lattice chains may share the batch).  Graph chains take int8 Ising
+ * (flip or random proposals) or Heisenberg spins; the matrix is copied. */
+int vg_batch_create_csr(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs,
+                        const vg_csr* exchange, vg_batch** out);
+/* Colour classes (passes per sweep) of a chain: 2 for sc and bcc, 4 for fcc,
+ * the greedy colouring's count for a graph chain. */
+int vg_batch_colours(const vg_batch* batch, uint32_t chain, uint32_t* colours);
+/* Scale of each of the five vg_sweep_sums quantities of a chain (value =
+ * integer * scale).  Lattice chains: vg_sums_scale(spin) for all five.
+ * Graph chains: bond = sum over the off-diagonal nonzeros of J_ij s_i.s_j
+ * (the couplings included) at 2^-27; mag and axis2 as for lattice chains of
+ * the same spin kind. */
+int vg_batch_sums_scales(const vg_batch* batch, uint32_t chain, double scales[5]);
 int vg_batch_destroy(vg_batch* batch);
 int vg_batch_size(const vg_batch* batch, uint32_t* n_chains);
 int vg_batch_num_sites(const vg_batch* batch, uint32_t chain, uint64_t* n_sites);

@@ -975,3 +975,310 @@ void or_seq_sweeps(or_seq* s, uint64_t nsweeps, double* energy, double* mag) {
         if (mag) mag[k] = or_seq_magnetization(s);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Arbitrary exchange matrices: Exchange::new(CsMat<f64>) (energy.rs:170-   */
+/* 172, evaluated at 194-213), swept by colour classes.                     */
+/*                                                                          */
+/* Preparation: the symmetrised pattern (j != i with J_ij or J_ji nonzero)  */
+/* is coloured greedily in site order (each site takes the smallest colour  */
+/* none of its lower-numbered neighbours has).  Site i's merged row lists   */
+/* that pattern by ascending j with a_ij = the J_ij of row i summed in      */
+/* entry order and w_ij = a_ij + (J_ji summed likewise) when colour(j) <    */
+/* colour(i), else 0; diagonal entries add up to the constant diag.  A      */
+/* sweep visits colours 0, 1, ...; the sites of a colour (sorted by id) are */
+/* independent.  Per site, in f64 (Ising) or fp32 (Heisenberg), left to     */
+/* right: h = sum a_ij s_j and hb = sum w_ij s_j over the merged row; the   */
+/* field h gives dE; after the update s_i.hb is the site's share of the     */
+/* bond sum sum_{i != j} J_ij s_i.s_j (each pair counted at its            */
+/* higher-coloured end, when both spins are final for the sweep).           */
+/* ------------------------------------------------------------------------ */
+struct or_graph {
+    uint64_t n;
+    uint32_t ncol;
+    uint32_t* colour;   /* per site */
+    uint64_t* mptr;     /* merged rows, n + 1 */
+    uint64_t* mcol;     /* neighbour site ids, ascending per row */
+    double* a;
+    double* w;
+    double diag;
+    uint64_t* order;    /* sites by (colour, id) */
+    uint64_t* cstart;   /* ncol + 1 */
+};
+
+typedef struct {
+    uint64_t row, col, k; /* k: entry index; kind 0 = J_row,col, 1 = transposed */
+    uint32_t kind;
+    double v;
+} or_trip;
+
+static int cmp_trip(const void* pa, const void* pb) {
+    const or_trip* a = (const or_trip*)pa;
+    const or_trip* b = (const or_trip*)pb;
+    if (a->row != b->row) return a->row < b->row ? -1 : 1;
+    if (a->col != b->col) return a->col < b->col ? -1 : 1;
+    if (a->kind != b->kind) return a->kind < b->kind ? -1 : 1;
+    return (a->k > b->k) - (a->k < b->k);
+}
+
+or_graph* or_graph_create(uint64_t n, const uint64_t* row_ptr, const uint64_t* col,
+                          const double* val) {
+    or_graph* G = (or_graph*)calloc(1, sizeof(or_graph));
+    G->n = n;
+    const uint64_t nnz = row_ptr[n];
+    or_trip* t = (or_trip*)malloc((2 * nnz + 1) * sizeof(or_trip));
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        for (uint64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+            const uint64_t j = col[k];
+            if (j == i) {
+                G->diag += val[k];
+                continue;
+            }
+            t[m++] = (or_trip){i, j, k, 0, val[k]};
+            t[m++] = (or_trip){j, i, k, 1, val[k]};
+        }
+    qsort(t, (size_t)m, sizeof(or_trip), cmp_trip);
+    G->mptr = (uint64_t*)calloc(n + 1, sizeof(uint64_t));
+    G->mcol = (uint64_t*)malloc((m + 1) * sizeof(uint64_t));
+    G->a = (double*)malloc((m + 1) * sizeof(double));
+    G->w = (double*)malloc((m + 1) * sizeof(double));
+    double* b = (double*)malloc((m + 1) * sizeof(double));
+    uint64_t q = 0;
+    for (uint64_t e = 0; e < m; ++e) {
+        if (e == 0 || t[e].row != t[e - 1].row || t[e].col != t[e - 1].col) {
+            G->mcol[q] = t[e].col;
+            G->a[q] = 0.0;
+            b[q] = 0.0;
+            ++G->mptr[t[e].row + 1];
+            ++q;
+        }
+        if (t[e].kind == 0)
+            G->a[q - 1] = G->a[q - 1] + t[e].v;
+        else
+            b[q - 1] = b[q - 1] + t[e].v;
+    }
+    for (uint64_t i = 0; i < n; ++i) G->mptr[i + 1] += G->mptr[i];
+    free(t);
+    /* greedy colouring in site order */
+    G->colour = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint64_t maxdeg = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        if (G->mptr[i + 1] - G->mptr[i] > maxdeg) maxdeg = G->mptr[i + 1] - G->mptr[i];
+    uint64_t* stamp = (uint64_t*)calloc(maxdeg + 2, sizeof(uint64_t));
+    G->ncol = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        for (uint64_t e = G->mptr[i]; e < G->mptr[i + 1]; ++e)
+            if (G->mcol[e] < i) stamp[G->colour[G->mcol[e]]] = i + 1;
+        uint32_t c = 0;
+        while (stamp[c] == i + 1) ++c;
+        G->colour[i] = c;
+        if (c + 1 > G->ncol) G->ncol = c + 1;
+    }
+    free(stamp);
+    for (uint64_t i = 0; i < n; ++i)
+        for (uint64_t e = G->mptr[i]; e < G->mptr[i + 1]; ++e)
+            G->w[e] = G->colour[G->mcol[e]] < G->colour[i] ? G->a[e] + b[e] : 0.0;
+    free(b);
+    G->cstart = (uint64_t*)calloc(G->ncol + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) ++G->cstart[G->colour[i] + 1];
+    for (uint32_t c = 0; c < G->ncol; ++c) G->cstart[c + 1] += G->cstart[c];
+    G->order = (uint64_t*)malloc((n + 1) * sizeof(uint64_t));
+    uint64_t* fill = (uint64_t*)malloc((G->ncol + 1) * sizeof(uint64_t));
+    memcpy(fill, G->cstart, (G->ncol + 1) * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) G->order[fill[G->colour[i]]++] = i;
+    free(fill);
+    return G;
+}
+
+void or_graph_destroy(or_graph* G) {
+    if (!G) return;
+    free(G->colour);
+    free(G->mptr);
+    free(G->mcol);
+    free(G->a);
+    free(G->w);
+    free(G->order);
+    free(G->cstart);
+    free(G);
+}
+
+uint32_t or_graph_colours(const or_graph* G) { return G->ncol; }
+void or_graph_colouring(const or_graph* G, uint32_t* colour) {
+    memcpy(colour, G->colour, G->n * sizeof(uint32_t));
+}
+double or_graph_diag(const or_graph* G) { return G->diag; }
+
+/* e^x for x <= 0 in f64 (Cody and Waite's reduction x = n ln2 + r with ln2
+ * split so n * hi is exact, then the degree-13 Taylor polynomial of e^r,
+ * |r| <= ln2 / 2, coefficients 1/k! rounded to nearest); 0 below -708.
+ * Every step is one IEEE operation or an fma, as in the engine. */
+double or_exp_f64_nonpos(double x) {
+    if (x < -708.0) return 0.0;
+    static const double c[14] = {
+        0x1.0000000000000p+0, 0x1.0000000000000p+0, 0x1.0000000000000p-1,
+        0x1.5555555555555p-3, 0x1.5555555555555p-5, 0x1.1111111111111p-7,
+        0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-16,
+        0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
+        0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33};
+    const double n = rint(x * 0x1.71547652b82fep+0);
+    double r = fma(n, -0x1.62e42fee00000p-1, x);
+    r = fma(n, -0x1.a39ef35793c76p-33, r);
+    double p = c[13];
+    for (int k = 12; k >= 0; --k) p = fma(p, r, c[k]);
+    union {
+        uint64_t u;
+        double d;
+    } s;
+    s.u = (uint64_t)((int64_t)n + 1023) << 52;
+    return p * s.d;
+}
+
+void or_graph_ising_init(const or_graph* G, uint64_t seed, int8_t* spins) {
+    for (uint64_t i = 0; i < G->n; ++i) {
+        uint32_t r[4];
+        draw(seed, (uint32_t)i, (uint32_t)(i >> 32), 0, wword(0, 9, 0), r);
+        spins[i] = r[0] < 0x80000000u ? 1 : -1;
+    }
+}
+
+void or_graph_ising_sweeps(const or_graph* G, const or_model* M, uint64_t seed, uint64_t sweep0,
+                           uint64_t nsweeps, int8_t* spins, int64_t* sums) {
+    const double hm = (M->terms & 2u) ? fabs(M->field) : 0.0;
+    const double hz = M->field_dir[2] >= 0.0 ? 1.0 : -1.0;
+    const double T = M->temperature < DBL_EPSILON ? DBL_EPSILON : M->temperature;
+    const double beta = 1.0 / T;
+    for (uint64_t k = 0; k < nsweeps; ++k) {
+        const uint64_t t = sweep0 + k;
+        int64_t mag = 0, bond = 0;
+        for (uint32_t c = 0; c < G->ncol; ++c)
+            for (uint64_t p = G->cstart[c]; p < G->cstart[c + 1]; ++p) {
+                const uint64_t i = G->order[p];
+                uint32_t r[4];
+                draw(seed, (uint32_t)i, (uint32_t)(i >> 32), (uint32_t)t, wword(t, 8, 0), r);
+                double s = (double)spins[i];
+                /* Spin::rand proposes Up when its draw is below 1/2 */
+                const int change = M->proposal != 1 || (r[1] < 0x80000000u ? 1.0 : -1.0) != s;
+                double h = 0.0, hb = 0.0;
+                for (uint64_t e = G->mptr[i]; e < G->mptr[i + 1]; ++e) {
+                    const double sj = (double)spins[G->mcol[e]];
+                    h = h + G->a[e] * sj;
+                    hb = hb + G->w[e] * sj;
+                }
+                if (change) {
+                    const double dE = ((2.0 * s) * h) - ((2.0 * hm) * hz) * s;
+                    int accept = dE <= 0.0;
+                    if (!accept) accept = (double)r[0] * 0x1p-32 < or_exp_f64_nonpos(-(dE * beta));
+                    if (accept) {
+                        s = -s;
+                        spins[i] = (int8_t)s;
+                    }
+                }
+                mag += (int64_t)s;
+                bond += (int64_t)llrint((s * hb) * 134217728.0);
+            }
+        if (sums) {
+            int64_t* o = sums + 5 * k;
+            o[0] = bond;
+            o[1] = 0;
+            o[2] = 0;
+            o[3] = mag;
+            o[4] = (int64_t)G->n;
+        }
+    }
+}
+
+void or_graph_heis_init(const or_graph* G, uint64_t seed, float* xyz) {
+    for (uint64_t i = 0; i < G->n; ++i) {
+        float s[3] = {0.0f, 0.0f, 1.0f};
+        int done = 0;
+        for (uint32_t k = 0; k < 64 && !done; ++k) {
+            uint32_t r[4];
+            draw(seed, (uint32_t)i, (uint32_t)(i >> 32), k, wword(0, 4, 0), r);
+            for (int q = 0; q < 2 && !done; ++q) done = marsaglia_f32(r[2 * q], r[2 * q + 1], s);
+        }
+        memcpy(xyz + 3 * i, s, sizeof s);
+    }
+}
+
+void or_graph_heis_sweeps(const or_graph* G, const or_model* M, uint64_t seed, uint64_t sweep0,
+                          uint64_t nsweeps, float* xyz, int64_t* sums) {
+    const float kk = (M->terms & 4u) ? (float)M->anisotropy : 0.0f;
+    const float hm = (M->terms & 2u) ? (float)fabs(M->field) : 0.0f;
+    const double T = M->temperature < DBL_EPSILON ? DBL_EPSILON : M->temperature;
+    const float beta = (float)(1.0 / T);
+    const float axis[3] = {(float)M->axis[0], (float)M->axis[1], (float)M->axis[2]};
+    const float fdir[3] = {(float)M->field_dir[0], (float)M->field_dir[1], (float)M->field_dir[2]};
+    const float jn = -1.0f; /* the couplings are inside h */
+    for (uint64_t k = 0; k < nsweeps; ++k) {
+        const uint64_t t = sweep0 + k;
+        int64_t acc[5] = {0, 0, 0, 0, 0};
+        for (uint32_t c = 0; c < G->ncol; ++c)
+            for (uint64_t q = G->cstart[c]; q < G->cstart[c + 1]; ++q) {
+                const uint64_t i = G->order[q];
+                const uint32_t lo = (uint32_t)i, hi = (uint32_t)(i >> 32);
+                float* s = xyz + 3 * i;
+                float h[3] = {0.0f, 0.0f, 0.0f}, hb[3] = {0.0f, 0.0f, 0.0f};
+                for (int a = 0; a < 3; ++a)
+                    for (uint64_t e = G->mptr[i]; e < G->mptr[i + 1]; ++e) {
+                        const float sj = xyz[3 * G->mcol[e] + a];
+                        const float av = (float)G->a[e], wv = (float)G->w[e];
+                        h[a] = h[a] + av * sj;
+                        hb[a] = hb[a] + wv * sj;
+                    }
+                float p[3] = {-s[0], -s[1], -s[2]};
+                float u;
+                if (M->proposal == 1) {
+                    u = random_proposal(seed, lo, hi, t, 0, p);
+                } else {
+                    uint32_t fw[4];
+                    draw(seed, lo, hi, (uint32_t)t, wword(t, 16, 0), fw);
+                    u = unit24f(fw[0]);
+                }
+                const float d[3] = {p[0] - s[0], p[1] - s[1], p[2] - s[2]};
+                const float e_ex = jn * dot3f(d, h);
+                const float pe = dot3f(p, axis), oe = dot3f(s, axis);
+                const float pe2 = pe * pe, oe2 = oe * oe;
+                const float e_an = kk * (pe2 - oe2);
+                const float e_ze = hm * dot3f(d, fdir);
+                const float e1 = e_ex + e_an;
+                const float dE = e1 + e_ze;
+                int accept = dE < 0.0f;
+                if (!accept) {
+                    const float x = -(dE * beta);
+                    accept = u < or_exp_nonpos(x);
+                }
+                if (accept) memcpy(s, p, sizeof p);
+                acc[1] += fix27(s[0]);
+                acc[2] += fix27(s[1]);
+                acc[3] += fix27(s[2]);
+                const float se = dot3f(s, axis);
+                acc[4] += fix27(se * se);
+                acc[0] += (int64_t)llrintf(dot3f(s, hb) * 134217728.0f);
+            }
+        if (sums) memcpy(sums + 5 * k, acc, sizeof acc);
+    }
+}
+
+/* The reference's sequential Metropolis over a caller's matrix: the CSR is
+ * kept as given (row order, duplicates and diagonal included), as
+ * Exchange::new(matrix) evaluates it (energy.rs:194-206). */
+or_seq* or_seq_create_csr(const or_model* M, uint64_t n, const uint64_t* row_ptr,
+                          const uint64_t* col, const double* val, uint64_t seed) {
+    or_seq* s = (or_seq*)calloc(1, sizeof(or_seq));
+    s->M = *M;
+    s->n = n;
+    const uint64_t nnz = row_ptr[n];
+    s->indptr = (uint64_t*)malloc((n + 1) * sizeof(uint64_t));
+    s->indices = (uint64_t*)malloc((nnz + 1) * sizeof(uint64_t));
+    s->values = (double*)malloc((nnz + 1) * sizeof(double));
+    memcpy(s->indptr, row_ptr, (n + 1) * sizeof(uint64_t));
+    memcpy(s->indices, col, nnz * sizeof(uint64_t));
+    memcpy(s->values, val, nnz * sizeof(double));
+    if (M->spin == 2)
+        s->heis = (double*)calloc(3 * n, sizeof(double));
+    else
+        s->ising = (int8_t*)calloc(n, 1);
+    or_pcg64_seed(&s->rng, seed);
+    return s;
+}

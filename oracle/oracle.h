@@ -94,6 +94,27 @@ void or_wolff_threshold(const or_model* M, uint32_t* thr, int* all);
 void or_wolff_steps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t step0,
                     uint64_t nsteps, int8_t* spins, int64_t* sums);
 
+/* Arbitrary exchange matrices (Exchange::new, energy.rs:170-172) swept by
+ * the colour classes of a greedy colouring; see oracle.c for the exact
+ * semantics.  CSR input: row_ptr[n + 1], col, val (J_ij at row i). */
+typedef struct or_graph or_graph;
+or_graph* or_graph_create(uint64_t n, const uint64_t* row_ptr, const uint64_t* col,
+                          const double* val);
+void or_graph_destroy(or_graph* G);
+uint32_t or_graph_colours(const or_graph* G);
+void or_graph_colouring(const or_graph* G, uint32_t* colour);
+double or_graph_diag(const or_graph* G);
+double or_exp_f64_nonpos(double x);
+/* spins in natural order: Ising int8 +-1, Heisenberg 3 floats per site;
+ * sums {bond (2^-27), 0, 0, mag, N} (Ising) or {bond, mx, my, mz, axis2}
+ * (Heisenberg, all 2^-27), bond = sum over i != j of J_ij s_i.s_j. */
+void or_graph_ising_init(const or_graph* G, uint64_t seed, int8_t* spins);
+void or_graph_ising_sweeps(const or_graph* G, const or_model* M, uint64_t seed, uint64_t sweep0,
+                           uint64_t nsweeps, int8_t* spins, int64_t* sums);
+void or_graph_heis_init(const or_graph* G, uint64_t seed, float* xyz);
+void or_graph_heis_sweeps(const or_graph* G, const or_model* M, uint64_t seed, uint64_t sweep0,
+                          uint64_t nsweeps, float* xyz, int64_t* sums);
+
 /* PCG64 XSL-RR 128/64: state and increment as (hi, lo) 64-bit halves. */
 typedef struct or_pcg64 {
     uint64_t state_hi, state_lo, inc_hi, inc_lo;
@@ -104,6 +125,9 @@ uint64_t or_pcg64_next(or_pcg64* g);
 /* Sequential reference restatement. */
 typedef struct or_seq or_seq;
 or_seq* or_seq_create(const or_lattice* L, const or_model* M, uint64_t seed);
+/* The same over a caller's exchange matrix (CSR kept as given). */
+or_seq* or_seq_create_csr(const or_model* M, uint64_t n, const uint64_t* row_ptr,
+                          const uint64_t* col, const double* val, uint64_t seed);
 void or_seq_destroy(or_seq* s);
 void or_seq_set_thermostat(or_seq* s, double temperature, double field, const double dir[3]);
 /* spins: Ising int8 +-1 (n) or Heisenberg doubles (3n) */

@@ -70,6 +70,25 @@ def lib():
                                      C.c_void_p, C.c_void_p]
         L.or_exp_nonpos.argtypes = [C.c_float]
         L.or_exp_nonpos.restype = C.c_float
+        L.or_graph_create.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_graph_create.restype = C.c_void_p
+        L.or_graph_destroy.argtypes = [C.c_void_p]
+        L.or_graph_colours.argtypes = [C.c_void_p]
+        L.or_graph_colours.restype = C.c_uint32
+        L.or_graph_colouring.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_graph_diag.argtypes = [C.c_void_p]
+        L.or_graph_diag.restype = C.c_double
+        L.or_exp_f64_nonpos.argtypes = [C.c_double]
+        L.or_exp_f64_nonpos.restype = C.c_double
+        L.or_graph_ising_init.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        L.or_graph_ising_sweeps.argtypes = [C.c_void_p, P(Model), C.c_uint64, C.c_uint64,
+                                            C.c_uint64, C.c_void_p, C.c_void_p]
+        L.or_graph_heis_init.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        L.or_graph_heis_sweeps.argtypes = [C.c_void_p, P(Model), C.c_uint64, C.c_uint64,
+                                           C.c_uint64, C.c_void_p, C.c_void_p]
+        L.or_seq_create_csr.argtypes = [P(Model), C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_uint64]
+        L.or_seq_create_csr.restype = C.c_void_p
         L.or_pcg64_seed.argtypes = [P(Pcg64), C.c_uint64]
         L.or_pcg64_next.argtypes = [P(Pcg64)]
         L.or_pcg64_next.restype = C.c_uint64
@@ -201,11 +220,81 @@ def exp_nonpos(x: float) -> float:
     return float(lib().or_exp_nonpos(x))
 
 
-class Sequential:
-    """The reference's random-site Metropolis (integrator.rs:66-138) in f64."""
+def exp_f64_nonpos(x: float) -> float:
+    return float(lib().or_exp_f64_nonpos(x))
 
-    def __init__(self, L: Lattice, M: Model, seed: int):
+
+def csr_arrays(row_ptr, col, val):
+    """Contiguous uint64 / uint64 / float64 copies of a CSR matrix."""
+    return (np.ascontiguousarray(row_ptr, dtype=np.uint64),
+            np.ascontiguousarray(col, dtype=np.uint64),
+            np.ascontiguousarray(val, dtype=np.float64))
+
+
+class Graph:
+    """A chain's exchange given as an arbitrary CSR matrix (Exchange::new,
+    energy.rs:170-172), swept by the colour classes of its greedy colouring."""
+
+    def __init__(self, row_ptr, col, val):
+        self.row_ptr, self.col, self.val = csr_arrays(row_ptr, col, val)
+        self.n = len(self.row_ptr) - 1
+        self.h = lib().or_graph_create(self.n, _ptr(self.row_ptr), _ptr(self.col),
+                                       _ptr(self.val))
+
+    def close(self):
+        if self.h:
+            lib().or_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def colours(self) -> int:
+        return int(lib().or_graph_colours(self.h))
+
+    def colouring(self) -> np.ndarray:
+        c = np.zeros(self.n, dtype=np.uint32)
+        lib().or_graph_colouring(self.h, _ptr(c))
+        return c
+
+    def diag(self) -> float:
+        return float(lib().or_graph_diag(self.h))
+
+    def ising_init(self, seed: int) -> np.ndarray:
+        s = np.zeros(self.n, dtype=np.int8)
+        lib().or_graph_ising_init(self.h, seed, _ptr(s))
+        return s
+
+    def ising_sweeps(self, M: Model, seed: int, sweep0: int, n: int, spins: np.ndarray):
+        assert spins.dtype == np.int8 and spins.flags.c_contiguous and len(spins) == self.n
+        sums = np.zeros((n, 5), dtype=np.int64)
+        lib().or_graph_ising_sweeps(self.h, C.byref(M), seed, sweep0, n, _ptr(spins), _ptr(sums))
+        return sums
+
+    def heis_init(self, seed: int) -> np.ndarray:
+        s = np.zeros((self.n, 3), dtype=np.float32)
+        lib().or_graph_heis_init(self.h, seed, _ptr(s))
+        return s
+
+    def heis_sweeps(self, M: Model, seed: int, sweep0: int, n: int, xyz: np.ndarray):
+        assert xyz.dtype == np.float32 and xyz.flags.c_contiguous and len(xyz) == self.n
+        sums = np.zeros((n, 5), dtype=np.int64)
+        lib().or_graph_heis_sweeps(self.h, C.byref(M), seed, sweep0, n, _ptr(xyz), _ptr(sums))
+        return sums
+
+
+class Sequential:
+    """The reference's random-site Metropolis (integrator.rs:66-138) in f64,
+    on a lattice or (csr=(row_ptr, col, val)) on a caller's exchange matrix."""
+
+    def __init__(self, L, M: Model, seed: int, csr=None):
         self.L, self.M = L, M
+        if csr is not None:
+            self._csr = csr_arrays(*csr)
+            self.n = len(self._csr[0]) - 1
+            self.h = lib().or_seq_create_csr(C.byref(M), self.n, _ptr(self._csr[0]),
+                                             _ptr(self._csr[1]), _ptr(self._csr[2]), seed)
+            return
         self.n = num_sites(L)
         self.h = lib().or_seq_create(C.byref(L), C.byref(M), seed)
 

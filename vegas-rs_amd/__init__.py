@@ -20,7 +20,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libvegas_hip.so"
 
 # lattice kinds, pbc bits, terms, spins, proposals (vegas_hip.h)
-SC, BCC, FCC = 0, 1, 2
+SC, BCC, FCC, GRAPH = 0, 1, 2, 3
 PBC_X, PBC_Y, PBC_Z = 1, 2, 4
 EXCHANGE, ZEEMAN, ANISOTROPY, GAUGE = 1, 2, 4, 8
 ISING, ISING_BITS, HEISENBERG = 0, 1, 2
@@ -59,6 +59,11 @@ class ChainDesc(C.Structure):
 
 class SweepSums(C.Structure):
     _fields_ = [("bond", C.c_int64), ("mag", C.c_int64 * 3), ("axis2", C.c_int64)]
+
+
+class Csr(C.Structure):
+    _fields_ = [("n_sites", C.c_uint64), ("row_ptr", C.c_void_p), ("col", C.c_void_p),
+                ("value", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):
@@ -111,6 +116,9 @@ _SIGS = {
     "vg_ctx_create": [C.c_int, _P(_vp)],
     "vg_ctx_destroy": [_vp],
     "vg_batch_create": [_vp, C.c_uint32, _P(ChainDesc), _P(_vp)],
+    "vg_batch_create_csr": [_vp, C.c_uint32, _P(ChainDesc), _P(Csr), _P(_vp)],
+    "vg_batch_colours": [_vp, C.c_uint32, _P(C.c_uint32)],
+    "vg_batch_sums_scales": [_vp, C.c_uint32, _P(C.c_double)],
     "vg_batch_destroy": [_vp],
     "vg_batch_size": [_vp, _P(C.c_uint32)],
     "vg_batch_num_sites": [_vp, C.c_uint32, _P(C.c_uint64)],
@@ -208,6 +216,12 @@ def hamiltonian(exchange=None, zeeman=False, anisotropy=None, axis=(0.0, 0.0, 1.
                        (C.c_double * 3)(*axis), gauge or 0.0)
 
 
+def graph_lattice() -> Lattice:
+    """The lattice field of a chain whose exchange is a matrix (Batch(...,
+    exchange=[...]))."""
+    return Lattice(GRAPH, 0, 0, 0, 0)
+
+
 def chain(lattice: Lattice, spin=ISING, proposal=FLIP, hamiltonian: Hamiltonian | None = None,
           seed=0) -> ChainDesc:
     if hamiltonian is None:
@@ -236,11 +250,27 @@ class Context:
 class Batch:
     """Independent chains resident on the device (one Machine each)."""
 
-    def __init__(self, ctx: Context, descs):
+    def __init__(self, ctx: Context, descs, exchange=None):
+        """exchange: for batches with graph chains (lattice kind GRAPH), one
+        entry per chain -- a CSR matrix (row_ptr, col, value) for a graph
+        chain, None for a lattice chain (Exchange::new(matrix),
+        src/energy.rs:170-172)."""
         descs = list(descs)
         arr = (ChainDesc * len(descs))(*descs)
         h = _vp()
-        _check(_lib.vg_batch_create(ctx.h, len(descs), arr, C.byref(h)))
+        if exchange is None:
+            _check(_lib.vg_batch_create(ctx.h, len(descs), arr, C.byref(h)))
+        else:
+            keep, mats = [], (Csr * len(descs))()
+            for i, m in enumerate(exchange):
+                if m is None:
+                    continue
+                rp = np.ascontiguousarray(m[0], dtype=np.uint64)
+                cl = np.ascontiguousarray(m[1], dtype=np.uint64)
+                vl = np.ascontiguousarray(m[2], dtype=np.float64)
+                keep += [rp, cl, vl]
+                mats[i] = Csr(len(rp) - 1, rp.ctypes.data, cl.ctypes.data, vl.ctypes.data)
+            _check(_lib.vg_batch_create_csr(ctx.h, len(descs), arr, mats, C.byref(h)))
         self.h = h
         self.ctx = ctx
         self.descs = descs
@@ -256,6 +286,16 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+    def colours(self, i: int) -> int:
+        v = C.c_uint32()
+        _check(_lib.vg_batch_colours(self.h, i, C.byref(v)))
+        return v.value
+
+    def sums_scales(self, i: int) -> np.ndarray:
+        v = (C.c_double * 5)()
+        _check(_lib.vg_batch_sums_scales(self.h, i, v))
+        return np.array(list(v))
 
     def num_sites(self, i: int) -> int:
         n = C.c_uint64()

@@ -23,26 +23,37 @@ int coordination(int kind) { return kind == VG_LATTICE_SC ? 6 : (kind == VG_LATT
 
 int validate_desc(const vg_chain_desc& d) {
     const vg_lattice& L = d.lattice;
-    if (L.kind != VG_LATTICE_SC && L.kind != VG_LATTICE_BCC && L.kind != VG_LATTICE_FCC)
+    if (L.kind == VG_LATTICE_GRAPH) {
+        // Exchange::new(matrix): int8 Ising or Heisenberg, Metropolis moves
+        if (d.spin != VG_SPIN_ISING && d.spin != VG_SPIN_HEISENBERG)
+            return fail(VG_ERR_NOT_IMPLEMENTED, "graph chains take int8 Ising or Heisenberg spins");
+        if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM)
+            return fail(VG_ERR_NOT_IMPLEMENTED, "graph chains take flip or random proposals");
+        if (!(d.hamiltonian.terms & VG_TERM_EXCHANGE))
+            return fail(VG_ERR_INVALID_ARGUMENT, "a graph chain's Hamiltonian needs its Exchange term");
+    } else if (L.kind != VG_LATTICE_SC && L.kind != VG_LATTICE_BCC && L.kind != VG_LATTICE_FCC) {
         return fail(VG_ERR_UNSUPPORTED_LATTICE, "unknown lattice kind %d", L.kind);
-    if (L.lx == 0 || L.ly == 0 || L.lz == 0)
-        return fail(VG_ERR_INVALID_ARGUMENT, "lattice dimensions must be positive");
-    if (L.pbc & ~7u) return fail(VG_ERR_INVALID_ARGUMENT, "unknown pbc bits 0x%x", L.pbc);
-    if (L.kind == VG_LATTICE_SC) {
-        if (L.lx % 2)
-            return fail(VG_ERR_UNSUPPORTED_LATTICE,
-                        "sc lattice needs an even x length for the two-colour layout (lx=%u)",
-                        L.lx);
-        const uint32_t dims[3] = {L.lx, L.ly, L.lz};
-        for (int a = 0; a < 3; ++a)
-            if ((L.pbc >> a) & 1u && dims[a] % 2)
+    }
+    if (L.kind != VG_LATTICE_GRAPH) {
+        if (L.lx == 0 || L.ly == 0 || L.lz == 0)
+            return fail(VG_ERR_INVALID_ARGUMENT, "lattice dimensions must be positive");
+        if (L.pbc & ~7u) return fail(VG_ERR_INVALID_ARGUMENT, "unknown pbc bits 0x%x", L.pbc);
+        if (L.kind == VG_LATTICE_SC) {
+            if (L.lx % 2)
                 return fail(VG_ERR_UNSUPPORTED_LATTICE,
-                            "periodic sc axis %c has odd length %u: not bipartite", "xyz"[a],
-                            dims[a]);
-    }  // bcc, fcc: the sublattices are the colours for any sizes and boundaries
-    const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
-    if ((uint64_t)L.ly * L.lz > 0xFFFFFFFFull || cells > (1ull << 34))
-        return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
+                            "sc lattice needs an even x length for the two-colour layout (lx=%u)",
+                            L.lx);
+            const uint32_t dims[3] = {L.lx, L.ly, L.lz};
+            for (int a = 0; a < 3; ++a)
+                if ((L.pbc >> a) & 1u && dims[a] % 2)
+                    return fail(VG_ERR_UNSUPPORTED_LATTICE,
+                                "periodic sc axis %c has odd length %u: not bipartite", "xyz"[a],
+                                dims[a]);
+        }  // bcc, fcc: the sublattices are the colours for any sizes and boundaries
+        const uint64_t cells = (uint64_t)L.lx * L.ly * L.lz;
+        if ((uint64_t)L.ly * L.lz > 0xFFFFFFFFull || cells > (1ull << 34))
+            return fail(VG_ERR_INVALID_ARGUMENT, "lattice too large");
+    }
     if (d.spin != VG_SPIN_ISING && d.spin != VG_SPIN_HEISENBERG && d.spin != VG_SPIN_ISING_BITS)
         return fail(VG_ERR_INVALID_ARGUMENT, "unknown spin kind %d", d.spin);
     if (d.proposal != VG_PROPOSAL_FLIP && d.proposal != VG_PROPOSAL_RANDOM &&
@@ -137,6 +148,15 @@ void heis_params(const ChainHost& ch, ChainDev& d) {
     d.fz = (float)ch.field_dir[2];
 }
 
+// Ising parameters of a graph chain, f64: 1/T, |h| (0 without Zeeman) and the
+// sign of the field's orientation (Field of an Ising system is Up or Down).
+void graph_ising_params(const ChainHost& ch, ChainDev& d) {
+    const vg_hamiltonian& H = ch.desc.hamiltonian;
+    d.gbeta = 1.0 / ch.temperature;
+    d.ghm = (H.terms & VG_TERM_ZEEMAN) ? std::fabs(ch.field_mag) : 0.0;
+    d.ghz = ch.field_dir[2] >= 0.0 ? 1.0 : -1.0;
+}
+
 // Wolff bond threshold: a bond is active iff its 32-bit draw is below
 // ceil(p 2^32), p = 1 - exp(-2J/T) in f64 (integrator.rs:188-192).
 void wolff_threshold(const ChainHost& ch, ChainDev& d) {
@@ -156,12 +176,18 @@ void wolff_threshold(const ChainHost& ch, ChainDev& d) {
 void refresh_params(vg_batch* B, uint32_t i) {
     ChainHost& ch = B->chains[i];
     ChainDev& d = B->groups[ch.group].host[ch.index];
-    if (ch.group == kIsingWolff)
+    if (ch.group == kIsingWolff) {
         wolff_threshold(ch, d);
-    else if (is_ising(ch.group))
-        ising_thresholds(ch, d);
-    else
+    } else if (ch.group == kIsingGraph) {
+        graph_ising_params(ch, d);
+    } else if (ch.group == kHeisGraph) {
         heis_params(ch, d);
+        d.jn = -1.0f;  // the couplings are inside the field h
+    } else if (is_ising(ch.group)) {
+        ising_thresholds(ch, d);
+    } else {
+        heis_params(ch, d);
+    }
 }
 
 // Ising passes write {mag, bond} per wave (4 waves per workgroup); Heisenberg
@@ -186,6 +212,18 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
     if (g == kIsingWolff) {  // natural order; no planes or work items
         d.plane = d.nsites;
         wolff_threshold(ch, d);
+        return VG_OK;
+    }
+    if (is_graph(g)) {  // sections and colour classes are set by the caller
+        d.nsites = ch.n_sites;
+        d.ncol = 0;
+        d.z = 0;
+        if (g == kIsingGraph) {
+            graph_ising_params(ch, d);
+        } else {
+            heis_params(ch, d);
+            d.jn = -1.0f;
+        }
         return VG_OK;
     }
     const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
@@ -215,8 +253,10 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
 
 uint64_t colour_elems(int g, const ChainDev& d) { return (is_ising(g) ? 1 : 3) * d.plane; }
 
-uint32_t quantities(int group) { return is_ising(group) ? 8u : 5u; }
-uint32_t quantity_bytes(int group) { return is_ising(group) ? 4u : 8u; }
+// Ising lattice passes write {mag, bond} per wave (4 waves); Heisenberg and
+// graph passes write {bond, mx, my, mz, axis2} per workgroup.
+uint32_t quantities(int group) { return is_ising(group) && !is_graph(group) ? 8u : 5u; }
+uint32_t quantity_bytes(int group) { return is_ising(group) && !is_graph(group) ? 4u : 8u; }
 
 void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev* dp,
                        hipStream_t st) {
@@ -278,6 +318,8 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
         uint64_t items;
         if (g == kIsingWolff) {
             items = 1;  // one workgroup per chain
+        } else if (is_graph(g)) {
+            items = d.items;  // one thread per site of the largest colour class (set at creation)
         } else if (is_zmarch(g)) {
             d.nzc = (d.lz + tile - 1) / tile;
             items = (uint64_t)d.nzc * 2 * d.hy * d.segs;
@@ -305,7 +347,7 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
     G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
     if (is_zmarch(g)) G.prio = all_resident(g, tile, G.grid) ? 1u : 0u;
     G.partial_bytes_per_sweep = std::max<uint64_t>(
-        256, (uint64_t)colours_of(g) * G.total_blocks * quantities(g) * quantity_bytes(g));
+        256, (uint64_t)passes_of(G, g) * G.total_blocks * quantities(g) * quantity_bytes(g));
     for (auto& kv : B->graphs) quiet(hipGraphExecDestroy(kv.second));
     B->graphs.clear();
     return VG_OK;
@@ -440,7 +482,7 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
         for (int g = 0; g < kNumGroups; ++g) {
             Group& G = B->groups[g];
             if (G.members.empty()) continue;
-            for (uint32_t c = 0; c < colours_of(g); ++c) {
+            for (uint32_t c = 0; c < passes_of(G, g); ++c) {
                 const bool timed = B->timing && !B->use_graph;
                 if (timed) {
                     const uint32_t k = B->pass_events_used;
@@ -468,6 +510,14 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
                     hipLaunchKernelGGL(vg::heis_bcc_pass, grid, block, 0, st, G.d_chains,
                                        G.d_block_chain, (float*)G.arena, (int64_t*)G.partials,
                                        c, s, G.total_blocks, G.grid);
+                else if (g == kIsingGraph)
+                    hipLaunchKernelGGL(vg::graph_ising_pass, dim3(G.total_blocks), block, 0, st,
+                                       G.d_chains, G.d_block_chain, (uint8_t*)G.arena,
+                                       (int64_t*)G.partials, c, s, G.total_blocks, G.npass);
+                else if (g == kHeisGraph)
+                    hipLaunchKernelGGL(vg::graph_heis_pass, dim3(G.total_blocks), block, 0, st,
+                                       G.d_chains, G.d_block_chain, (uint8_t*)G.arena,
+                                       (int64_t*)G.partials, c, s, G.total_blocks, G.npass);
                 else
                     launch_fcc_pass(G, st, c, s, 2);
                 if (timed) {
@@ -484,13 +534,13 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
         if (g == kIsingWolff)
             hipLaunchKernelGGL(vg::wolff_run, dim3((uint32_t)G.members.size()), dim3(256), 0, st,
                                G.d_chains, (uint8_t*)G.arena, nsw, B->d_records);
-        else if (is_ising(g))
+        else if (is_ising(g) && !is_graph(g))
             hipLaunchKernelGGL(vg::ising_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
                                (const int32_t*)G.partials, G.total_blocks, colours_of(g), nsw,
                                B->d_records);
         else
             hipLaunchKernelGGL(vg::heis_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
-                               (const int64_t*)G.partials, G.total_blocks, colours_of(g), nsw,
+                               (const int64_t*)G.partials, G.total_blocks, passes_of(G, g), nsw,
                                B->d_records);
     }
     VG_HIP(hipGetLastError());
@@ -545,7 +595,10 @@ void observables(const ChainHost& ch, const vg_sweep_sums& r, double* E, double*
     const vg_hamiltonian& H = ch.desc.hamiltonian;
     const double sc = sums_scale(ch.desc.spin);
     const double N = (double)ch.n_sites;
-    const double bond = (double)r.bond * sc;
+    // graph chains: bond = sum over i != j of J_ij s_i.s_j (couplings inside)
+    // at 2^-27, and sum_i e_i of the exchange is -(bond + sum_i J_ii)
+    const bool graph = is_graph(ch.group);
+    const double bond = (double)r.bond * (graph ? 1.0 / 134217728.0 : sc);
     double m[3] = {(double)r.mag[0] * sc, (double)r.mag[1] * sc, (double)r.mag[2] * sc};
     double axis2 = (double)r.axis2 * sc;
     double hdir[3] = {ch.field_dir[0], ch.field_dir[1], ch.field_dir[2]};
@@ -558,12 +611,12 @@ void observables(const ChainHost& ch, const vg_sweep_sums& r, double* E, double*
     const double mdoth = m[0] * hdir[0] + m[1] * hdir[1] + m[2] * hdir[2];
     double e = 0.0;
     if (H.compound) {
-        if (H.terms & VG_TERM_EXCHANGE) e += -2.0 * H.exchange * bond;
+        if (H.terms & VG_TERM_EXCHANGE) e += graph ? -(bond + ch.diag) : -2.0 * H.exchange * bond;
         if (H.terms & VG_TERM_ZEEMAN) e += hm * mdoth;
         if (H.terms & VG_TERM_ANISOTROPY) e += H.anisotropy * axis2;
         if (H.terms & VG_TERM_GAUGE) e += H.gauge * N;
     } else if (H.terms & VG_TERM_EXCHANGE) {
-        e = -H.exchange * bond;
+        e = graph ? -(bond + ch.diag) / 2.0 : -H.exchange * bond;
     } else if (H.terms & VG_TERM_ZEEMAN) {
         e = -hm * mdoth;
     } else if (H.terms & VG_TERM_ANISOTROPY) {
@@ -626,7 +679,8 @@ int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
             if (timed) {
                 size_t need = 0;  // a start/stop pair per colour pass
                 for (int g = 0; g < kNumGroups; ++g)
-                    if (!B->groups[g].members.empty()) need += (size_t)nsw * 2 * colours_of(g);
+                    if (!B->groups[g].members.empty())
+                        need += (size_t)nsw * 2 * passes_of(B->groups[g], g);
                 while (B->pass_events.size() < need) {
                     hipEvent_t e;
                     VG_HIP(hipEventCreate(&e));
@@ -662,7 +716,7 @@ int run_impl(vg_batch* B, uint64_t sweeps, vg_sweep_sums* sums, double* energy,
             for (uint32_t sw = 0; sw < nsw; ++sw)
                 for (int g = 0; g < kNumGroups; ++g) {
                     if (B->groups[g].members.empty()) continue;
-                    for (uint32_t c = 0; c < colours_of(g); ++c, k += 2) {
+                    for (uint32_t c = 0; c < passes_of(B->groups[g], g); ++c, k += 2) {
                         float t = 0.0f;
                         VG_HIP(hipEventElapsedTime(&t, B->pass_events[k], B->pass_events[k + 1]));
                         B->pass_ms[g] += t;
@@ -759,13 +813,59 @@ int vg_ctx_destroy(vg_ctx* ctx) {
     return VG_OK;
 }
 
-int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, vg_batch** out) {
+}  // extern "C"
+
+namespace vgi {
+
+// Copies a host array into the arena on the batch stream.
+template <class T>
+int upload_section(vg_batch* B, Group& G, uint64_t off, const std::vector<T>& v) {
+    if (v.empty()) return VG_OK;
+    VG_HIP(hipMemcpyAsync((uint8_t*)G.arena + off, v.data(), v.size() * sizeof(T),
+                          hipMemcpyHostToDevice, B->ctx->stream));
+    return VG_OK;
+}
+
+// The colour classes, merged rows and site ids of graph chain k of group g.
+int upload_graph(vg_batch* B, int g, size_t k, const GraphBuild& gr) {
+    Group& G = B->groups[g];
+    const ChainDev& d = G.host[k];
+    int rc = upload_section(B, G, d.off[1], gr.ptr);
+    if (!rc) rc = upload_section(B, G, d.off[2], gr.nbr);
+    if (!rc) rc = upload_section(B, G, d.goff[1], gr.site_of_pos);
+    if (!rc) rc = upload_section(B, G, d.goff[2], gr.cstart);
+    if (rc) return rc;
+    if (g == kIsingGraph) {
+        rc = upload_section(B, G, d.off[3], gr.a);
+        if (!rc) rc = upload_section(B, G, d.goff[0], gr.w);
+    } else {  // Heisenberg couplings in fp32
+        std::vector<float> af(gr.a.begin(), gr.a.end()), wf(gr.w.begin(), gr.w.end());
+        rc = upload_section(B, G, d.off[3], af);
+        if (!rc) rc = upload_section(B, G, d.goff[0], wf);
+    }
+    // the host arrays may go away once this returns
+    const hipError_t e = hipStreamSynchronize(B->ctx->stream);
+    if (!rc && e != hipSuccess) rc = fail(VG_ERR_DEVICE, "uploading a graph: %s", hipGetErrorString(e));
+    return rc;
+}
+
+int create_impl(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, const vg_csr* graphs,
+                vg_batch** out) {
     if (!ctx || !descs || !out) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (n_chains == 0) return fail(VG_ERR_INVALID_ARGUMENT, "a batch needs at least one chain");
+    std::vector<GraphBuild> gb(n_chains);
     for (uint32_t i = 0; i < n_chains; ++i) {
         int rc = validate_desc(descs[i]);
         if (rc) return fail(rc, "chain %u: %s", i, g_error.c_str());
+        if (descs[i].lattice.kind == VG_LATTICE_GRAPH) {
+            if (!graphs)
+                return fail(VG_ERR_INVALID_ARGUMENT,
+                            "chain %u has an exchange matrix: create the batch with "
+                            "vg_batch_create_csr", i);
+            rc = build_graph(graphs[i], gb[i]);
+            if (rc) return fail(rc, "chain %u: %s", i, g_error.c_str());
+        }
     }
     VG_HIP(hipSetDevice(ctx->device));
     vg_batch* B = new vg_batch;
@@ -775,8 +875,14 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         ChainHost& ch = B->chains[i];
         ch.desc = descs[i];
         const vg_lattice& L = ch.desc.lattice;
-        ch.n_sites = sites_of(L);
         ch.group = group_of(ch.desc);
+        if (is_graph(ch.group)) {
+            ch.n_sites = graphs[i].n_sites;
+            ch.diag = gb[i].diag;
+            ch.graph_pos = gb[i].pos_of_site;
+        } else {
+            ch.n_sites = sites_of(L);
+        }
         Group& G = B->groups[ch.group];
         ch.index = (uint32_t)G.members.size();
         G.members.push_back(i);
@@ -796,6 +902,25 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
                 return rc;
             }
             const uint64_t elem_align = 256 / elem_bytes(g);  // 256-byte chain alignment
+            if (is_graph(g)) {  // bytes: spins, row offsets, neighbours, a, w, site ids, classes
+                const GraphBuild& gr = gb[G.members[k]];
+                const uint64_t n = ch.n_sites, m = gr.nbr.size();
+                const uint64_t vb = g == kIsingGraph ? 8 : 4;
+                const uint64_t sizes[7] = {g == kIsingGraph ? n : 12 * n, 4 * (n + 1), 4 * m,
+                                           vb * m, vb * m, 4 * n, 4 * ((uint64_t)gr.ncol + 1)};
+                uint64_t offs[7];
+                for (int part = 0; part < 7; ++part) {
+                    offset = round_up(offset, 256);
+                    offs[part] = offset;
+                    offset += sizes[part];
+                }
+                for (int part = 0; part < 4; ++part) d.off[part] = offs[part];
+                for (int part = 0; part < 3; ++part) d.goff[part] = offs[4 + part];
+                d.ncol = gr.ncol;
+                d.items = gr.max_class;
+                G.npass = std::max(G.npass, gr.ncol);
+                continue;
+            }
             if (g == kIsingWolff) {  // bytes: spins, stamps, queue
                 const uint64_t sizes[3] = {d.nsites, 4 * d.nsites, 4 * d.nsites};
                 for (int part = 0; part < 3; ++part) {
@@ -823,6 +948,15 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
             return fail(e == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE,
                         "allocating the chain arena (%llu bytes): %s",
                         (unsigned long long)G.arena_bytes, hipGetErrorString(e));
+        }
+        if (is_graph(g)) {
+            for (size_t k = 0; k < G.members.size(); ++k) {
+                int rc = upload_graph(B, g, k, gb[G.members[k]]);
+                if (rc) {
+                    free_batch(B);
+                    return rc;
+                }
+            }
         }
         // Work layout with the finest tile sizes the partial buffers.
         int rc = layout_work(B, g, 1);
@@ -875,6 +1009,40 @@ int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, 
         return fail(VG_ERR_DEVICE, "%s", hipGetErrorString(e));
     }
     *out = B;
+    return VG_OK;
+}
+
+}  // namespace vgi
+
+extern "C" {
+
+int vg_batch_create(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, vg_batch** out) {
+    return create_impl(ctx, n_chains, descs, nullptr, out);
+}
+
+int vg_batch_create_csr(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs,
+                        const vg_csr* exchange, vg_batch** out) {
+    if (!exchange) return fail(VG_ERR_INVALID_ARGUMENT, "null exchange matrices");
+    return create_impl(ctx, n_chains, descs, exchange, out);
+}
+
+int vg_batch_colours(const vg_batch* batch, uint32_t chain, uint32_t* colours) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!colours) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    const ChainHost& ch = batch->chains[chain];
+    *colours = is_graph(ch.group) ? batch->groups[ch.group].host[ch.index].ncol
+                                  : colours_of(ch.group);
+    return VG_OK;
+}
+
+int vg_batch_sums_scales(const vg_batch* batch, uint32_t chain, double scales[5]) {
+    int rc = check_chain(batch, chain);
+    if (rc) return rc;
+    if (!scales) return fail(VG_ERR_INVALID_ARGUMENT, "null argument");
+    const ChainHost& ch = batch->chains[chain];
+    for (int k = 0; k < 5; ++k) scales[k] = sums_scale(ch.desc.spin);
+    if (is_graph(ch.group)) scales[0] = 1.0 / 134217728.0;
     return VG_OK;
 }
 
@@ -933,6 +1101,12 @@ int vg_batch_randomize(vg_batch* batch) {
         if (ch.group == kIsingWolff) {
             hipLaunchKernelGGL(vg::wolff_init, dim3((uint32_t)((d.nsites + 255) / 256)), dim3(256),
                                0, st, dp, (uint8_t*)G.arena);
+        } else if (ch.group == kIsingGraph) {
+            hipLaunchKernelGGL(vg::graph_ising_init, dim3((uint32_t)((d.nsites + 255) / 256)),
+                               dim3(256), 0, st, dp, (uint8_t*)G.arena);
+        } else if (ch.group == kHeisGraph) {
+            hipLaunchKernelGGL(vg::graph_heis_init, dim3((uint32_t)((d.nsites + 255) / 256)),
+                               dim3(256), 0, st, dp, (uint8_t*)G.arena);
         } else if (is_ising(ch.group)) {
             launch_ising_init(ch.group, G, d, dp, st);
         } else {
@@ -964,6 +1138,14 @@ int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
     if (rc) return rc;
     if (ch.group == kIsingWolff) {  // natural order, int8 +-1
         VG_HIP(hipMemcpyAsync((uint8_t*)G.arena + G.host[ch.index].off[0], spins, ch.n_sites,
+                              hipMemcpyHostToDevice, st));
+        VG_HIP(hipStreamSynchronize(st));
+        return VG_OK;
+    }
+    if (ch.group == kIsingGraph) {  // colour-sorted positions, 1 = Up
+        std::vector<uint8_t> v(ch.n_sites);
+        for (uint64_t i = 0; i < ch.n_sites; ++i) v[ch.graph_pos[i]] = spins[i] > 0 ? 1 : 0;
+        VG_HIP(hipMemcpyAsync((uint8_t*)G.arena + G.host[ch.index].off[0], v.data(), ch.n_sites,
                               hipMemcpyHostToDevice, st));
         VG_HIP(hipStreamSynchronize(st));
         return VG_OK;
@@ -1006,6 +1188,14 @@ int vg_batch_get_ising(vg_batch* batch, uint32_t chain, int8_t* spins) {
         VG_HIP(hipStreamSynchronize(st));
         return VG_OK;
     }
+    if (ch.group == kIsingGraph) {
+        std::vector<uint8_t> v(ch.n_sites);
+        VG_HIP(hipMemcpyAsync(v.data(), (const uint8_t*)G.arena + G.host[ch.index].off[0],
+                              ch.n_sites, hipMemcpyDeviceToHost, st));
+        VG_HIP(hipStreamSynchronize(st));
+        for (uint64_t i = 0; i < ch.n_sites; ++i) spins[i] = v[ch.graph_pos[i]] ? 1 : -1;
+        return VG_OK;
+    }
     int8_t* tmp = nullptr;
     VG_HIP(hipMalloc(&tmp, ch.n_sites));
     if (is_bits(ch.group))
@@ -1036,6 +1226,16 @@ int vg_batch_set_heisenberg(vg_batch* batch, uint32_t chain, const double* xyz) 
     VG_HIP(hipSetDevice(batch->ctx->device));
     rc = upload_descriptors(batch, ch.group);
     if (rc) return rc;
+    if (ch.group == kHeisGraph) {  // x, y, z planes over the colour-sorted positions
+        const uint64_t n = ch.n_sites;
+        std::vector<float> v(3 * n);
+        for (uint64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) v[a * n + ch.graph_pos[i]] = (float)xyz[3 * i + a];
+        VG_HIP(hipMemcpyAsync((uint8_t*)G.arena + G.host[ch.index].off[0], v.data(),
+                              v.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        VG_HIP(hipStreamSynchronize(st));
+        return VG_OK;
+    }
     double* tmp = nullptr;
     const uint64_t bytes = ch.n_sites * 3 * sizeof(double);
     VG_HIP(hipMalloc(&tmp, bytes));
@@ -1063,6 +1263,16 @@ int vg_batch_get_heisenberg(vg_batch* batch, uint32_t chain, double* xyz) {
     VG_HIP(hipSetDevice(batch->ctx->device));
     rc = upload_descriptors(batch, ch.group);
     if (rc) return rc;
+    if (ch.group == kHeisGraph) {
+        const uint64_t n = ch.n_sites;
+        std::vector<float> v(3 * n);
+        VG_HIP(hipMemcpyAsync(v.data(), (const uint8_t*)G.arena + G.host[ch.index].off[0],
+                              v.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        VG_HIP(hipStreamSynchronize(st));
+        for (uint64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) xyz[3 * i + a] = (double)v[a * n + ch.graph_pos[i]];
+        return VG_OK;
+    }
     double* tmp = nullptr;
     const uint64_t bytes = ch.n_sites * 3 * sizeof(double);
     VG_HIP(hipMalloc(&tmp, bytes));

@@ -54,6 +54,15 @@ __global__ void heis_init(const ChainDev*, float*);
 __global__ void heis_scatter(const ChainDev*, float*, const double*, uint64_t);
 __global__ void heis_gather(const ChainDev*, const float*, double*, uint64_t);
 __global__ void zero_i64(int64_t*, uint64_t);
+// Graph chains (an arbitrary exchange matrix): one pass per colour class,
+// args (chains, block -> chain map, arena, partials, colour, sweep, total
+// blocks, passes per sweep); initial states.
+__global__ void graph_ising_pass(const ChainDev*, const uint32_t*, uint8_t*, int64_t*, uint32_t,
+                                 uint32_t, uint32_t, uint32_t);
+__global__ void graph_heis_pass(const ChainDev*, const uint32_t*, uint8_t*, int64_t*, uint32_t,
+                                uint32_t, uint32_t, uint32_t);
+__global__ void graph_ising_init(const ChainDev*, uint8_t*);
+__global__ void graph_heis_init(const ChainDev*, uint8_t*);
 }  // namespace vg
 
 using vg::ChainDev;
@@ -91,13 +100,17 @@ enum GroupKind {
     kIsingWolff = 7,  // Wolff cluster steps, any lattice, natural site order
     kIsingBccBits = 8,  // bcc, 1 bit per site
     kIsingFccBits = 9,  // fcc, 1 bit per site
-    kNumGroups = 10
+    kIsingGraph = 10,   // arbitrary exchange matrix, int8 Ising, colour classes
+    kHeisGraph = 11,    // arbitrary exchange matrix, fp32 Heisenberg
+    kNumGroups = 12
 };
 
 inline bool is_ising(int g) {
     return g == kIsingSc || g == kIsingBcc || g == kIsingBits || g == kIsingFcc ||
-           g == kIsingWolff || g == kIsingBccBits || g == kIsingFccBits;
+           g == kIsingWolff || g == kIsingBccBits || g == kIsingFccBits || g == kIsingGraph;
 }
+// Groups of chains whose exchange is a matrix (VG_LATTICE_GRAPH).
+inline bool is_graph(int g) { return g == kIsingGraph || g == kHeisGraph; }
 // Groups whose planes hold one bit per site.
 inline bool is_bits(int g) {
     return g == kIsingBits || g == kIsingBccBits || g == kIsingFccBits;
@@ -107,15 +120,17 @@ inline bool is_zmarch(int g) {
     return g == kIsingSc || g == kIsingBits || g == kIsingBcc || g == kIsingFcc ||
            g == kIsingBccBits || g == kIsingFccBits;
 }
-// Colours (planes, passes per sweep) of a group's chains; Wolff chains
-// have no colour passes.
+// Colours (planes, passes per sweep) of a lattice group's chains; Wolff
+// chains have no colour passes; graph groups keep theirs in Group::npass.
 inline uint32_t colours_of(int g) {
-    return g == kIsingWolff ? 0u
-                            : (g == kIsingFcc || g == kHeisFcc || g == kIsingFccBits ? 4u : 2u);
+    return g == kIsingWolff || is_graph(g)
+               ? 0u
+               : (g == kIsingFcc || g == kHeisFcc || g == kIsingFccBits ? 4u : 2u);
 }
 
 inline int group_of(const vg_chain_desc& d) {
     const int k = d.lattice.kind;
+    if (k == VG_LATTICE_GRAPH) return d.spin == VG_SPIN_HEISENBERG ? kHeisGraph : kIsingGraph;
     if (d.proposal == VG_PROPOSAL_WOLFF) return kIsingWolff;
     if (d.spin == VG_SPIN_HEISENBERG)
         return k == VG_LATTICE_FCC ? kHeisFcc : (k == VG_LATTICE_BCC ? kHeisBcc : kHeisSc);
@@ -143,6 +158,7 @@ struct Group {
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
     uint32_t prio = 0;  // z-marching Ising passes: every workgroup resident at once (wave priorities on)
     uint32_t tie_cap = 64;  // z-marching Ising passes: tie records queued per wave (at most 64)
+    uint32_t npass = 0;     // graph groups: passes per sweep (the most colours of a member)
     void* arena = nullptr;
     uint64_t arena_bytes = 0;
     void* partials = nullptr;  // [sweep][colour][block][quantity]
@@ -163,7 +179,14 @@ struct ChainHost {
     double field_dir[3] = {0.0, 0.0, 1.0};
     int group = 0;
     uint32_t index = 0;  // index in its group
+    // Graph chains: sum of the matrix's diagonal (a constant -J_ii in every
+    // e_i) and the position of each site in the colour-sorted order.
+    double diag = 0.0;
+    std::vector<uint32_t> graph_pos;
 };
+
+// Passes per sweep of group g.
+inline uint32_t passes_of(const Group& G, int g) { return is_graph(g) ? G.npass : colours_of(g); }
 
 }  // namespace vgi
 
@@ -196,6 +219,19 @@ struct vg_batch {
 
 namespace vgi {
 
+// A graph chain's matrix prepared for the graph passes (graph_host.hip).
+struct GraphBuild {
+    uint32_t ncol = 0;                  // colour classes
+    uint32_t max_class = 0;             // sites of the largest class
+    std::vector<uint32_t> cstart;       // class starts, ncol + 1 (positions)
+    std::vector<uint32_t> site_of_pos;  // positions sorted by (colour, site id)
+    std::vector<uint32_t> pos_of_site;
+    std::vector<uint32_t> ptr, nbr;     // merged rows by position; neighbours as positions
+    std::vector<double> a, w;           // J_ij; J_ij + J_ji for lower-colour neighbours, else 0
+    double diag = 0.0;                  // sum of the diagonal entries
+};
+int build_graph(const vg_csr& m, GraphBuild& out);
+
 int validate_desc(const vg_chain_desc& d);
 int coordination(int kind);
 void refresh_params(vg_batch* B, uint32_t i);
@@ -223,6 +259,6 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d);
 // Elements (bytes for Ising, floats for Heisenberg) of one colour of a chain.
 uint64_t colour_elems(int g, const ChainDev& d);
 // Bytes of one element in group g's arena.
-inline uint64_t elem_bytes(int g) { return is_ising(g) ? 1 : 4; }
+inline uint64_t elem_bytes(int g) { return is_ising(g) || is_graph(g) ? 1 : 4; }
 
 }  // namespace vgi

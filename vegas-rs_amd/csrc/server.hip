@@ -488,6 +488,9 @@ void worker_main(vg_server* S) {
 int validate_job(const vg_job_desc& j) {
     int rc = validate_desc(j.chain);
     if (rc) return rc;
+    if (j.chain.lattice.kind == VG_LATTICE_GRAPH)
+        return fail(VG_ERR_NOT_IMPLEMENTED,
+                    "the job server runs lattice chains; graph chains run in batches");
     if (j.chain.proposal == VG_PROPOSAL_WOLFF)
         return fail(VG_ERR_NOT_IMPLEMENTED, "the job server runs Metropolis chains; Wolff chains run in batches");
     if (j.n_stages == 0 || !j.stages) return fail(VG_ERR_NO_STEPS, "a job needs at least one stage");

@@ -81,6 +81,8 @@ enum Purpose : uint32_t {
     kHeisInit = 4,       // initial spin stream, one position per stream
     kWolffBond = 5,      // Wolff: a bond's activation draw, keyed by its lower site and slot
     kWolffSeed = 6,      // Wolff: the cluster's seed site, one call per step
+    kGraphIsing = 8,     // graph chains, Ising sweep: acceptance (word 0), proposal (word 1)
+    kGraphInit = 9,      // graph chains, Ising initial spin
     kHeisStream = 16,    // Heisenberg sweep: proposal call k of a site uses purpose 16 + k
 };
 constexpr uint32_t kHeisMaxCalls = 16;
@@ -149,6 +151,15 @@ struct ChainDev {
     // a bond is active iff its 32-bit draw < wolff_thr, or always.
     uint64_t nsites;
     uint32_t wolff_thr, wolff_all;
+    // Graph chains (byte offsets in the arena): off[0] spins (Ising 1 byte
+    // per position; Heisenberg x, y, z planes of nsites floats), off[1]
+    // merged-row offsets (u32, nsites + 1), off[2] neighbour positions
+    // (u32), off[3] couplings a (f64 Ising, f32 Heisenberg), goff[0]
+    // lower-colour couplings w, goff[1] site id of each position (u32),
+    // goff[2] colour class starts (u32, ncol + 1).  Ising parameters in f64:
+    // 1/T, |h| (0 without Zeeman) and the field's sign.
+    uint64_t goff[3];
+    double gbeta, ghm, ghz;
 };
 
 constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass
@@ -202,6 +213,38 @@ VG_HD float exp_nonpos(float x) {
     } s;
     s.u = (uint32_t)e << 23;
     return p * s.f;
+}
+
+// e^x for x <= 0 in f64, the graph chains' acceptance: Cody and Waite's
+// reduction x = n ln2 + r (ln2 split so n * hi is exact, |r| <= ln2 / 2),
+// then the degree-13 Taylor polynomial of e^r (coefficients 1/k! rounded to
+// nearest), times 2^n; 0 below -708, where 2^n would leave the normal range.
+// Every step is one IEEE operation or an fma, so host and device agree.
+VG_HD double exp_f64_nonpos(double x) {
+    if (x < -708.0) return 0.0;
+    const double n = __builtin_rint(x * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(n, -0x1.62e42fee00000p-1, x);
+    r = __builtin_fma(n, -0x1.a39ef35793c76p-33, r);
+    double p = 0x1.6124613a86d09p-33;
+    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0x1.0000000000000p-1);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    union {
+        uint64_t u;
+        double d;
+    } s;
+    s.u = (uint64_t)((int64_t)n + 1023) << 52;
+    return p * s.d;
 }
 
 // Correctly rounded fp32 square root for normal x > 0.  The hardware
