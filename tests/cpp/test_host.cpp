@@ -164,6 +164,49 @@ static void exchange_ground_states() {
     EXPECT(same);
 }
 
+// Exchange::new(matrix) and Exchange::from_lattice's one-sided edge filter
+// (energy.rs:170-187).
+static void exchange_matrices() {
+    // TriMat::to_csr: rows in order, columns ascending, duplicates summed
+    const CsrMatrix m = CsrMatrix::from_triplets(3, {2, 0, 0, 2, 1}, {0, 2, 1, 0, 1},
+                                                 {1.0, 2.0, 3.0, 4.0, 5.0});
+    EXPECT((m.row_ptr == std::vector<uint64_t>{0, 2, 3, 4}));
+    EXPECT((m.col == std::vector<uint64_t>{1, 2, 1, 0}));
+    EXPECT((m.value == std::vector<double>{3.0, 2.0, 5.0, 5.0}));
+    // the unfiltered lattice matrix is the stencil's: equal site energies
+    const Lattice l = Lattice::bcc().expand(3, 2, 4).drop_y();
+    const auto stencil = Exchange::from_lattice(0.7, l);
+    const auto mat = Exchange::from_matrix(lattice_matrix(l, 0.7, EdgeFilter::None));
+    EXPECT(mat.lattice() == nullptr && mat.matrix() != nullptr && stencil.matrix() == nullptr);
+    std::mt19937_64 rng(5);
+    const auto st = State<HeisenbergSpin>::rand_with_size(rng, l.num_sites());
+    const auto th = Thermostat<HeisenbergSpin>(1.0, Field<HeisenbergSpin>::zero());
+    bool same = true;
+    for (size_t i = 0; i < st.len(); ++i)
+        same = same && close(stencil.energy(th, st, i), mat.energy(th, st, i), 1e-12);
+    EXPECT(same);
+    EXPECT(describe(mat).exchange == 1.0 && describe(stencil).exchange == 0.7);
+    // One-sided filter on the CLI's 10x10x1 sample: the periodic wrap edges
+    // run from the last cell to the first (source > target) and are
+    // dropped, so the ground state is that of open x and y boundaries:
+    // 180 bonds, Compound E/N = -2 * 180 / 100 = -3.6 (unfiltered: -4).
+    const Lattice c1 = Lattice::sc().expand(10, 10, 1).drop_z();
+    const auto filt = hamiltonian(Exchange::from_lattice(1.0, c1, EdgeFilter::SourceNotAfterTarget),
+                                  Zeeman<IsingSpin>());
+    const auto full = hamiltonian(Exchange::from_lattice(1.0, c1, EdgeFilter::None), Zeeman<IsingSpin>());
+    const auto up = State<IsingSpin>::up_with_size(100);
+    const auto thi = Thermostat<IsingSpin>(1.0, Field<IsingSpin>::zero());
+    EXPECT(close(filt.total_energy(thi, up) / 100.0, -3.6));
+    EXPECT(close(full.total_energy(thi, up) / 100.0, -4.0));
+    const auto open = lattice_matrix(Lattice::sc().expand(10, 10, 1).drop_x().drop_y().drop_z(), 1.0,
+                                     EdgeFilter::None);
+    const auto one = lattice_matrix(c1, 1.0, EdgeFilter::SourceNotAfterTarget);
+    EXPECT(one.row_ptr == open.row_ptr && one.col == open.col && one.value == open.value);
+    EXPECT(throws_kind([] { lattice_matrix(Lattice::bcc().expand_all(2), 1.0,
+                                           EdgeFilter::SourceNotAfterTarget); },
+                       ErrorKind::NotImplemented));
+}
+
 static void accumulator_moments() {
     Accumulator a;
     for (double v : {1.0, 2.0, 3.0}) a.collect(v);
@@ -393,6 +436,42 @@ static void integrator_step_adapter() {
                  h, hs);
         },
         ErrorKind::NotImplemented));
+}
+
+// A Machine over Exchange::new(matrix) runs as a graph chain: its per-sweep
+// energy is the reference's total_energy of the downloaded state, and the
+// trajectory is the oracle's graph restatement's.
+static void matrix_machine_runs() {
+    const Lattice c1 = Lattice::sc().expand(10, 10, 1).drop_z();
+    const auto h = hamiltonian(Exchange::from_lattice(1.0, c1, EdgeFilter::SourceNotAfterTarget),
+                               Zeeman<IsingSpin>());
+    const Thermostat<IsingSpin> th(2.0, Field<IsingSpin>(IsingSpin::up(), 0.2));
+    std::mt19937_64 rng(21), rng_copy(21);
+    const uint64_t seed = rng_copy();
+    auto st = State<IsingSpin>::up_with_size(100);
+    Chain<IsingSpin> c(lattice_of(h, 100), MetropolisFlipIntegrator::proposal, describe(h), seed,
+                       default_context(), h.matrix());
+    c.upload(st);
+    c.set_thermostat(th);
+    std::vector<SweepRecord> rec;
+    c.run(7, rec);
+    const State<IsingSpin> got = c.download();
+    EXPECT(close(rec.back().energy, h.total_energy(th, got), 1e-9));
+    const CsrMatrix* m = h.matrix();
+    or_graph* g = or_graph_create(m->n, m->row_ptr.data(), m->col.data(), m->value.data());
+    or_model om{};
+    om.terms = 3;
+    om.compound = 1;
+    om.exchange = 1.0;
+    om.temperature = 2.0;
+    om.field = 0.2;
+    om.field_dir[2] = 1.0;
+    std::vector<int8_t> s(100, 1);
+    or_graph_ising_sweeps(g, &om, seed, 0, 7, s.data(), nullptr);
+    or_graph_destroy(g);
+    bool same = true;
+    for (size_t i = 0; i < 100; ++i) same &= (int8_t)got.at(i).value() == s[i];
+    EXPECT(same);
 }
 
 static void heisenberg_machine_runs() {
@@ -743,6 +822,7 @@ int main(int argc, char** argv) {
                  {"field_and_thermostat", field_and_thermostat},
                  {"energy_unit_tests", energy_unit_tests},
                  {"exchange_ground_states", exchange_ground_states},
+                 {"exchange_matrices", exchange_matrices},
                  {"accumulator_moments", accumulator_moments},
                  {"program_schedules", program_schedules},
                  {"program_validation", program_validation},
@@ -754,6 +834,7 @@ int main(int argc, char** argv) {
                  {"state_sensor_snapshots", state_sensor_snapshots},
                  {"integrator_step_adapter", integrator_step_adapter},
                  {"heisenberg_machine_runs", heisenberg_machine_runs},
+                 {"matrix_machine_runs", matrix_machine_runs},
                  {"job_server_programs", job_server_programs}};
     }
     for (auto& t : tests) {

@@ -10,9 +10,11 @@
 //   default sum of site energies (energy.rs:55-59, 248-257).
 #pragma once
 
+#include <memory>
 #include <utility>
 
 #include "lattice.hpp"
+#include "matrix.hpp"
 #include "thermostat.hpp"
 
 namespace vegas {
@@ -51,6 +53,7 @@ class Gauge {
         d.gauge = value_;
     }
     const Lattice* lattice() const { return nullptr; }
+    const CsrMatrix* matrix() const { return nullptr; }
 
   private:
     double value_;
@@ -79,6 +82,7 @@ class UniaxialAnisotropy {
         vector_of(reference_, d.anisotropy_axis);
     }
     const Lattice* lattice() const { return nullptr; }
+    const CsrMatrix* matrix() const { return nullptr; }
 
   private:
     S reference_;
@@ -99,19 +103,42 @@ class Zeeman {
     }
     void describe(vg_hamiltonian& d) const { d.terms |= VG_TERM_ZEEMAN; }
     const Lattice* lattice() const { return nullptr; }
+    const CsrMatrix* matrix() const { return nullptr; }
 };
 
-// Exchange (energy.rs:165-214): e_i = -J sum_j s_i.s_j over the lattice
-// bonds (every neighbour slot, duplicates counted, as the CSR built by
-// Exchange::from_lattice sums them); total = sum_i e_i / 2.
+// Exchange (energy.rs:165-214): e_i = -sum_j J_ij s_i.s_j over row i of the
+// coupling matrix; total = sum_i e_i / 2.  Built from a lattice (every
+// neighbour slot with J, duplicates counted, as the CSR built by
+// Exchange::from_lattice sums them; the engine sweeps it by its sc/bcc/fcc
+// stencil) or from any matrix (Exchange::new, energy.rs:170-172; the engine
+// sweeps it as a graph chain, by colour classes).
 class Exchange {
   public:
     static Exchange from_lattice(double exchange, const Lattice& lattice) {
         return Exchange(exchange, lattice);
     }
+    // Exchange::from_lattice with an explicit edge filter (matrix.hpp):
+    // None is from_lattice above; SourceNotAfterTarget applies the
+    // reference's `source <= target` filter to the unit cell's edges and runs
+    // the resulting matrix.
+    static Exchange from_lattice(double exchange, const Lattice& lattice, EdgeFilter filter) {
+        if (filter == EdgeFilter::None) return Exchange(exchange, lattice);
+        return from_matrix(lattice_matrix(lattice, exchange, filter));
+    }
+    // Exchange::new(CsMat<f64>) (energy.rs:170-172): the rows as given.
+    static Exchange from_matrix(CsrMatrix matrix) {
+        Exchange e(1.0, Lattice::sc());
+        e.matrix_ = std::make_shared<const CsrMatrix>(std::move(matrix));
+        return e;
+    }
     template <class S>
     double energy(const Thermostat<S>&, const State<S>& st, size_t i) const {
         double e = 0.0;
+        if (matrix_) {
+            for (uint64_t k = matrix_->row_ptr[i]; k < matrix_->row_ptr[i + 1]; ++k)
+                e += -matrix_->value[k] * st.at(i).dot(st.at((size_t)matrix_->col[k]));
+            return e;
+        }
         for (int64_t j : lattice_.neighbours(i))
             if (j >= 0) e += -j_ * st.at(i).dot(st.at((size_t)j));
         return e;
@@ -122,14 +149,16 @@ class Exchange {
     }
     void describe(vg_hamiltonian& d) const {
         d.terms |= VG_TERM_EXCHANGE;
-        d.exchange = j_;
+        d.exchange = matrix_ ? 1.0 : j_;  // a matrix carries its own couplings
     }
-    const Lattice* lattice() const { return &lattice_; }
+    const Lattice* lattice() const { return matrix_ ? nullptr : &lattice_; }
+    const CsrMatrix* matrix() const { return matrix_.get(); }
 
   private:
     Exchange(double j, Lattice l) : j_(j), lattice_(l) {}
     double j_;
     Lattice lattice_;
+    std::shared_ptr<const CsrMatrix> matrix_;
 };
 
 // Compound (energy.rs:221-257): e_i = a_i + b_i, default total.
@@ -151,6 +180,7 @@ class Compound {
         d.compound = 1;
     }
     const Lattice* lattice() const { return a_.lattice() ? a_.lattice() : b_.lattice(); }
+    const CsrMatrix* matrix() const { return a_.matrix() ? a_.matrix() : b_.matrix(); }
 
   private:
     A a_;

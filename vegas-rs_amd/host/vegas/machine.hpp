@@ -52,15 +52,23 @@ inline vg_ctx* default_context() {
 template <class S>
 class Chain {
   public:
+    // matrix: the Exchange's coupling matrix when it was built from one
+    // (the chain is then a graph chain; lattice is ignored).
     Chain(const vg_lattice& lattice, int proposal, const vg_hamiltonian& h, uint64_t seed,
-          vg_ctx* ctx = default_context()) {
+          vg_ctx* ctx = default_context(), const CsrMatrix* matrix = nullptr) {
         vg_chain_desc d{};
         d.lattice = lattice;
         d.spin = S::kind;
         d.proposal = proposal;
         d.hamiltonian = h;
         d.seed = seed;
-        check(vg_batch_create(ctx, 1, &d, &batch_));
+        if (matrix) {
+            d.lattice = vg_lattice{VG_LATTICE_GRAPH, 0, 0, 0, 0};
+            const vg_csr m = matrix->view();
+            check(vg_batch_create_csr(ctx, 1, &d, &m, &batch_));
+        } else {
+            check(vg_batch_create(ctx, 1, &d, &batch_));
+        }
         uint64_t n = 0;
         check(vg_batch_num_sites(batch_, 0, &n));
         n_ = n;
@@ -135,6 +143,7 @@ inline State<HeisenbergSpin> Chain<HeisenbergSpin>::download() const {
 template <class H>
 vg_lattice lattice_of(const H& h, size_t n) {
     if (const Lattice* l = h.lattice()) return l->descriptor();
+    if (h.matrix()) return vg_lattice{VG_LATTICE_GRAPH, 0, 0, 0, 0};
     if (n % 2 != 0)
         throw Error(ErrorKind::UnsupportedLattice,
                     "a Hamiltonian without Exchange needs an even number of sites");
@@ -149,7 +158,8 @@ vg_lattice lattice_of(const H& h, size_t n) {
 // Integrator::step (integrator.rs:40-49) for a host State: one sweep.
 template <class I, class Rng, class H, class S>
 State<S> step(const I&, Rng& rng, const Thermostat<S>& th, const H& h, const State<S>& st) {
-    Chain<S> c(lattice_of(h, st.len()), I::proposal, describe(h), rng());
+    Chain<S> c(lattice_of(h, st.len()), I::proposal, describe(h), rng(), default_context(),
+               h.matrix());
     c.upload(st);
     c.set_thermostat(th);
     std::vector<SweepRecord> r;
@@ -195,7 +205,8 @@ class Machine {
         if (!chain_) {
             chain_ = std::make_unique<Chain<S>>(lattice_of(hamiltonian_, state_.len()),
                                                 I::proposal, describe(hamiltonian_),
-                                                (uint64_t)rng());
+                                                (uint64_t)rng(), default_context(),
+                                                hamiltonian_.matrix());
             chain_->upload(state_);
         }
         chain_->set_thermostat(thermostat_);

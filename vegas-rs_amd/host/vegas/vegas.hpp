@@ -12,6 +12,7 @@
 #include "instrument.hpp"
 #include "lattice.hpp"
 #include "machine.hpp"
+#include "matrix.hpp"
 #include "parquet.hpp"
 #include "program.hpp"
 #include "server.hpp"
