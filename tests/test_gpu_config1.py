@@ -4,18 +4,28 @@ h = 0; Ising spins i.i.d. from the seed; flip proposals): Relax 1000 sweeps
 at T = 4.0, then CoolDown 4.0 -> 0.1 by 0.1 (39 temperatures) with 1000
 relax + 20000 measured sweeps each -- 819,000 sweeps per chain.
 
-An ensemble of independent chains runs the whole program on the engine (one
-batch, chain seeds 42 + k) and another runs it on the oracle's restatement of
-the reference's sequential MetropolisFlipIntegrator (seeds 1000 + k, one
-thread per chain).  For every one of the 39 measured stages the ensemble
-means of the per-chain stage values <E>/N, <|M|>/N, C = Var(E)/(N T^2) and
-chi = Var(|M|)/(N T) (StatSensor, instrument.rs:113-123) agree within 3
-sigma, sigma from the spread of each ensemble's chains.  Averaging over
-chains covers the low temperatures too, where a single 2D chain may spend
-whole stages in a stripe state.  The seeds are fixed, so every run is the
-same; every z is printed."""
+An ensemble of 512 independent chains runs the whole program on the engine
+(one batch, chain seeds 42 + k) and another of 512 runs it on the oracle's
+restatement of the reference's sequential MetropolisFlipIntegrator (seeds
+1000 + k, one thread per chain).  For every one of the 39 measured stages
+the ensemble means of the per-chain stage values <E>/N, <|M|>/N,
+C = Var(E)/(N T^2) and chi = Var(|M|)/(N T) (StatSensor,
+instrument.rs:113-123) agree within 3 sigma, sigma from the spread of each
+ensemble's chains, and the engine's agree within 3 sigma with the exact
+values of the 10x10 torus (row transfer matrix, tests/golden/
+make_ising10x10_exact.py).  Averaging over chains covers the temperatures
+just below T_c and the low ones, where a single 2D chain may spend long
+stretches in a two-domain state.  There the per-chain stage means are heavy
+tailed: at T = 2.1 the means of 64-chain blocks of the sequential
+restatement scatter 1.4 times more than their standard errors say, and a
+128-chain block (seeds 1000-1127) sat 4.5 of its sigmas from the exact
+value while 512 chains sit 1.0 from it -- hence ensembles of 512.  The
+seeds are fixed, so every run is the same; every z is printed."""
+import json
+import math
 import os
 from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -24,8 +34,9 @@ from _stats import z_score
 
 pytestmark = pytest.mark.gpu
 
-GPU_CHAINS = 64
-CPU_CHAINS = 32
+GPU_CHAINS = 512
+CPU_CHAINS = 512
+EXACT = Path(__file__).resolve().parent / "golden" / "ising10x10_exact.json"
 N = 100
 
 
@@ -37,6 +48,15 @@ def _stage_values(e, m, T):
     """Per-chain <E>/N, <|M|>/N, C, chi of one stage (last axis = sweeps)."""
     return np.stack([e.mean(-1) / N, m.mean(-1) / N, e.var(-1) / (N * T * T),
                      m.var(-1) / (N * T)], axis=-1)
+
+
+def _resolution(q, T, chains, sweeps=20000):
+    """Change of the ensemble mean of quantity q (E/N, |M|/N, C, chi) made by
+    one single-spin excitation of the ground state (E_obs up by 16, |M| down
+    by 2) lasting one sweep in one chain."""
+    per_chain = [16.0 / N / sweeps, 2.0 / N / sweeps, 256.0 / sweeps / (N * T * T),
+                 4.0 / sweeps / (N * T)][q]
+    return per_chain / chains
 
 
 def _gpu_ensemble(vg, ctx, stages):
@@ -86,20 +106,33 @@ def test_config1_cli_program_ensemble(vg, orc, ctx):
         cpu = np.stack(list(ex.map(lambda k: _cpu_chain(orc, stages, 1000 + k),
                                    range(CPU_CHAINS))))
     labels = ["E/N", "|M|/N", "C", "chi"]
-    zs, fails = [], []
+    exact = json.loads(EXACT.read_text())["rows"]
+    assert [round(r["T"], 9) for r in exact] == [round(T, 9) for T in temps]
+    zs, zx, fails = [], [], []
     for j, T in enumerate(temps):
         for q, lab in enumerate(labels):
             a, b = gpu[:, j, q], cpu[:, j, q]
             ma, ea = float(a.mean()), float(a.std(ddof=1) / np.sqrt(len(a)))
             mb, eb = float(b.mean()), float(b.std(ddof=1) / np.sqrt(len(b)))
-            z = z_score(ma, ea, mb, eb) if (ea > 0 or eb > 0 or ma != mb) else 0.0
+            # Near the ground state (T <= 0.4) excitations are so rare that
+            # an ensemble may see none: its spread is then 0 and cannot
+            # resolve deviations below the weight of one excitation in the
+            # ensemble mean.  sigma gets a floor of three such events: a
+            # single flip (dE_obs = 16, dM = 2) lasting one sweep in one chain.
+            fl = 3.0 * _resolution(q, T, len(a))
+            z = z_score(ma, math.hypot(ea, fl), mb, eb)
+            ex = exact[j][lab]
+            zex = z_score(ma, math.hypot(ea, fl), ex, 0.0)
             zs.append(z)
+            zx.append(zex)
             print(f"T={T:.2f} {lab:6s} engine {ma:.6f} +- {ea:.6f}  sequential {mb:.6f} +- "
-                  f"{eb:.6f}  z={z:.2f}", flush=True)
-            if not z < 3.0:
-                fails.append((T, lab, ma, ea, mb, eb, z))
-    print(f"config1: {len(zs)} comparisons, max z {max(zs):.2f}, mean z {np.mean(zs):.2f}")
-    assert len(zs) == 156
+                  f"{eb:.6f}  exact {ex:.6f}  z={z:.2f}  z_exact={zex:.2f}", flush=True)
+            if not (z < 3.0 and zex < 3.0):
+                fails.append((T, lab, ma, ea, mb, eb, ex, z, zex))
+    print(f"config1: {len(zs)} comparisons with the sequential chains (max z {max(zs):.2f}, "
+          f"mean {np.mean(zs):.2f}) and {len(zx)} with the exact values (max z {max(zx):.2f}, "
+          f"mean {np.mean(zx):.2f})")
+    assert len(zs) == len(zx) == 156
     assert not fails, fails
     # the ground state is reached at the end of the cooling (E/N = -4 with
     # the Compound's double-counted exchange)

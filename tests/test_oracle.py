@@ -389,6 +389,37 @@ def test_checkerboard_heisenberg_single_spin_closed_forms(orc):
     assert z_score(mean, err, num / den, 0.0) < 4
 
 
+def test_ising_10x10_matches_transfer_matrix(orc):
+    """The reference's CPU-runnable lattice (sc 10x10x1, x and y periodic,
+    J = 1, h = 0; Compound energy): long runs of the checkerboard
+    restatement and of the sequential restatement of MetropolisFlipIntegrator
+    reproduce the exact <E>/N and <|M|>/N of the 10x10 torus (row transfer
+    matrix, tests/golden/make_ising10x10_exact.py) within 4 sigma, at a
+    temperature above, near and below T_c = 2.269."""
+    import json
+    from pathlib import Path
+    rows = json.loads((Path(__file__).resolve().parent / "golden" /
+                       "ising10x10_exact.json").read_text())["rows"]
+    L = orc.lattice("sc", 10, 10, 1, pbc=(1, 1, 0))
+    for T in (3.0, 2.3, 1.9):
+        ex = min(rows, key=lambda r: abs(r["T"] - T))   # schedule temperatures are f64 sums
+        assert abs(ex["T"] - T) < 1e-9
+        M = orc.model(terms=("exchange", "zeeman"), temperature=T)
+        s = orc.ising_init(L, 3)
+        sums = orc.ising_sweeps(L, M, 3, 0, 150000, s)[2000:]
+        e, m = -2.0 * sums[:, 0] / 100, np.abs(sums[:, 3]) / 100
+        for val, want in ((binned(e), ex["E/N"]), (binned(m), ex["|M|/N"])):
+            assert z_score(*val, want, 0.0) < 4, (T, val, want)
+        seq = orc.Sequential(L, M, 4)
+        seq.set_thermostat(T)
+        seq.set_state(orc.ising_init(L, 4))
+        seq.sweeps(2000, observe=False)
+        e, m = seq.sweeps(150000)
+        seq.close()
+        for val, want in ((binned(e / 100), ex["E/N"]), (binned(m / 100), ex["|M|/N"])):
+            assert z_score(*val, want, 0.0) < 4, ("sequential", T, val, want)
+
+
 def test_heisenberg_exchange_dimer_closed_form(orc):
     """Two unit spins joined by one exchange bond (sc 2x1x1, open axes):
     with H = -J s1.s2 the bond's distribution is p(c) ~ exp(J c / T) on
