@@ -272,6 +272,39 @@ int vg_batch_set_timing(vg_batch* batch, int enabled);
 int vg_batch_pass_stats(const vg_batch* batch, int32_t spin, double* total_ms, uint64_t* launches);
 
 /* ------------------------------------------------------------------ */
+/* Allocation-free sweeps.  No counterpart in the reference: for callers */
+/* that cannot allocate device memory (code under HIP graph capture,     */
+/* tight serving loops, harnesses that forbid hipMalloc).  The sweeps of  */
+/* vg_batch_run_sums for one Metropolis Ising chain (int8 or bit-packed;  */
+/* sc, bcc or fcc; tile 4) on caller-owned device buffers, enqueued on    */
+/* `stream` (a hipStream_t; NULL = the null stream) of the calling        */
+/* thread's current device; the call returns once the work is enqueued.  */
+/*   spins: the chain's colour planes, plane c (c < 2, or < 4 on fcc) at  */
+/*     byte offset c * round_up(ly * lz * pitch, 256); row r = y + ly * z */
+/*     at r * pitch; position i of a row holds the colour-c site at       */
+/*     x = 2i + ((c + y + z) & 1) on sc, the sublattice-c site of cell    */
+/*     x = i on bcc and fcc; w = lx / 2 (sc) or lx positions.  int8: one  */
+/*     byte per position, 1 = Up, 0 = Down, pitch = round_up(w, 64).      */
+/*     Bit-packed: bit (i & 7) of byte i >> 3, pitch = round_up(2 *       */
+/*     ceil(w / 16), 16).  Padding bytes zero.  Updated in place.          */
+/*   sums: sweeps * 5 int64, per sweep {bond, 0, 0, mag, 0} as in          */
+/*     vg_sweep_sums (the batch path reports N as the fifth).             */
+/*   sweep0: the chain's sweep counter before the first sweep (Philox).   */
+/* The thermostat follows vg_batch_set_thermostat.  Descriptor, block map */
+/* and pass partials live in a 16 MiB device array that each GPU holds    */
+/* until the process ends.  A sweep's partials take colours x workgroups  */
+/* x 32 bytes (N / 512 bytes on sc, cells / 128 on fcc); a chain whose    */
+/* partials for one sweep do not fit is refused (VG_ERR_INVALID_ARGUMENT):*/
+/* every sc and bcc chain a batch accepts fits, fcc up to about 2 x 10^9  */
+/* cells.                                                                 */
+/* Calls on one GPU share that array: calls that overlap on one GPU race */
+/* (serialise them, e.g. on one stream).  Nothing needs clearing between */
+/* calls: a call writes all it reads.                                     */
+int vg_run_sums_noalloc(const vg_chain_desc* desc, double temperature, double field_magnitude,
+                        const double field_orientation[3], uint64_t sweep0, uint64_t sweeps,
+                        void* spins, int64_t* sums, void* stream);
+
+/* ------------------------------------------------------------------ */
 /* Job server: continuous batching of simulation jobs on one GPU.       */
 /* A job is a chain plus a program -- the stages a Program::run drives   */
 /* (Relax, CoolDown, HysteresisLoop, program.rs:97-336) -- and its      */

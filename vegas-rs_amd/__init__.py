@@ -155,6 +155,9 @@ for _name, _args in _SIGS.items():
     _f = getattr(_lib, _name)
     _f.argtypes = _args
     _f.restype = C.c_int
+_lib.vg_run_sums_noalloc.argtypes = [_P(ChainDesc), C.c_double, C.c_double, _P(C.c_double),
+                                     C.c_uint64, C.c_uint64, _vp, _vp, _vp]
+_lib.vg_run_sums_noalloc.restype = C.c_int
 _lib.vg_last_error.restype = C.c_char_p
 _lib.vg_version.restype = C.c_char_p
 _lib.vg_sums_scale.argtypes = [C.c_int32]
@@ -188,6 +191,16 @@ def philox(ctr, key) -> tuple:
     o = (C.c_uint32 * 4)()
     _lib.vg_philox4x32_10(c, k, o)
     return tuple(o)
+
+
+def run_sums_noalloc(desc: ChainDesc, temperature: float, field: float, field_dir,
+                     sweep0: int, sweeps: int, spins_ptr: int, sums_ptr: int,
+                     stream: int = 0) -> None:
+    """vg_run_sums_noalloc: sweeps of one Ising chain on caller-owned device
+    buffers (device addresses as ints), enqueued on `stream`."""
+    d = (C.c_double * 3)(*field_dir)
+    _check(_lib.vg_run_sums_noalloc(C.byref(desc), temperature, field, d, sweep0, sweeps,
+                                    _vp(spins_ptr), _vp(sums_ptr), _vp(stream)))
 
 
 def sums_scale(spin: int) -> float:

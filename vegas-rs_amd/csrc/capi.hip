@@ -471,6 +471,18 @@ void launch_fcc_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s, int
     }
 }
 
+// One colour pass of a z-marching Ising group (sc, bcc, fcc; int8 or bits).
+void launch_zmarch_pass(const Group& G, int g, hipStream_t st, uint32_t c, uint32_t s) {
+    switch (g) {
+        case kIsingBits: launch_ising_pass<true>(G, st, c, s); break;
+        case kIsingBcc: launch_bcc_pass<false>(G, st, c, s); break;
+        case kIsingBccBits: launch_bcc_pass<true>(G, st, c, s); break;
+        case kIsingFcc: launch_fcc_pass(G, st, c, s, 0); break;
+        case kIsingFccBits: launch_fcc_pass(G, st, c, s, 1); break;
+        default: launch_ising_pass<false>(G, st, c, s); break;
+    }
+}
+
 // Enqueues the sweeps of one block (nsw sweeps, every colour of every group,
 // then the per-chain reductions) on the batch stream.
 int enqueue_block(vg_batch* B, uint32_t nsw) {

@@ -249,6 +249,10 @@ uint32_t quantity_bytes(int group);
 // `slot` (0, or 1 for the second buffer of a batch).
 int upload_descriptors(vg_batch* B, int g, int slot = 0);
 int layout_work(vg_batch* B, int g, uint32_t tile);
+// Whether `grid` workgroups of group g's z-marching pass at this tile are all
+// resident at once (the falling wave priorities are used only then).
+bool all_resident(int g, uint32_t tile, uint32_t grid);
+void launch_zmarch_pass(const Group& G, int g, hipStream_t st, uint32_t c, uint32_t s);
 int enqueue_block(vg_batch* B, uint32_t nsw);
 int launch_block(vg_batch* B, uint32_t nsw);
 double sums_scale(int spin);
