@@ -252,13 +252,14 @@ int vg_batch_prepare(vg_batch* batch, uint64_t sweeps);
 int vg_batch_set_tile(vg_batch* batch, uint32_t tile);
 int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile);
 /* Times `trial_sweeps` sweeps with every tile (replays of graphs captured
- * beforehand, best of two) and keeps the fastest, for each of the six
- * groups.  The chains' spins and sweep counters are
+ * beforehand, three rounds over the tiles, best of each) and keeps the
+ * fastest, for each of the six groups.  The chains' spins and sweep counters are
  * restored after every trial, so tuning does not change any trajectory. */
 int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen_tile);
-/* Records (0..64, default 64) the per-wave tie queue of the Ising sc, bcc
+/* Records (0..128, default 128) the per-wave tie queue of the Ising sc, bcc
  * and fcc passes may hold before a wave settles further tied rows in the
- * lane that found them.  No counterpart in the reference; a test hook, so
+ * lane that found them (each pass holds at most its own queue: 128 records
+ * in the sc pass, 64 in the bcc and fcc passes).  No counterpart in the reference; a test hook, so
  * that the queue-overflow path can be checked against the oracle: results
  * do not depend on it.  Drops captured graphs. */
 int vg_batch_set_tie_capacity(vg_batch* batch, uint32_t records);

@@ -222,6 +222,30 @@ void or_ising_init(const or_lattice* L, uint64_t seed, int8_t* spins) {
     }
 }
 
+/* High byte of the acceptance draw of position pos (purpose IsingA).
+ * bcc and fcc: byte pos & 3 of word (pos & 15) >> 2 of the call keyed by
+ * pos >> 4.  sc (the multispin pass): the 32 positions of a 32-aligned
+ * segment share the calls keyed by 2 * (pos >> 5) and 2 * (pos >> 5) + 1;
+ * bit k of the byte is bit pos & 31 of word k & 3 of the first call
+ * (k < 4) or of the second (k >= 4). */
+static uint32_t bcc_fcc_high_byte(uint64_t seed, uint32_t pos, uint32_t row, uint64_t t,
+                                  uint32_t c) {
+    uint32_t ra[4];
+    draw(seed, pos >> 4, row, (uint32_t)t, wword(t, 0, c), ra);
+    return (ra[(pos & 15) >> 2] >> (8 * (pos & 3))) & 0xFFu;
+}
+
+static uint32_t sc_high_byte(uint64_t seed, uint32_t pos, uint32_t row, uint64_t t, uint32_t c) {
+    uint32_t r0[4], r1[4], v = 0;
+    draw(seed, 2 * (pos >> 5), row, (uint32_t)t, wword(t, 0, c), r0);
+    draw(seed, 2 * (pos >> 5) + 1, row, (uint32_t)t, wword(t, 0, c), r1);
+    for (int k = 0; k < 4; ++k) {
+        v |= ((r0[k] >> (pos & 31)) & 1u) << k;
+        v |= ((r1[k] >> (pos & 31)) & 1u) << (k + 4);
+    }
+    return v;
+}
+
 void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint64_t sweep0,
                      uint64_t nsweeps, int8_t* spins, int64_t* sums) {
     const uint64_t n = or_num_sites(L);
@@ -250,10 +274,10 @@ void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint
                 if ((always[b] >> S) & 1u) {
                     flip = 1;
                 } else {
-                    uint32_t ra[4], rb[4];
-                    draw(seed, pos >> 4, row, (uint32_t)t, wword(t, 0, c), ra);
+                    uint32_t rb[4];
                     draw(seed, pos >> 2, row, (uint32_t)t, wword(t, 1, c), rb);
-                    uint32_t hi8 = (ra[(pos & 15) >> 2] >> (8 * (pos & 3))) & 0xFFu;
+                    uint32_t hi8 = L->kind == 0 ? sc_high_byte(seed, pos, row, t, c)
+                                                : bcc_fcc_high_byte(seed, pos, row, t, c);
                     uint32_t lo24 = rb[pos & 3] >> 8;
                     uint32_t draw32 = (hi8 << 24) | lo24;
                     flip = draw32 < thr[b * OR_THR_STRIDE + S];
@@ -288,7 +312,7 @@ void or_ising_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint
 }
 
 /* The same sc checkerboard, written for speed rather than generality: the
- * CPU baseline's best-CPU line (one Philox call per 16 positions for the
+ * CPU baseline's best-CPU line (two Philox calls per 32 positions for the
  * high bytes, as the GPU does; rows of a colour shared among `threads`
  * OpenMP threads).  Every draw is the one or_ising_sweeps makes, so the
  * result equals it bit for bit for any thread count. */
@@ -313,9 +337,12 @@ void or_ising_sweeps_sc_fast(const or_lattice* L, const or_model* M, uint64_t se
                 const int64_t dym = y > 0 ? -sy : sy * (ly - 1), dyp = y + 1 < ly ? sy : -sy * (ly - 1);
                 const int64_t dzm = zz > 0 ? -sz : sz * (lz - 1), dzp = zz + 1 < lz ? sz : -sz * (lz - 1);
                 const uint32_t off = (c + y + zz) & 1u;
-                uint32_t ra[4] = {0, 0, 0, 0}, rp[4] = {0, 0, 0, 0};
+                uint32_t ra[8] = {0, 0, 0, 0, 0, 0, 0, 0}, rp[4] = {0, 0, 0, 0};
                 for (uint32_t pos = 0; pos < w; ++pos) {
-                    if ((pos & 15) == 0) draw(seed, pos >> 4, row, (uint32_t)t, wword(t, 0, c), ra);
+                    if ((pos & 31) == 0) {
+                        draw(seed, 2 * (pos >> 5), row, (uint32_t)t, wword(t, 0, c), ra);
+                        draw(seed, 2 * (pos >> 5) + 1, row, (uint32_t)t, wword(t, 0, c), ra + 4);
+                    }
                     if (M->proposal == 1 && (pos & 127) == 0)
                         draw(seed, pos >> 7, row, (uint32_t)t, wword(t, 2, c), rp);
                     const uint32_t x = 2 * pos + off;
@@ -336,7 +363,8 @@ void or_ising_sweeps_sc_fast(const or_lattice* L, const or_model* M, uint64_t se
                         flip = 1;
                     } else {
                         const uint32_t th = thr[b * OR_THR_STRIDE + S];
-                        const uint32_t hi8 = (ra[(pos & 15) >> 2] >> (8 * (pos & 3))) & 0xFFu;
+                        uint32_t hi8 = 0;
+                        for (int kb = 0; kb < 8; ++kb) hi8 |= ((ra[kb] >> (pos & 31)) & 1u) << kb;
                         if (hi8 != th >> 24) {
                             flip = hi8 < th >> 24;
                         } else {

@@ -3,8 +3,8 @@
 A wave queues one record per row segment whose draws tie a threshold's high
 byte (probability 2^-8 per site) and settles them after its planes; when
 the queue is full the lane that found the tie settles it in place.  At the
-default 64 records that path is rare (a few waves per 10^4 sweeps of
-sc 256^3 x 3), so here the queue is shrunk to 0, 1 and 2 records and the
+default capacity (128 records in the sc pass, 64 in bcc and fcc) that path
+is rare, so here the queue is shrunk to 0, 1 and 2 records and the
 passes -- sc, bcc and fcc, int8 and bit-packed, open and periodic axes,
 flip and random proposals -- must still equal the oracle bit for bit."""
 import numpy as np
@@ -42,7 +42,7 @@ def test_small_tie_queue_matches_oracle(vg, orc, ctx, kind, dims, pbc, T, h, pro
         s = orc.ising_init(OL, d.seed)
         sums = orc.ising_sweeps(OL, OM, d.seed, 0, nsw, s)
         want.append((sums, s))
-    for cap, tile in [(0, 8), (1, 8), (2, 4), (1, 1), (64, 8)]:
+    for cap, tile in [(0, 8), (1, 8), (2, 4), (1, 1), (128, 8)]:
         print(f"ties {kind} {dims} bits={bits} cap={cap} tile={tile}", flush=True)
         with vg.Batch(ctx, descs) as b:
             b.randomize()
@@ -60,7 +60,7 @@ def test_tie_capacity_bounds(vg, ctx):
     L = vg.lattice(vg.SC, 8, 8, 8)
     with vg.Batch(ctx, [vg.chain(L)]) as b:
         with pytest.raises(vg.VegasError) as e:
-            b.set_tie_capacity(65)
+            b.set_tie_capacity(129)
         assert e.value.kind == "InvalidArgument"
 
 

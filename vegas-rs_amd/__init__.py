@@ -389,8 +389,9 @@ class Batch:
         return v.value
 
     def set_tie_capacity(self, records: int) -> None:
-        """Test hook: tie records a wave may queue (0..64) before it settles
-        further tied rows in place; results do not depend on it."""
+        """Test hook: tie records a wave may queue (0..128; the bcc and fcc
+        passes hold at most 64) before it settles further tied rows in place;
+        results do not depend on it."""
         _check(_lib.vg_batch_set_tie_capacity(self.h, records))
 
     def set_timing(self, enabled: bool) -> None:

@@ -109,19 +109,27 @@ void ising_thresholds(const ChainHost& ch, ChainDev& d) {
                 d.thr[b][S] = (uint32_t)th;
         }
     }
-    // Byte tables of the packed decision: entry c of table (g, b) is class
-    // S = 2c + g (c up neighbours among 6 - g present slots).
-    std::memset(d.t8, 0, sizeof d.t8);
-    std::memset(d.a8, 0, sizeof d.a8);
-    for (int g = 0; g < 5; ++g)
-        for (int b = 0; b < 2; ++b)
-            for (int c = 0; c < 8; ++c) {
-                const int S = 2 * c + g;
-                if (S > 2 * z) continue;
-                const uint32_t al = (d.always[b] >> S) & 1u ? 0xFFu : 0u;
-                const uint32_t hi = al ? 0xFFu : d.thr[b][S] >> 24;
-                d.t8[g][b][c >> 2] |= hi << (8 * (c & 3));
-                d.a8[g][b][c >> 2] |= al << (8 * (c & 3));
+    // Class plans of the multispin sc pass (z = 6): for a site with g absent
+    // neighbour slots, p = 6 - g present and a of them aligned with its bit b,
+    // the up-neighbour count is c = a (b = 1) or p - a (b = 0) and the class
+    // S = 2c + g.  Classes always accepted are left out.
+    std::memset(d.plan, 0, sizeof d.plan);
+    if (z == 6)
+        for (uint32_t g = 0; g < vg::kPlanG; ++g)
+            for (uint32_t a = 0; a <= 6 - g; ++a) {
+                uint32_t slot[2] = {0, 0};
+                for (uint32_t b = 0; b < 2; ++b) {
+                    const uint32_t S = 2 * (b ? a : 6 - g - a) + g;
+                    if ((d.always[b] >> S) & 1u) continue;
+                    const uint32_t th = d.thr[b][S];
+                    slot[b] = (1u << (3 + b)) | ((th != 0 ? 1u : 0u) << 5) | ((th >> 24) << 8);
+                }
+                if (slot[0] && slot[1] && (slot[0] >> 5) == (slot[1] >> 5)) {
+                    d.plan[g][a][0] = slot[0] | slot[1];
+                } else {
+                    d.plan[g][a][0] = slot[0] ? slot[0] : slot[1];
+                    d.plan[g][a][1] = slot[0] ? slot[1] : 0;
+                }
             }
 }
 
@@ -226,10 +234,12 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
         }
         return VG_OK;
     }
-    const uint32_t seg = is_ising(g) ? vg::kIsingSeg : vg::kHeisSeg;
+    const uint32_t seg = (g == kIsingSc || g == kIsingBits) ? vg::kScSeg
+                         : is_ising(g)                        ? vg::kIsingSeg
+                                                              : vg::kHeisSeg;
     const uint32_t align = is_ising(g) ? 64u : 16u;  // 64-byte aligned rows
-    if (is_bits(g))  // bytes: whole 16-position segments, 16-byte aligned rows
-        d.pitch = (uint32_t)round_up((d.w + 15) / 16 * 2, 16);
+    if (is_bits(g))  // bytes: whole segments (2 or 4 bytes), 16-byte aligned rows
+        d.pitch = (uint32_t)round_up((d.w + seg - 1) / seg * (seg / 8), 16);
     else
         d.pitch = (uint32_t)round_up(d.w, align);
     d.segs = (d.w + seg - 1) / seg;
@@ -1453,22 +1463,23 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
         uint32_t best = start_tile;
         double best_ms = 1e300;
         const uint32_t tiles[4] = {1, 2, 4, 8};
-        for (uint32_t ti = 0; ti < 4 && rc == VG_OK; ++ti) {
-            rc = layout_work(batch, g, tiles[ti]);
-            if (rc == VG_OK) rc = run_impl(batch, 2, nullptr, nullptr, nullptr);
-            if (rc == VG_OK) rc = restore();
-            // Timed from graph replays captured beforehand (plain launches
-            // of passes this short measure the launch rate), best of two.
-            if (rc == VG_OK) rc = vg_batch_prepare(batch, trial_sweeps);
-            for (int rep = 0; rep < 2 && rc == VG_OK; ++rep) {
-                rc = run_impl(batch, trial_sweeps, nullptr, nullptr, nullptr);
+        // Three rounds over the tiles, each tile's best counted: a drift of
+        // the clocks during tuning then touches every tile alike.
+        for (int rep = 0; rep < 3 && rc == VG_OK; ++rep)
+            for (uint32_t ti = 0; ti < 4 && rc == VG_OK; ++ti) {
+                rc = layout_work(batch, g, tiles[ti]);
+                if (rc == VG_OK) rc = run_impl(batch, 2, nullptr, nullptr, nullptr);
+                if (rc == VG_OK) rc = restore();
+                // Timed from graph replays captured beforehand (plain
+                // launches of passes this short measure the launch rate).
+                if (rc == VG_OK) rc = vg_batch_prepare(batch, trial_sweeps);
+                if (rc == VG_OK) rc = run_impl(batch, trial_sweeps, nullptr, nullptr, nullptr);
                 if (rc == VG_OK && batch->last_ms < best_ms) {
                     best_ms = batch->last_ms;
                     best = tiles[ti];
                 }
                 if (rc == VG_OK) rc = restore();
             }
-        }
         if (rc != VG_OK) {
             // A trial failed part way (graph capture, memory): put the spins
             // and sweep counters back as they were, best effort, keeping the
@@ -1489,8 +1500,8 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
 
 int vg_batch_set_tie_capacity(vg_batch* batch, uint32_t records) {
     if (!batch) return fail(VG_ERR_INVALID_ARGUMENT, "null batch");
-    if (records > 64)
-        return fail(VG_ERR_INVALID_ARGUMENT, "tie capacity must be 0..64 (got %u)", records);
+    if (records > 128)
+        return fail(VG_ERR_INVALID_ARGUMENT, "tie capacity must be 0..128 (got %u)", records);
     for (Group& G : batch->groups) G.tie_cap = records;
     // captured graphs hold the old kernel arguments
     for (auto& kv : batch->graphs) quiet(hipGraphExecDestroy(kv.second));

@@ -157,7 +157,9 @@ struct Group {
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
     uint32_t prio = 0;  // z-marching Ising passes: every workgroup resident at once (wave priorities on)
-    uint32_t tie_cap = 64;  // z-marching Ising passes: tie records queued per wave (at most 64)
+    // z-marching Ising passes: tie records a wave may queue (each pass holds
+    // at most its own queue: 128 records in the sc pass, 64 in bcc and fcc)
+    uint32_t tie_cap = 128;
     uint32_t npass = 0;     // graph groups: passes per sweep (the most colours of a member)
     void* arena = nullptr;
     uint64_t arena_bytes = 0;

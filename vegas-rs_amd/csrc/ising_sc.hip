@@ -1,5 +1,6 @@
-// Int8 Ising checkerboard colour pass on the simple-cubic lattice, for any
-// number of chains packed in one arena (a single chain is a batch of one).
+// Ising checkerboard colour pass on the simple-cubic lattice, multispin
+// coded: 32 sites per 32-bit word, int8 or bit-packed planes, for any number
+// of chains packed in one arena (a single chain is a batch of one).
 //
 // Replaces the inner loop of MetropolisFlipIntegrator::step and
 // MetropolisIntegrator::step (reference src/integrator.rs:66-92, 109-138) with
@@ -10,35 +11,195 @@
 // -- the reference's `u < exp(-dE/T)` rule (integrator.rs:127-134) with u
 // quantised to 2^-32.
 //
-// Work item: one 16-position segment of a row, marched through K (the tile,
-// a template parameter picked by the host) consecutive z planes.  Along z the row below is the previous row's
-// neighbour above, so each plane costs four 16-byte loads (own, y-1, y+1,
-// z+1) instead of six.  Items of one wave share the parity of y, so the
-// direction of the x-neighbour byte shift (which alternates with y+z) is
-// uniform across the wave.
+// Work item: one 32-position segment of a row (bit j of every word below is
+// position i0 + j), marched through K (the tile, a template parameter picked
+// by the host) consecutive z planes.  Along z the row below is the previous
+// row's neighbour above, so each plane loads four rows (own, y-1, y+1, z+1)
+// instead of six.  Int8 rows are packed to words on load (v_dot4 with bit
+// weights) and expanded on store; bit-packed rows are one 32-bit load.
 //
-// Per site everything is done four sites per 32-bit register: neighbour
-// counts by byte-wise adds, the acceptance class lookup by v_perm_b32 on
-// 8-entry byte tables, and the comparison of the draw's high byte with the
-// threshold's high byte by borrow-free byte arithmetic.  The high bytes of a
-// segment's 16 draws come from one Philox call.  When a draw's high byte
-// equals the threshold's (probability 2^-8) the low 24 bits decide; those
-// ties are rare per row but common per wave, so a row with ties only leaves a
-// record (slot by ballot) in the wave's LDS queue; once the wave has finished
-// its planes, one lane per record resolves its ties, writes the flipped bytes
-// and corrects the wave's sums.  The result equals comparing the full 32-bit
-// draw.  No workgroup barrier is needed after the tables are loaded.
+// The decision works on whole words.  Each neighbour word is replaced by
+// the complement of the site bits where that neighbour is absent (open
+// boundary), and a carry-save adder counts, per site, the present neighbours
+// aligned with it: a = 0..6 in three bit planes.  With g absent slots and
+// p = 6 - g present, dE = 2J(2a - p) -+ 2h, so the class (b, a) with b the
+// current bit fixes the threshold; the classes that are not always accepted
+// come from the chain's plan for g (a few per temperature: a = 4, 5, 6 in a
+// ferromagnet at h = 0).  For each such class the site mask is one bit
+// operation on the count planes, and the draw's high byte is compared with
+// the class threshold's high byte bit-serially: the eight high bits of the
+// 32 sites' draws are the eight words of the two Philox calls keyed by
+// positions i0 / 16 and i0 / 16 + 1 (bit j of word k is bit k of site j's
+// high byte).  When a draw's high byte equals the threshold's (probability
+// 2^-8) the low 24 bits decide; those ties leave a record (slot by ballot)
+// in the wave's LDS queue and, once the wave has finished its planes, one
+// lane per record draws them, stores the settled word and corrects the
+// wave's sums.  The result equals comparing the full 32-bit draw.  No
+// workgroup barrier is needed after the threshold table is loaded.
 #include "ising_swar.hpp"
 
 namespace vg {
 
 constexpr uint32_t kWaves = kBlock / 64;
-constexpr uint32_t kTieRows = 64;     // tie records per wave (one per row with ties)
-constexpr uint32_t kTieWords = 10;
+constexpr uint32_t kTieRows = 128;  // tie records per wave (one per row segment with ties)
+constexpr uint32_t kTieWords = 6;   // meta, tie mask, count planes a0..a2, site bits
 
-__device__ __forceinline__ uint32_t pick4(uint32_t k, uint32_t a, uint32_t b, uint32_t c,
-                                          uint32_t d) {
-    return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
+// 16 bytes of 0/1 -> 16 bits (byte j -> bit j).
+__device__ __forceinline__ uint32_t pack_bytes16(u32x4 v) {
+    const uint32_t lo = __builtin_amdgcn_udot4(
+        v.y, 0x80402010u, __builtin_amdgcn_udot4(v.x, 0x08040201u, 0u, false), false);
+    const uint32_t hi = __builtin_amdgcn_udot4(
+        v.w, 0x80402010u, __builtin_amdgcn_udot4(v.z, 0x08040201u, 0u, false), false);
+    return lo | (hi << 8);
+}
+
+// A row segment of 32 positions as loaded (int8: 32 bytes of 0/1;
+// bit-packed: one word), and its site bits.  kNoRow reads 0.
+template <bool BITS>
+struct RowRaw {
+    uint32_t w;
+};
+template <>
+struct RowRaw<false> {
+    u32x4 a, b;
+};
+
+template <bool BITS>
+__device__ __forceinline__ RowRaw<BITS> load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    RowRaw<BITS> v;
+    if constexpr (BITS) {
+        v.w = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    } else {
+        v.a = bload16(r, off);
+        v.b = bload16(r, off + 16u);
+    }
+    return v;
+}
+
+template <bool BITS>
+__device__ __forceinline__ uint32_t site_bits(const RowRaw<BITS>& v) {
+    if constexpr (BITS)
+        return v.w;
+    else
+        return pack_bytes16(v.a) | (pack_bytes16(v.b) << 16);
+}
+
+template <bool BITS>
+__device__ __forceinline__ uint32_t load_word(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return site_bits<BITS>(load_raw<BITS>(r, off));
+}
+
+template <bool BITS>
+__device__ __forceinline__ void store_word(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
+    if (BITS) {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
+        return;
+    }
+    u32x4 a, b;
+    a.x = nibble_bytes(v);
+    a.y = nibble_bytes(v >> 4);
+    a.z = nibble_bytes(v >> 8);
+    a.w = nibble_bytes(v >> 12);
+    b.x = nibble_bytes(v >> 16);
+    b.y = nibble_bytes(v >> 20);
+    b.z = nibble_bytes(v >> 24);
+    b.w = nibble_bytes(v >> 28);
+    __builtin_amdgcn_raw_buffer_store_b128(a, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(b, r, off + 16u, 0, 0);
+}
+
+template <bool BITS>
+__device__ __forceinline__ uint32_t word_off(uint32_t i0) {
+    return BITS ? i0 >> 3 : i0;
+}
+
+// Bitwise function of three words given by its truth table: bit i of TT is
+// the result for (a, b, c) = (i >> 2 & 1, i >> 1 & 1, i & 1).  One v_bitop3.
+template <uint32_t TT>
+__device__ __forceinline__ uint32_t bop3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, TT);
+}
+constexpr uint32_t kXor3 = 0x96;  // a ^ b ^ c
+constexpr uint32_t kMaj3 = 0xE8;  // at least two of a, b, c
+
+// One plan slot: of the sites in class mask e (restricted to the slot's
+// bits), those whose draw high byte (planes d, bit k of site j = bit j of
+// d[k]) is not below the slot's threshold byte are rejected, and those equal
+// to it are ties when the threshold is nonzero.
+__device__ __forceinline__ void plan_slot(uint32_t slot, uint32_t e, uint32_t o, uint32_t no,
+                                          const uint32_t d[8], uint32_t& rej, uint32_t& tie) {
+    const uint32_t cover = (slot >> 3) & 3u;
+    const uint32_t m = cover == 3u ? e : e & (cover == 2u ? o : no);
+    // d < t bit-serially from the least significant bit: at bit k,
+    // lt = (~d & t) | (~(d ^ t) & lt)
+    uint32_t lt = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) lt = bop3<0x8E>(d[k], 0u - ((slot >> (8 + k)) & 1u), lt);
+    rej = bop3<0xF4>(rej, m, lt);  // rej | (m & ~lt)
+    if (slot & 32u) {
+        uint32_t eq = ~0u;  // eq & ~(d ^ t)
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) eq = bop3<0x90>(eq, d[k], 0u - ((slot >> (8 + k)) & 1u));
+        tie = bop3<0xF8>(tie, m, eq);  // tie | (m & eq)
+    }
+}
+
+// Sites whose aligned count (planes a0, a1, a2) is A: one v_bitop3.
+template <uint32_t A>
+__device__ __forceinline__ void plan_count(const uint32_t pw[14], uint32_t a0, uint32_t a1,
+                                           uint32_t a2, uint32_t o, uint32_t no,
+                                           const uint32_t d[8], uint32_t& rej, uint32_t& tie) {
+    if (pw[2 * A] == 0u) return;
+    const uint32_t e = bop3<(1u << A)>(a2, a1, a0);
+    plan_slot(pw[2 * A], e, o, no, d, rej, tie);
+    if (pw[2 * A + 1] != 0u) plan_slot(pw[2 * A + 1], e, o, no, d, rej, tie);
+}
+
+// Decision of the sites in `live` under the plan of absent count g: the
+// accepted sites (ties excluded) and the tied ones.
+__device__ __forceinline__ void decide(const ChainDev& C, uint32_t g, uint32_t a0, uint32_t a1,
+                                       uint32_t a2, uint32_t o, uint32_t no, const uint32_t d[8],
+                                       uint32_t live, uint32_t& acc, uint32_t& tie) {
+    uint32_t pw[14];  // the plan's 14 words, one scalar load
+#pragma unroll
+    for (uint32_t k = 0; k < 14; ++k) pw[k] = C.plan[g][k >> 1][k & 1u];
+    uint32_t rej = 0, t = 0;
+    plan_count<0>(pw, a0, a1, a2, o, no, d, rej, t);
+    plan_count<1>(pw, a0, a1, a2, o, no, d, rej, t);
+    plan_count<2>(pw, a0, a1, a2, o, no, d, rej, t);
+    plan_count<3>(pw, a0, a1, a2, o, no, d, rej, t);
+    plan_count<4>(pw, a0, a1, a2, o, no, d, rej, t);
+    plan_count<5>(pw, a0, a1, a2, o, no, d, rej, t);
+    plan_count<6>(pw, a0, a1, a2, o, no, d, rej, t);
+    acc = live & ~rej;
+    tie = live & t;
+}
+
+// Settles the tied sites tm of one row segment: site j has bit b = bit j of
+// ob, aligned count from a0..a2, absent slots g_row (+1 at position xe), so
+// its class is S = 2c + g with c = a (b = 1) or 6 - g - a (b = 0).  Returns
+// the accepted flips; dm, db collect their magnetisation and bond changes.
+__device__ __forceinline__ uint32_t settle(const uint32_t* lut, const ChainDev& C, uint32_t tm,
+                                           uint32_t a0, uint32_t a1, uint32_t a2, uint32_t ob,
+                                           uint32_t g_row, uint32_t xe, uint32_t i0,
+                                           uint32_t row, uint64_t t, uint32_t colour,
+                                           uint32_t k0, uint32_t k1, int32_t& dm, int32_t& db) {
+    uint32_t flips = 0;
+    while (tm) {
+        const uint32_t j = __builtin_ctz(tm);
+        tm &= tm - 1u;
+        const uint32_t bj = (ob >> j) & 1u;
+        const uint32_t a = ((a0 >> j) & 1u) | (((a1 >> j) & 1u) << 1) | (((a2 >> j) & 1u) << 2);
+        const uint32_t g = g_row + (j == xe ? 1u : 0u);
+        const uint32_t S = 2u * (bj ? a : 6u - g - a) + g;
+        if (tie_accept(lut, C.always[bj], bj, S, i0 + j, row, t, colour, k0, k1)) {
+            flips |= 1u << j;
+            const int32_t ds = 2 - 4 * (int32_t)bj;
+            dm += ds;
+            if (colour == 1) db += ds * ((int32_t)S - 6);  // bonds to colour 0
+        }
+    }
+    return flips;
 }
 
 template <uint32_t K, bool BITS>
@@ -50,8 +211,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                      uint32_t total_blocks, uint32_t grid,
                                                      uint32_t opts) {
     __shared__ uint32_t lut[2 * kThrS];
-    __shared__ uint32_t tab[20];  // t8, [g][b][2]
-    // per wave: kTieRows records of {meta, tie mask, cnt[4], own[4]}
+    // per wave: kTieRows records of {meta, tie mask, a0, a1, a2, site bits}
     __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
     // opts: bit 0 = falling wave priorities; bits 8-15 = records the tie
     // queue may hold (at most kTieRows; fewer only to exercise the overflow
@@ -65,13 +225,11 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid < 2 * kThrS) lut[tid] = (&C.thr[0][0])[tid];
-    if (tid >= 64 && tid < 84) tab[tid - 64] = (&C.t8[0][0][0])[tid - 64];
     __syncthreads();
 
     const uint64_t t = C.sweep0 + s;
     const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
     const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
-    const uint32_t always0 = C.always[0], always1 = C.always[1];
     // A colour plane is below 4 GiB (checked on the host): 32-bit offsets.
     uint8_t* const own_plane = arena + C.off[colour];
     const uint8_t* const oth = arena + C.off[colour ^ 1u];
@@ -79,10 +237,12 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     const __amdgpu_buffer_rsrc_t ro = plane_rsrc(oth, (uint32_t)C.plane);
     const uint32_t zstep = ly * pitch;  // bytes between planes z and z+1
     const uint32_t local = (b - C.block0) * kBlock + tid;
+    const uint32_t wA = ctr_w(t, kIsingA, colour);
     uint32_t (*const q_wave)[kTieWords] = tie_q[wave];
     uint32_t nrec = 0;  // records queued by this wave, as counted by this lane
 
     int32_t m_part = 0, b_part = 0;
+    uint32_t ups = 0;  // up sites after each plane's update, summed
     uint32_t seg = 0, y = 0, zc = 0;
     if (local < C.items) {
         const uint32_t r = fast_div(local, C.segs_m, C.segs_s);
@@ -94,12 +254,9 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     if (local < C.items && y < ly) {
         const uint32_t z0 = zc * K;
         const uint32_t nk = min(K, lz - z0);
-        const uint32_t i0 = seg * kIsingSeg;
-        const uint32_t nvalid = min(kIsingSeg, w - i0);
-        const uint32_t vbits = nvalid >= 16 ? 0xFFFFu : ((1u << nvalid) - 1u);
-        uint32_t validH[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) validH[k] = nibble_bytes(vbits >> (4 * k)) << 7;
+        const uint32_t i0 = seg * kScSeg;
+        const uint32_t nvalid = min(kScSeg, w - i0);
+        const uint32_t vmask = nvalid >= 32u ? 0xFFFFFFFFu : (1u << nvalid) - 1u;
         const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
         const bool ym_ok = y > 0 || py;
         const bool yp_ok = y + 1 < ly || py;
@@ -107,229 +264,165 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         const int32_t dym = y > 0 ? -(int32_t)pitch : (int32_t)((ly - 1) * pitch);
         const int32_t dyp = y + 1 < ly ? (int32_t)pitch : -(int32_t)((ly - 1) * pitch);
         const uint32_t gy = (ym_ok ? 0u : 1u) + (yp_ok ? 0u : 1u);
-        // x neighbour outside the 16 positions: i0-1 (q = 0) or i0+16 (q = 1)
+        // x neighbour outside the 32 positions: i0-1 (q = 0) or i0+32 (q = 1)
         const bool xl_ok = i0 > 0 || px;
         const uint32_t xl_pos = i0 > 0 ? i0 - 1 : w - 1;
-        const uint32_t pw = w - i0;  // positions left in the row, 1..16 in the last segment
-        const bool xr_in = i0 + kIsingSeg < w;
-        uint32_t off = (y + ly * z0) * pitch + seg_off<BITS>(i0);  // row (y, z0), this segment
+        const bool xr_in = i0 + kScSeg < w;
+        // bit of the last valid position, which takes the right neighbour's bit
+        const uint32_t xr_sh = xr_in ? 31u : nvalid - 1u;
+        uint32_t off = (y + ly * z0) * pitch + word_off<BITS>(i0);  // row (y, z0), this segment
         // the row window along z: row (y, z-1) and row (y, z)
         const bool zm0_ok = z0 > 0 || pz;
-        u32x4 below =
-            load_seg<BITS>(ro, !zm0_ok ? kNoRow : (z0 > 0 ? off - zstep : off + (lz - 1) * zstep));
-        u32x4 here = load_seg<BITS>(ro, off);
+        uint32_t below =
+            load_word<BITS>(ro, !zm0_ok ? kNoRow : (z0 > 0 ? off - zstep : off + (lz - 1) * zstep));
+        uint32_t here = load_word<BITS>(ro, off);
         bool below_ok = zm0_ok;
 
-#pragma unroll
-        for (uint32_t k = 0; k < K; ++k) {
-            if (k >= nk) break;
+        const bool rnd = C.proposal != 0;
+        const uint32_t wP = ctr_w(t, kIsingProposal, colour);
+        // not unrolled: one plane's code is large, and the instruction
+        // cache is shared by the waves of two CUs
+#pragma unroll 1
+        for (uint32_t k = 0; k < nk; ++k) {
             // The SIMD arbiter favours older waves; when every wave of the
             // pass is resident at once, a wave's priority falls as it
             // marches, so the waves of a SIMD progress together and the pass
-            // does not end in a tail of a few young waves.  (With a second
-            // round of waves, staggered ends let it start early instead.)
+            // does not end in a tail of a few young waves.
             if (prio) set_prio(((K - 1u - k) * 4u) / K);
             const uint32_t z = z0 + k;
             const uint32_t row = y + ly * z;
             const bool zp_ok = z + 1 < lz || pz;
-            const u32x4 o4 = load_seg<BITS>(rw_own, off);
-            const u32x4 ym4 = load_seg<BITS>(ro, ym_ok ? off + dym : kNoRow);
-            const u32x4 yp4 = load_seg<BITS>(ro, yp_ok ? off + dyp : kNoRow);
-            const u32x4 above = load_seg<BITS>(
-                ro, !zp_ok ? kNoRow : (z + 1 < lz ? off + zstep : off - (lz - 1) * zstep));
-            const uint32_t g = gy + (below_ok ? 0u : 1u) + (zp_ok ? 0u : 1u);
-
-            // x neighbours of position i: i-1,i (q = 0) or i,i+1 (q = 1) in
-            // the other plane's same row.  The byte outside the segment is
-            // position i0-1 (q = 0), or i0+16, or position 0 across the
-            // periodic end (q = 1); kNoRow reads zero when x is open.
+            // x neighbours of position i: i-1, i (q = 0) or i, i+1 (q = 1) in
+            // the other plane's same row.  The bit outside the segment is
+            // position i0-1 (q = 0), or i0+32, or position 0 across the
+            // periodic end (q = 1); it reads 0 when x is open there.
             const uint32_t q = (colour + y + z) & 1u;
-            const uint32_t orow = off - seg_off<BITS>(i0);
-            uint32_t m[4] = {here.x, here.y, here.z, here.w};
-            uint32_t xs[4];
-            uint32_t gx = 0xFFFFFFFFu;  // position whose x neighbour is absent
-            if (q == 0) {
-                const uint32_t extra = load_pos<BITS>(ro, orow, xl_pos, xl_ok);
-                if (!xl_ok) gx = 0;
-                xs[0] = __builtin_amdgcn_alignbyte(m[0], extra << 24, 3);
-                xs[1] = __builtin_amdgcn_alignbyte(m[1], m[0], 3);
-                xs[2] = __builtin_amdgcn_alignbyte(m[2], m[1], 3);
-                xs[3] = __builtin_amdgcn_alignbyte(m[3], m[2], 3);
-            } else {
-                uint32_t extra = load_pos<BITS>(ro, orow, xr_in ? i0 + kIsingSeg : 0, xr_in || px);
-                if (!xr_in) {
-                    // Position w-1 needs position w: position 0 across the
-                    // periodic boundary, or nothing when x is open.
-                    if (!px) gx = pw - 1;
-                    if (pw != kIsingSeg) {
-#pragma unroll
-                        for (uint32_t kk = 0; kk < 4; ++kk)
-                            if ((pw >> 2) == kk) {
-                                const uint32_t sh = 8u * (pw & 3u);
-                                m[kk] = (m[kk] & ~(0xFFu << sh)) | (extra << sh);
-                            }
-                        extra = 0;
-                    }
-                }
-                xs[0] = __builtin_amdgcn_alignbyte(m[1], m[0], 1);
-                xs[1] = __builtin_amdgcn_alignbyte(m[2], m[1], 1);
-                xs[2] = __builtin_amdgcn_alignbyte(m[3], m[2], 1);
-                xs[3] = __builtin_amdgcn_alignbyte(extra, m[3], 1);
-            }
-            const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
-            const uint32_t dn[4] = {below.x, below.y, below.z, below.w};
-            const uint32_t up4[4] = {above.x, above.y, above.z, above.w};
-            const uint32_t ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
-            const uint32_t ypv[4] = {yp4.x, yp4.y, yp4.z, yp4.w};
-            const uint32_t t0lo = tab[g * 4 + 0], t0hi = tab[g * 4 + 1];
-            const uint32_t t1lo = tab[g * 4 + 2], t1hi = tab[g * 4 + 3];
+            const RowRaw<BITS> c_o = load_raw<BITS>(rw_own, off);
+            const RowRaw<BITS> c_ym = load_raw<BITS>(ro, ym_ok ? off + dym : kNoRow);
+            const RowRaw<BITS> c_yp = load_raw<BITS>(ro, yp_ok ? off + dyp : kNoRow);
+            const RowRaw<BITS> c_ab = load_raw<BITS>(
+                ro, !zp_ok ? kNoRow : (z + 1 < lz ? off + zstep : off - (lz - 1) * zstep));
+            const uint32_t extra =
+                load_pos<BITS>(ro, off - word_off<BITS>(i0), q ? (xr_in ? i0 + kScSeg : 0u) : xl_pos,
+                               q ? (xr_in || px) : xl_ok);
 
-            const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, colour)}, k0, k1);
-            const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
-            uint32_t live[4] = {validH[0], validH[1], validH[2], validH[3]};
-            if (C.proposal) {
-                const U4 rp = philox(
-                    U4{i0 >> 7, row, (uint32_t)t, ctr_w(t, kIsingProposal, colour)}, k0, k1);
-                const uint32_t bits = (u4_word(rp, (i0 >> 5) & 3u) >> (i0 & 31u)) & 0xFFFFu;
+            // draws first: they do not wait for the loads
+            uint32_t prop = 0;
+            if (rnd) {
                 // Spin::rand proposes Up when its draw is below 1/2
                 // (state.rs:76-84): proposal bit 0 -> Up.  A proposal equal
                 // to the current spin changes nothing.
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk)
-                    live[kk] &= (~(nibble_bytes(bits >> (4 * kk)) ^ o[kk]) & kOne) << 7;
+                const U4 rp = philox(U4{i0 >> 7, row, (uint32_t)t, wP}, k0, k1);
+                prop = u4_word(rp, (i0 >> 5) & 3u);
             }
-            uint32_t cnt[4], accH[4], tieH[4];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                cnt[kk] = m[kk] + xs[kk] + ymv[kk] + ypv[kk] + dn[kk] + up4[kk];  // 0..6
-                const uint32_t bm = __builtin_amdgcn_perm(0u, 0xFF00u, o[kk]);   // 0xFF where Up
-                const uint32_t th0 = __builtin_amdgcn_perm(t0hi, t0lo, cnt[kk]);
-                const uint32_t th1 = __builtin_amdgcn_perm(t1hi, t1lo, cnt[kk]);
-                const uint32_t th = (th1 & bm) | (th0 & ~bm);
-                accH[kk] = bytes_lt(rw[kk], th) & live[kk];
-                tieH[kk] = bytes_eq(rw[kk], th) & live[kk];
-            }
-            // The one position with an absent x neighbour (open x boundary)
-            // has S = 2c + g + 1, which the tables above do not cover.
-            if (gx < nvalid) {
-                const uint32_t kk = gx >> 2, sh = 8u * (gx & 3u);
-                const uint32_t c = (pick4(kk, cnt[0], cnt[1], cnt[2], cnt[3]) >> sh) & 0xFFu;
-                const uint32_t bj = (pick4(kk, o[0], o[1], o[2], o[3]) >> sh) & 1u;
-                const uint32_t S = 2u * c + g + 1u;
-                const uint32_t alw = ((bj ? always1 : always0) >> S) & 1u;
-                const uint32_t th_hi = alw ? 0xFFu : lut[bj * kThrS + S] >> 24;
-                const uint32_t r8 = (pick4(kk, rw[0], rw[1], rw[2], rw[3]) >> sh) & 0xFFu;
-                const uint32_t lv = (pick4(kk, live[0], live[1], live[2], live[3]) >> sh) & 0x80u;
-                const uint32_t a = (lv && r8 < th_hi) ? 0x80u : 0u;
-                const uint32_t ti = (lv && r8 == th_hi) ? 0x80u : 0u;
-#pragma unroll
-                for (uint32_t q2 = 0; q2 < 4; ++q2)
-                    if (q2 == kk) {
-                        accH[q2] = (accH[q2] & ~(0x80u << sh)) | (a << sh);
-                        tieH[q2] = (tieH[q2] & ~(0x80u << sh)) | (ti << sh);
-                    }
-            }
-            uint32_t n[4], any = 0;
-            int32_t upc = 0;
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const uint32_t f = accH[kk] >> 7;
-                any |= f;
-                n[kk] = o[kk] ^ f;
-                upc += __popc(n[kk]);
-            }
-            if (any) {
-                if (BITS) {
-                    __builtin_amdgcn_raw_buffer_store_b16(
-                        (unsigned short)pack16(n[0], n[1], n[2], n[3]), rw_own, off, 0, 0);
-                } else {
-                    u32x4 nv;
-                    nv.x = n[0];
-                    nv.y = n[1];
-                    nv.z = n[2];
-                    nv.w = n[3];
-                    __builtin_amdgcn_raw_buffer_store_b128(nv, rw_own, off, 0, 0);
+            const U4 r0 = philox(U4{i0 >> 4, row, (uint32_t)t, wA}, k0, k1);
+            const U4 r1 = philox(U4{(i0 >> 4) + 1u, row, (uint32_t)t, wA}, k0, k1);
+            const uint32_t d[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+
+            const uint32_t o = site_bits<BITS>(c_o);
+            const uint32_t ym = site_bits<BITS>(c_ym);
+            const uint32_t yp = site_bits<BITS>(c_yp);
+            const uint32_t above = site_bits<BITS>(c_ab);
+            const uint32_t g_row = gy + (below_ok ? 0u : 1u) + (zp_ok ? 0u : 1u);
+            const uint32_t xe = q ? (!xr_in && !px ? nvalid - 1u : 32u) : (!xl_ok ? 0u : 32u);
+            const uint32_t eb = xe < 32u ? 1u << xe : 0u;  // site with an absent x neighbour
+            const uint32_t no = ~o;
+            const uint32_t mrow = here & vmask;
+            // an absent neighbour counts as anti-aligned with its site
+            const uint32_t xa = q ? mrow : ((((mrow << 1) | extra) & ~eb) | (no & eb));
+            const uint32_t xb = q ? ((((mrow >> 1) | (extra << xr_sh)) & ~eb) | (no & eb)) : mrow;
+            const uint32_t nym = ym_ok ? ym : no;
+            const uint32_t nyp = yp_ok ? yp : no;
+            const uint32_t nzm = below_ok ? below : no;
+            const uint32_t nzp = zp_ok ? above : no;
+            // aligned count a = 0..6 of the present neighbours equal to the
+            // site (n aligned where ~(n ^ o)): two full adders over aligned
+            // triples, then their sum
+            const uint32_t s1 = bop3<kXor3>(bop3<kXor3>(xa, xb, nym), o, ~0u);
+            const uint32_t c1 = bop3<kXor3>(bop3<kMaj3>(xa, xb, nym), o, ~0u);
+            const uint32_t s2 = bop3<kXor3>(bop3<kXor3>(nyp, nzm, nzp), o, ~0u);
+            const uint32_t c2 = bop3<kXor3>(bop3<kMaj3>(nyp, nzm, nzp), o, ~0u);
+            const uint32_t a0 = s1 ^ s2, h = s1 & s2;
+            const uint32_t a1 = bop3<kXor3>(c1, c2, h);
+            const uint32_t a2 = bop3<kMaj3>(c1, c2, h);
+            const uint32_t live = rnd ? vmask & ~(prop ^ o) : vmask;
+
+            // Sites of this row segment take the plan of their absent count:
+            // g_row, and g_row + 1 at position xe.  Usually every lane of the
+            // wave has one and the same count.
+            uint32_t acc, tie;
+            const uint32_t gmask = (1u << g_row) | (eb ? 2u << g_row : 0u);
+            const uint32_t gu = __builtin_amdgcn_readfirstlane(g_row);
+            if (__ballot(gmask != (1u << gu)) == 0) {
+                decide(C, gu, a0, a1, a2, o, no, d, live, acc, tie);
+            } else {
+                acc = 0;
+                tie = 0;
+#pragma unroll 1
+                for (uint32_t g = 0; g < kPlanG; ++g) {
+                    if (__ballot((gmask >> g) & 1u) == 0) continue;
+                    const uint32_t sel = (g_row == g ? ~eb : 0u) | (g_row + 1u == g ? eb : 0u);
+                    uint32_t ag, tg;
+                    decide(C, g, a0, a1, a2, o, no, d, live & sel, ag, tg);
+                    acc |= ag;
+                    tie |= tg;
                 }
             }
-            m_part += 2 * upc - (int32_t)nvalid;
-            if (colour == 1) {
-                // sum_j s_j (S_j - 6), s = 2n - 1, S = 2c + g (+1 at an open x end)
-                uint32_t nc = 0, cs = 0;
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const uint32_t cv = cnt[kk] & ((validH[kk] >> 7) * 0xFFu);
-                    nc = __builtin_amdgcn_udot4(n[kk], cv, nc, false);
-                    cs = __builtin_amdgcn_udot4(cv, kOne, cs, false);
-                }
-                const int32_t g6 = (int32_t)g - 6;
-                b_part += 4 * (int32_t)nc + 2 * g6 * upc - 2 * (int32_t)cs - g6 * (int32_t)nvalid;
-                if (gx < nvalid) {
-                    const uint32_t kk = gx >> 2, sh = 8u * (gx & 3u);
-                    const int32_t nj = (int32_t)((pick4(kk, n[0], n[1], n[2], n[3]) >> sh) & 1u);
-                    b_part += 2 * nj - 1;
-                }
-            }
-            // Rows with ties leave one record each in the wave's queue (their
-            // tied bytes were written unchanged above); slots by ballot.
-            // tie flags -> 16-bit mask (byte MSBs gathered by a multiply)
-            const uint32_t tmask = ((((tieH[0] >> 7) & kOne) * 0x10204080u) >> 28) |
-                                   (((((tieH[1] >> 7) & kOne) * 0x10204080u) >> 28) << 4) |
-                                   (((((tieH[2] >> 7) & kOne) * 0x10204080u) >> 28) << 8) |
-                                   (((((tieH[3] >> 7) & kOne) * 0x10204080u) >> 28) << 12);
-            const uint64_t has = __ballot(tmask != 0);
+
+            // Segments with ties leave one record each in the wave's queue
+            // (their tied sites keep their bits in the store below); slots by
+            // ballot.  When the queue is full, this lane settles them now.
+            const uint64_t has = __ballot(tie != 0);
             if (has) {
-                if (tmask) {
+                if (tie) {
                     const uint32_t slot =
                         nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
                     if (slot < qcap) {
                         uint32_t* rec = q_wave[slot];
-                        rec[0] = lane | (k << 6) | (g << 9) | (min(gx, 31u) << 12);
-                        rec[1] = tmask;
-                        *reinterpret_cast<uint4*>(rec + 2) = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
-                        // own bytes after the untied flips (tied ones are
-                        // unchanged); packed, they are the row's stored bits
-                        *reinterpret_cast<uint4*>(rec + 6) = make_uint4(n[0], n[1], n[2], n[3]);
+                        rec[0] = lane | (k << 6) | (g_row << 9) | (xe << 12);
+                        rec[1] = tie;
+                        rec[2] = a0;
+                        rec[3] = a1;
+                        rec[4] = a2;
+                        rec[5] = o ^ acc;
                     } else {
-                        // Queue full: resolve in this lane.  (A wave marching 8
-                        // planes queues ~31 records on average against 64
-                        // slots, so this happens a few times per 10^4
-                        // sweeps of sc 256^3 x 3.)
-                        uint32_t tm = tmask;
-                        uint32_t flips = 0;
-                        while (tm) {
-                            const uint32_t j = __builtin_ctz(tm);
-                            tm &= tm - 1u;
-                            const uint32_t kk = j >> 2, sh = 8u * (j & 3u);
-                            const uint32_t c = (pick4(kk, cnt[0], cnt[1], cnt[2], cnt[3]) >> sh) & 0xFFu;
-                            const uint32_t bj = (pick4(kk, o[0], o[1], o[2], o[3]) >> sh) & 1u;
-                            const uint32_t S = 2u * c + g + (j == gx ? 1u : 0u);
-                            if (tie_accept(lut, bj ? always1 : always0, bj, S, i0 + j, row, t,
-                                           colour, k0, k1)) {
-                                if (!BITS) own_plane[off + j] = (uint8_t)(bj ^ 1u);
-                                flips |= 1u << j;
-                                const int32_t ds = 2 - 4 * (int32_t)bj;
-                                m_part += ds;
-                                if (colour == 1) b_part += ds * ((int32_t)S - 6);
-                            }
-                        }
-                        if (BITS && flips)
-                            __builtin_amdgcn_raw_buffer_store_b16(
-                                (unsigned short)(pack16(n[0], n[1], n[2], n[3]) ^ flips), rw_own,
-                                off, 0, 0);
+                        // Queue full (rare: a wave marching 8 planes near
+                        // T_c queues a few dozen records against 128 slots).
+                        int32_t dm = 0, db = 0;
+                        acc |= settle(lut, C, tie, a0, a1, a2, o, g_row, xe, i0, row, t, colour,
+                                      k0, k1, dm, db);
                     }
                 }
                 nrec += __builtin_popcountll(has);
+            }
+            const uint32_t n = o ^ acc;
+            if (acc) store_word<BITS>(rw_own, off, n);
+            ups += __popc(n & vmask);
+            if (colour == 1) {
+                // sum over the segment's sites of s_j * (sum of its present
+                // neighbours' spins) = 2a' - p, a' the count aligned with the
+                // new bit: a for kept sites, p - a for flipped ones, p = 6 - g
+                const uint32_t kept = vmask & ~acc;
+                const int32_t sak = (int32_t)(__popc(a0 & kept) + 2u * __popc(a1 & kept) +
+                                              4u * __popc(a2 & kept));
+                const int32_t saf =
+                    (int32_t)(__popc(a0 & acc) + 2u * __popc(a1 & acc) + 4u * __popc(a2 & acc));
+                const int32_t nf = (int32_t)__popc(acc);
+                const int32_t p = 6 - (int32_t)g_row;
+                const int32_t xf = (acc & eb) ? 1 : 0, xk = (kept & eb) ? 1 : 0;
+                b_part += 2 * (sak - saf) - p * ((int32_t)nvalid - 2 * nf) + xk - xf;
             }
             below = here;
             here = above;
             below_ok = true;
             off += zstep;
         }
+        m_part += 2 * (int32_t)ups - (int32_t)(nk * nvalid);
     }
 
-    // Resolve this wave's queued ties: one lane per record draws the low 24
-    // bits of each tied site, writes the flipped byte and corrects the sums.
+    // Settle this wave's queued ties: one lane per record draws the low 24
+    // bits of each tied site, stores the settled word and corrects the sums.
     // The wave's own row stores must land first.  nrec grows by ballot counts
     // among the lanes still marching, and a lane that stops (last z chunk)
     // or never started (past the chain's items, or the row y = ly of an odd
@@ -342,7 +435,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t* rec = q_wave[e];
             const uint32_t meta = rec[0];
             const uint32_t owner = (wave << 6) | (meta & 63u), k = (meta >> 6) & 7u;
-            const uint32_t g = (meta >> 9) & 7u, gx = (meta >> 12) & 31u;
+            const uint32_t g_row = (meta >> 9) & 7u, xe = (meta >> 12) & 63u;
             const uint32_t l = (b - C.block0) * kBlock + owner;
             const uint32_t r = fast_div(l, C.segs_m, C.segs_s);
             const uint32_t sg = l - r * C.segs;
@@ -350,30 +443,24 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
             const uint32_t zz = (r2 >> 1) * K + k;
             const uint32_t row = yy + ly * zz;
-            uint32_t tm = rec[1];
-            uint32_t flips = 0;
-            while (tm) {
-                const uint32_t j = __builtin_ctz(tm);
-                tm &= tm - 1u;
-                const uint32_t sh = 8u * (j & 3u);
-                const uint32_t c = (rec[2 + (j >> 2)] >> sh) & 0xFFu;
-                const uint32_t bj = (rec[6 + (j >> 2)] >> sh) & 1u;
-                const uint32_t S = 2u * c + g + (j == gx ? 1u : 0u);
-                const uint32_t pos = sg * kIsingSeg + j;
-                if (tie_accept(lut, bj ? always1 : always0, bj, S, pos, row, t, colour, k0, k1)) {
-                    if (!BITS) own_plane[row * pitch + pos] = (uint8_t)(bj ^ 1u);
-                    flips |= 1u << j;
-                    const int32_t ds = 2 - 4 * (int32_t)bj;
-                    m_part += ds;
-                    if (colour == 1) b_part += ds * ((int32_t)S - 6);
+            const uint32_t i0 = sg * kScSeg;
+            const uint32_t bits = rec[5];
+            const uint32_t flips = settle(lut, C, rec[1], rec[2], rec[3], rec[4], bits, g_row, xe,
+                                          i0, row, t, colour, k0, k1, m_part, b_part);
+            if (flips) {
+                // the owner stored the segment with the tied sites unchanged
+                if (BITS) {
+                    __builtin_amdgcn_raw_buffer_store_b32(bits ^ flips, rw_own,
+                                                          row * pitch + word_off<BITS>(i0), 0, 0);
+                } else {
+                    uint32_t f = flips;
+                    while (f) {
+                        const uint32_t j = __builtin_ctz(f);
+                        f &= f - 1u;
+                        own_plane[row * pitch + i0 + j] = (uint8_t)(((bits >> j) & 1u) ^ 1u);
+                    }
                 }
             }
-            // Bit-packed: the owner stored the row's 16 bits with the tied
-            // sites unchanged; this lane rewrites them with the accepted ones.
-            if (BITS && flips)
-                __builtin_amdgcn_raw_buffer_store_b16(
-                    (unsigned short)(pack16(rec[6], rec[7], rec[8], rec[9]) ^ flips), rw_own,
-                    row * pitch + seg_off<BITS>(sg * kIsingSeg), 0, 0);
         }
     }
 
@@ -400,17 +487,17 @@ template __global__ void ising_sc_pass<8, false>(const ChainDev*, const uint32_t
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
                                                   uint32_t, uint32_t);
 template __global__ void ising_sc_pass<1, true>(const ChainDev*, const uint32_t*, uint8_t*,
-                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                 int32_t*, uint32_t, uint32_t, uint32_t,
+                                                 uint32_t, uint32_t);
 template __global__ void ising_sc_pass<2, true>(const ChainDev*, const uint32_t*, uint8_t*,
-                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                 int32_t*, uint32_t, uint32_t, uint32_t,
+                                                 uint32_t, uint32_t);
 template __global__ void ising_sc_pass<4, true>(const ChainDev*, const uint32_t*, uint8_t*,
-                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                 int32_t*, uint32_t, uint32_t, uint32_t,
+                                                 uint32_t, uint32_t);
 template __global__ void ising_sc_pass<8, true>(const ChainDev*, const uint32_t*, uint8_t*,
-                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                 int32_t*, uint32_t, uint32_t, uint32_t,
+                                                 uint32_t, uint32_t);
 
 // Sums the pass partials of each (chain, sweep) over its ncol colour passes
 // into its record: record = {bond, 0, 0, mag, 0} (int64).  A pass adds the

@@ -134,7 +134,7 @@ extern "C" int vg_run_sums_noalloc(const vg_chain_desc* desc, double temperature
     G.total_blocks = d.nblocks;
     G.grid = (uint32_t)round_up(d.nblocks, vg::kNumXcd);
     G.prio = all_resident(g, tile, G.grid) ? 1u : 0u;
-    G.tie_cap = 64;
+    G.tie_cap = 128;
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (uint64_t done = 0; done < sweeps;) {
         const uint32_t nsw = (uint32_t)std::min<uint64_t>(chunk, sweeps - done);

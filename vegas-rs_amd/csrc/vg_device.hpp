@@ -101,6 +101,8 @@ constexpr float kFixScale = 134217728.0f;  // 2^27
 
 // Ising acceptance classes per current bit: S = 0 .. 2z, z <= 12.
 constexpr uint32_t kThrS = 25;
+// Class plans of the multispin sc pass: absent-slot counts 0..5 of a site.
+constexpr uint32_t kPlanG = 6;
 
 // ---------------------------------------------------------------------------
 // Chain descriptor as the kernels see it (one per chain of a kernel group).
@@ -131,14 +133,14 @@ struct ChainDev {
     // neighbour slots of 2*b_j (present) or 1 (absent), S in [0, 2z].
     uint32_t thr[2][kThrS];  // accept iff draw < thr (32-bit draw)
     uint32_t always[2];    // bit S set: class always accepted
-    // The same classes as byte tables for the packed sc decision, indexed by
-    // g (absent y/z neighbour slots, 0..4), b, and the up-neighbour count c
-    // (S = 2c + g): t8 = high byte of thr, or 0xFF when the class is always
-    // accepted (a draw byte of 0xFF then reads as a tie, which the tie
-    // resolver accepts without drawing); a8 = 0xFF when always accepted.
-    // Each 8-entry table is two words, entries 0-3 in the first.
-    uint32_t t8[5][2][2];
-    uint32_t a8[5][2][2];
+    // The multispin sc decision's class plans, per absent-slot count g =
+    // 0..5 of a site (p = 6 - g present neighbours) and aligned count a =
+    // 0..p (present neighbours equal to the site's bit b): up to two slots
+    // for the classes (b, a) that are not always accepted, 0 for none.  Slot
+    // bits: 3 covers b = 0, 4 covers b = 1, 5 the threshold is nonzero (a
+    // draw whose high byte equals bits 8-15 is a tie), 8-15 the threshold's
+    // high byte.  One slot covers both bits when those fields agree.
+    uint32_t plan[kPlanG][7][2];
     // Heisenberg parameters, fp32.
     float jn;              // -J
     float kk;              // K
@@ -163,7 +165,8 @@ struct ChainDev {
 };
 
 constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass
-constexpr uint32_t kIsingSeg = 16;     // positions per Ising row segment
+constexpr uint32_t kIsingSeg = 16;     // positions per Ising row segment (bcc, fcc)
+constexpr uint32_t kScSeg = 32;        // positions per sc Ising work item (one word of site bits)
 constexpr uint32_t kHeisSeg = 4;       // positions per Heisenberg work item
 constexpr uint32_t kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 
