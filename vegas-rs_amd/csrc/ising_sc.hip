@@ -13,9 +13,10 @@
 //
 // Work item: one 32-position segment of a row (bit j of every word below is
 // position i0 + j), marched through K (the tile, a template parameter picked
-// by the host) consecutive z planes.  Along z the row below is the previous
-// row's neighbour above, so each plane loads four rows (own, y-1, y+1, z+1)
-// instead of six.  Int8 rows are packed to words on load (v_dot4 with bit
+// by the host) consecutive z planes, upwards in even chunks of K planes and
+// downwards in odd ones.  Along z the row behind is the previous row's
+// neighbour ahead, so each plane loads four rows (own, y-1, y+1, the next
+// plane's) instead of six.  Int8 rows are packed to words on load (v_dot4 with bit
 // weights) and expanded on store; bit-packed rows are one 32-bit load.
 //
 // The decision works on whole words.  Each neighbour word is replaced by
@@ -270,13 +271,25 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         const bool xr_in = i0 + kScSeg < w;
         // bit of the last valid position, which takes the right neighbour's bit
         const uint32_t xr_sh = xr_in ? 31u : nvalid - 1u;
-        uint32_t off = (y + ly * z0) * pitch + word_off<BITS>(i0);  // row (y, z0), this segment
-        // the row window along z: row (y, z-1) and row (y, z)
-        const bool zm0_ok = z0 > 0 || pz;
-        uint32_t below =
-            load_word<BITS>(ro, !zm0_ok ? kNoRow : (z0 > 0 ? off - zstep : off + (lz - 1) * zstep));
+        // Even z chunks march up, odd ones down: two chunks meeting at a seam
+        // then read the seam planes in the same step, and the L2 fetches
+        // those rows once.
+        const bool down = zc & 1u;
+        const uint32_t zfirst = down ? z0 + nk - 1u : z0;
+        const int32_t dstep = down ? -(int32_t)zstep : (int32_t)zstep;
+        uint32_t off = (y + ly * zfirst) * pitch + word_off<BITS>(i0);  // this segment's row
+        // offset and presence of the row in plane z + dz (dz = +-1) of the
+        // other colour, for the row at offset of in plane z
+        const auto zrow = [&](uint32_t z, uint32_t of, bool up, bool& ok) {
+            ok = up ? (z + 1 < lz || pz) : (z > 0 || pz);
+            if (!ok) return kNoRow;
+            return up ? (z + 1 < lz ? of + zstep : of - (lz - 1) * zstep)
+                      : (z > 0 ? of - zstep : of + (lz - 1) * zstep);
+        };
+        // the row window along z: the plane behind the march and the current one
+        bool behind_ok;
+        uint32_t behind = load_word<BITS>(ro, zrow(zfirst, off, down, behind_ok));
         uint32_t here = load_word<BITS>(ro, off);
-        bool below_ok = zm0_ok;
 
         const bool rnd = C.proposal != 0;
         const uint32_t wP = ctr_w(t, kIsingProposal, colour);
@@ -289,9 +302,10 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             // marches, so the waves of a SIMD progress together and the pass
             // does not end in a tail of a few young waves.
             if (prio) set_prio(((K - 1u - k) * 4u) / K);
-            const uint32_t z = z0 + k;
+            const uint32_t z = down ? zfirst - k : z0 + k;
             const uint32_t row = y + ly * z;
-            const bool zp_ok = z + 1 < lz || pz;
+            bool ahead_ok;
+            const uint32_t ahead_off = zrow(z, off, !down, ahead_ok);
             // x neighbours of position i: i-1, i (q = 0) or i, i+1 (q = 1) in
             // the other plane's same row.  The bit outside the segment is
             // position i0-1 (q = 0), or i0+32, or position 0 across the
@@ -300,8 +314,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const RowRaw<BITS> c_o = load_raw<BITS>(rw_own, off);
             const RowRaw<BITS> c_ym = load_raw<BITS>(ro, ym_ok ? off + dym : kNoRow);
             const RowRaw<BITS> c_yp = load_raw<BITS>(ro, yp_ok ? off + dyp : kNoRow);
-            const RowRaw<BITS> c_ab = load_raw<BITS>(
-                ro, !zp_ok ? kNoRow : (z + 1 < lz ? off + zstep : off - (lz - 1) * zstep));
+            const RowRaw<BITS> c_ab = load_raw<BITS>(ro, ahead_off);
             const uint32_t extra =
                 load_pos<BITS>(ro, off - word_off<BITS>(i0), q ? (xr_in ? i0 + kScSeg : 0u) : xl_pos,
                                q ? (xr_in || px) : xl_ok);
@@ -322,8 +335,8 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t o = site_bits<BITS>(c_o);
             const uint32_t ym = site_bits<BITS>(c_ym);
             const uint32_t yp = site_bits<BITS>(c_yp);
-            const uint32_t above = site_bits<BITS>(c_ab);
-            const uint32_t g_row = gy + (below_ok ? 0u : 1u) + (zp_ok ? 0u : 1u);
+            const uint32_t ahead = site_bits<BITS>(c_ab);
+            const uint32_t g_row = gy + (behind_ok ? 0u : 1u) + (ahead_ok ? 0u : 1u);
             const uint32_t xe = q ? (!xr_in && !px ? nvalid - 1u : 32u) : (!xl_ok ? 0u : 32u);
             const uint32_t eb = xe < 32u ? 1u << xe : 0u;  // site with an absent x neighbour
             const uint32_t no = ~o;
@@ -333,8 +346,8 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t xb = q ? ((((mrow >> 1) | (extra << xr_sh)) & ~eb) | (no & eb)) : mrow;
             const uint32_t nym = ym_ok ? ym : no;
             const uint32_t nyp = yp_ok ? yp : no;
-            const uint32_t nzm = below_ok ? below : no;
-            const uint32_t nzp = zp_ok ? above : no;
+            const uint32_t nzm = behind_ok ? behind : no;
+            const uint32_t nzp = ahead_ok ? ahead : no;
             // aligned count a = 0..6 of the present neighbours equal to the
             // site (n aligned where ~(n ^ o)): two full adders over aligned
             // triples, then their sum
@@ -380,7 +393,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
                     if (slot < qcap) {
                         uint32_t* rec = q_wave[slot];
-                        rec[0] = lane | (k << 6) | (g_row << 9) | (xe << 12);
+                        rec[0] = lane | ((z - z0) << 6) | (g_row << 9) | (xe << 12);
                         rec[1] = tie;
                         rec[2] = a0;
                         rec[3] = a1;
@@ -413,10 +426,10 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                 const int32_t xf = (acc & eb) ? 1 : 0, xk = (kept & eb) ? 1 : 0;
                 b_part += 2 * (sak - saf) - p * ((int32_t)nvalid - 2 * nf) + xk - xf;
             }
-            below = here;
-            here = above;
-            below_ok = true;
-            off += zstep;
+            behind = here;
+            here = ahead;
+            behind_ok = true;
+            off += dstep;
         }
         m_part += 2 * (int32_t)ups - (int32_t)(nk * nvalid);
     }
@@ -434,14 +447,14 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         for (uint32_t e = lane; e < nq; e += 64) {
             const uint32_t* rec = q_wave[e];
             const uint32_t meta = rec[0];
-            const uint32_t owner = (wave << 6) | (meta & 63u), k = (meta >> 6) & 7u;
+            const uint32_t owner = (wave << 6) | (meta & 63u), kz = (meta >> 6) & 7u;
             const uint32_t g_row = (meta >> 9) & 7u, xe = (meta >> 12) & 63u;
             const uint32_t l = (b - C.block0) * kBlock + owner;
             const uint32_t r = fast_div(l, C.segs_m, C.segs_s);
             const uint32_t sg = l - r * C.segs;
             const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
             const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
-            const uint32_t zz = (r2 >> 1) * K + k;
+            const uint32_t zz = (r2 >> 1) * K + kz;
             const uint32_t row = yy + ly * zz;
             const uint32_t i0 = sg * kScSeg;
             const uint32_t bits = rec[5];
