@@ -355,6 +355,9 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
     G.tile = tile;
     G.total_blocks = (uint32_t)block_chain.size();
     G.grid = (uint32_t)round_up(G.total_blocks, vg::kNumXcd);
+    G.uniform_nb = G.host.empty() ? 0u : G.host[0].nblocks;
+    for (const ChainDev& d : G.host)
+        if (d.nblocks != G.uniform_nb) G.uniform_nb = 0;
     if (is_zmarch(g)) G.prio = all_resident(g, tile, G.grid) ? 1u : 0u;
     G.partial_bytes_per_sweep = std::max<uint64_t>(
         256, (uint64_t)passes_of(G, g) * G.total_blocks * quantities(g) * quantity_bytes(g));
@@ -393,22 +396,22 @@ void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
         case 1:
             hipLaunchKernelGGL((vg::ising_sc_pass<1, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8));
+                               G.prio | (G.tie_cap << 8), G.uniform_nb);
             break;
         case 2:
             hipLaunchKernelGGL((vg::ising_sc_pass<2, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8));
+                               G.prio | (G.tie_cap << 8), G.uniform_nb);
             break;
         case 8:
             hipLaunchKernelGGL((vg::ising_sc_pass<8, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8));
+                               G.prio | (G.tie_cap << 8), G.uniform_nb);
             break;
         default:
             hipLaunchKernelGGL((vg::ising_sc_pass<4, BITS>), grid, block, 0, st, G.d_chains,
                                G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8));
+                               G.prio | (G.tie_cap << 8), G.uniform_nb);
             break;
     }
 }

@@ -18,9 +18,11 @@
 #include "vg_device.hpp"
 
 namespace vg {
+// sc pass args: (chains, block -> chain map, arena, partials, colour, sweep,
+// total blocks, grid, options, blocks per chain when uniform else 0).
 template <uint32_t K, bool BITS>
 __global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
-                              uint32_t, uint32_t, uint32_t, uint32_t);
+                              uint32_t, uint32_t, uint32_t, uint32_t, uint32_t);
 __global__ void ising_init_bits(const ChainDev*, uint8_t*);
 __global__ void ising_scatter_bits(const ChainDev*, uint8_t*, const int8_t*);
 __global__ void ising_gather_bits(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
@@ -156,6 +158,9 @@ struct Group {
     uint32_t tile = 4;  // Ising sc/bcc/fcc (int8, bits): z planes per work item (template parameter)
     uint32_t total_blocks = 0;
     uint32_t grid = 0;  // total_blocks rounded up to a multiple of 8
+    // blocks per chain when every chain of the group has the same number (a
+    // pass then finds a block's chain without the map), else 0
+    uint32_t uniform_nb = 0;
     uint32_t prio = 0;  // z-marching Ising passes: every workgroup resident at once (wave priorities on)
     // z-marching Ising passes: tie records a wave may queue (each pass holds
     // at most its own queue: 128 records in the sc pass, 64 in bcc and fcc)

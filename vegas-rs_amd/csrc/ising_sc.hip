@@ -180,7 +180,7 @@ __device__ __forceinline__ void decide(const ChainDev& C, uint32_t g, uint32_t a
 // ob, aligned count from a0..a2, absent slots g_row (+1 at position xe), so
 // its class is S = 2c + g with c = a (b = 1) or 6 - g - a (b = 0).  Returns
 // the accepted flips; dm, db collect their magnetisation and bond changes.
-__device__ __forceinline__ uint32_t settle(const uint32_t* lut, const ChainDev& C, uint32_t tm,
+__device__ __forceinline__ uint32_t settle(const uint32_t* thr, const ChainDev& C, uint32_t tm,
                                            uint32_t a0, uint32_t a1, uint32_t a2, uint32_t ob,
                                            uint32_t g_row, uint32_t xe, uint32_t i0,
                                            uint32_t row, uint64_t t, uint32_t colour,
@@ -193,7 +193,7 @@ __device__ __forceinline__ uint32_t settle(const uint32_t* lut, const ChainDev& 
         const uint32_t a = ((a0 >> j) & 1u) | (((a1 >> j) & 1u) << 1) | (((a2 >> j) & 1u) << 2);
         const uint32_t g = g_row + (j == xe ? 1u : 0u);
         const uint32_t S = 2u * (bj ? a : 6u - g - a) + g;
-        if (tie_accept(lut, C.always[bj], bj, S, i0 + j, row, t, colour, k0, k1)) {
+        if (tie_accept(thr, C.always[bj], bj, S, i0 + j, row, t, colour, k0, k1)) {
             flips |= 1u << j;
             const int32_t ds = 2 - 4 * (int32_t)bj;
             dm += ds;
@@ -210,8 +210,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                      int32_t* __restrict__ partials,
                                                      uint32_t colour, uint32_t s,
                                                      uint32_t total_blocks, uint32_t grid,
-                                                     uint32_t opts) {
-    __shared__ uint32_t lut[2 * kThrS];
+                                                     uint32_t opts, uint32_t uniform_nb) {
     // per wave: kTieRows records of {meta, tie mask, a0, a1, a2, site bits}
     __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
     // opts: bit 0 = falling wave priorities; bits 8-15 = records the tie
@@ -220,24 +219,47 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     const uint32_t prio = opts & 1u;
     const uint32_t qcap = min((opts >> 8) & 0xFFu, kTieRows);
 
+    // Blocks past the work (the grid is a multiple of 8) and chains that run
+    // fewer sweeps in this block do nothing; they are not sent away here so
+    // that every load below is issued at once.  Uniform over the workgroup.
     const uint32_t b = xcd_block(blockIdx.x, grid);
-    if (b >= total_blocks) return;  // uniform over the workgroup
-    const ChainDev& C = chains[block_chain[b]];
-    if (s >= C.limit) return;  // uniform: the workgroup belongs to one chain
+    const uint32_t bc = min(b, total_blocks - 1u);
+    const ChainDev& C = chains[uniform_nb ? bc / uniform_nb : block_chain[bc]];
+    // The descriptor's leading fields (all but the tables), loaded together:
+    // a pass is short, and a chain of dependent loads at its start is a
+    // good part of it.
+    constexpr uint32_t kHead = offsetof(ChainDev, thr) / 4u;
+    uint32_t hd[kHead];
+#pragma unroll
+    for (uint32_t k = 0; k < kHead; ++k) hd[k] = reinterpret_cast<const uint32_t*>(&C)[k];
+#define VG_FIELD(f) hd[offsetof(ChainDev, f) / 4u]
+#define VG_FIELD64(f) \
+    (((uint64_t)hd[offsetof(ChainDev, f) / 4u + 1u] << 32) | hd[offsetof(ChainDev, f) / 4u])
+    const bool active = b < total_blocks && s < VG_FIELD(limit);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid < 2 * kThrS) lut[tid] = (&C.thr[0][0])[tid];
-    __syncthreads();
+    const uint32_t* const thr = &C.thr[0][0];  // read by tie settling only
 
-    const uint64_t t = C.sweep0 + s;
-    const uint32_t k0 = (uint32_t)C.seed, k1 = (uint32_t)(C.seed >> 32);
-    const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
+    const uint64_t t = VG_FIELD64(sweep0) + s;
+    const uint64_t seed = VG_FIELD64(seed);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t ly = VG_FIELD(ly), lz = VG_FIELD(lz), w = VG_FIELD(w);
+    const uint32_t pitch = VG_FIELD(pitch), pbc = VG_FIELD(pbc);
+    const uint32_t items = VG_FIELD(items), block0 = VG_FIELD(block0);
+    const uint32_t segs = VG_FIELD(segs), segs_m = VG_FIELD(segs_m), segs_s = VG_FIELD(segs_s);
+    const uint32_t hy = VG_FIELD(hy), hy_m = VG_FIELD(hy_m), hy_s = VG_FIELD(hy_s);
+    const uint32_t plane = (uint32_t)VG_FIELD64(plane);
+    const uint64_t off_own = colour ? VG_FIELD64(off[1]) : VG_FIELD64(off[0]);
+    const uint64_t off_oth = colour ? VG_FIELD64(off[0]) : VG_FIELD64(off[1]);
+    const bool rnd = VG_FIELD(proposal) != 0;
+#undef VG_FIELD
+#undef VG_FIELD64
     // A colour plane is below 4 GiB (checked on the host): 32-bit offsets.
-    uint8_t* const own_plane = arena + C.off[colour];
-    const uint8_t* const oth = arena + C.off[colour ^ 1u];
-    const __amdgpu_buffer_rsrc_t rw_own = plane_rsrc(own_plane, (uint32_t)C.plane);
-    const __amdgpu_buffer_rsrc_t ro = plane_rsrc(oth, (uint32_t)C.plane);
+    uint8_t* const own_plane = arena + off_own;
+    const uint8_t* const oth = arena + off_oth;
+    const __amdgpu_buffer_rsrc_t rw_own = plane_rsrc(own_plane, plane);
+    const __amdgpu_buffer_rsrc_t ro = plane_rsrc(oth, plane);
     const uint32_t zstep = ly * pitch;  // bytes between planes z and z+1
-    const uint32_t local = (b - C.block0) * kBlock + tid;
+    const uint32_t local = (b - block0) * kBlock + tid;
     const uint32_t wA = ctr_w(t, kIsingA, colour);
     uint32_t (*const q_wave)[kTieWords] = tie_q[wave];
     uint32_t nrec = 0;  // records queued by this wave, as counted by this lane
@@ -245,14 +267,14 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     int32_t m_part = 0, b_part = 0;
     uint32_t ups = 0;  // up sites after each plane's update, summed
     uint32_t seg = 0, y = 0, zc = 0;
-    if (local < C.items) {
-        const uint32_t r = fast_div(local, C.segs_m, C.segs_s);
-        seg = local - r * C.segs;
-        const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
-        y = 2u * (r - r2 * C.hy) + (r2 & 1u);
+    if (active && local < items) {
+        const uint32_t r = fast_div(local, segs_m, segs_s);
+        seg = local - r * segs;
+        const uint32_t r2 = fast_div(r, hy_m, hy_s);
+        y = 2u * (r - r2 * hy) + (r2 & 1u);
         zc = r2 >> 1;
     }
-    if (local < C.items && y < ly) {
+    if (active && local < items && y < ly) {
         const uint32_t z0 = zc * K;
         const uint32_t nk = min(K, lz - z0);
         const uint32_t i0 = seg * kScSeg;
@@ -291,7 +313,6 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         uint32_t behind = load_word<BITS>(ro, zrow(zfirst, off, down, behind_ok));
         uint32_t here = load_word<BITS>(ro, off);
 
-        const bool rnd = C.proposal != 0;
         const uint32_t wP = ctr_w(t, kIsingProposal, colour);
         // not unrolled: one plane's code is large, and the instruction
         // cache is shared by the waves of two CUs
@@ -403,7 +424,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                         // Queue full (rare: a wave marching 8 planes near
                         // T_c queues a few dozen records against 128 slots).
                         int32_t dm = 0, db = 0;
-                        acc |= settle(lut, C, tie, a0, a1, a2, o, g_row, xe, i0, row, t, colour,
+                        acc |= settle(thr, C, tie, a0, a1, a2, o, g_row, xe, i0, row, t, colour,
                                       k0, k1, dm, db);
                     }
                 }
@@ -449,16 +470,16 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t meta = rec[0];
             const uint32_t owner = (wave << 6) | (meta & 63u), kz = (meta >> 6) & 7u;
             const uint32_t g_row = (meta >> 9) & 7u, xe = (meta >> 12) & 63u;
-            const uint32_t l = (b - C.block0) * kBlock + owner;
-            const uint32_t r = fast_div(l, C.segs_m, C.segs_s);
-            const uint32_t sg = l - r * C.segs;
-            const uint32_t r2 = fast_div(r, C.hy_m, C.hy_s);
-            const uint32_t yy = 2u * (r - r2 * C.hy) + (r2 & 1u);
+            const uint32_t l = (b - block0) * kBlock + owner;
+            const uint32_t r = fast_div(l, segs_m, segs_s);
+            const uint32_t sg = l - r * segs;
+            const uint32_t r2 = fast_div(r, hy_m, hy_s);
+            const uint32_t yy = 2u * (r - r2 * hy) + (r2 & 1u);
             const uint32_t zz = (r2 >> 1) * K + kz;
             const uint32_t row = yy + ly * zz;
             const uint32_t i0 = sg * kScSeg;
             const uint32_t bits = rec[5];
-            const uint32_t flips = settle(lut, C, rec[1], rec[2], rec[3], rec[4], bits, g_row, xe,
+            const uint32_t flips = settle(thr, C, rec[1], rec[2], rec[3], rec[4], bits, g_row, xe,
                                           i0, row, t, colour, k0, k1, m_part, b_part);
             if (flips) {
                 // the owner stored the segment with the tied sites unchanged
@@ -479,7 +500,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
 
     m_part = wave_sum(m_part);
     b_part = wave_sum(b_part);
-    if (lane == 0) {
+    if (active && lane == 0) {
         int32_t* p = partials + ((uint64_t)(s * 2u + colour) * total_blocks + b) * (2u * kWaves) +
                      2u * wave;
         p[0] = m_part;
@@ -489,28 +510,28 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
 
 template __global__ void ising_sc_pass<1, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                  uint32_t, uint32_t, uint32_t);
 template __global__ void ising_sc_pass<2, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                  uint32_t, uint32_t, uint32_t);
 template __global__ void ising_sc_pass<4, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                  uint32_t, uint32_t, uint32_t);
 template __global__ void ising_sc_pass<8, false>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
-                                                  uint32_t, uint32_t);
+                                                  uint32_t, uint32_t, uint32_t);
 template __global__ void ising_sc_pass<1, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                 uint32_t, uint32_t);
+                                                 uint32_t, uint32_t, uint32_t);
 template __global__ void ising_sc_pass<2, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                 uint32_t, uint32_t);
+                                                 uint32_t, uint32_t, uint32_t);
 template __global__ void ising_sc_pass<4, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                 uint32_t, uint32_t);
+                                                 uint32_t, uint32_t, uint32_t);
 template __global__ void ising_sc_pass<8, true>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
-                                                 uint32_t, uint32_t);
+                                                 uint32_t, uint32_t, uint32_t);
 
 // Sums the pass partials of each (chain, sweep) over its ncol colour passes
 // into its record: record = {bond, 0, 0, mag, 0} (int64).  A pass adds the

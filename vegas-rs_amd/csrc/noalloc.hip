@@ -132,6 +132,7 @@ extern "C" int vg_run_sums_noalloc(const vg_chain_desc* desc, double temperature
     G.arena = spins;
     G.tile = tile;
     G.total_blocks = d.nblocks;
+    G.uniform_nb = d.nblocks;
     G.grid = (uint32_t)round_up(d.nblocks, vg::kNumXcd);
     G.prio = all_resident(g, tile, G.grid) ? 1u : 0u;
     G.tie_cap = 128;
