@@ -268,6 +268,15 @@ uint64_t colour_elems(int g, const ChainDev& d) { return (is_ising(g) ? 1 : 3) *
 uint32_t quantities(int group) { return is_ising(group) && !is_graph(group) ? 8u : 5u; }
 uint32_t quantity_bytes(int group) { return is_ising(group) && !is_graph(group) ? 4u : 8u; }
 
+// After the int8 planes of an sc chain with shadows were written: rebuild
+// its shadows (one thread per shadow word).
+void launch_shadow_build(const Group& G, const ChainDev& d, const ChainDev* dp, hipStream_t st) {
+    if (!d.shadow) return;
+    const uint64_t n = 2ull * d.rows * (d.spitch / 4);
+    hipLaunchKernelGGL(vg::ising_shadow_build, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                       dp, (uint8_t*)G.arena);
+}
+
 void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev* dp,
                        hipStream_t st) {
     const uint64_t n = (uint64_t)d.ncol * d.plane;  // one thread per plane byte
@@ -276,6 +285,7 @@ void launch_ising_init(int g, const Group& G, const ChainDev& d, const ChainDev*
         hipLaunchKernelGGL(vg::ising_init_bits, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
     else
         hipLaunchKernelGGL(vg::ising_init, grid, dim3(256), 0, st, dp, (uint8_t*)G.arena);
+    if (g == kIsingSc) launch_shadow_build(G, d, dp, st);
 }
 
 int upload_descriptors(vg_batch* B, int g, int slot) {
@@ -293,12 +303,12 @@ constexpr uint32_t kDefaultTile = 4;
 template <uint32_t K>
 const void* zmarch_fn(int g) {
     switch (g) {
-        case kIsingBits: return (const void*)&vg::ising_sc_pass<K, true>;
+        case kIsingBits: return (const void*)&vg::ising_sc_pass<K, vg::kScBits>;
         case kIsingBcc: return (const void*)&vg::ising_bcc_pass<K, false>;
         case kIsingBccBits: return (const void*)&vg::ising_bcc_pass<K, true>;
         case kIsingFcc: return (const void*)&vg::ising_fcc_pass<0, K, false>;
         case kIsingFccBits: return (const void*)&vg::ising_fcc_pass<0, K, true>;
-        default: return (const void*)&vg::ising_sc_pass<K, false>;
+        default: return (const void*)&vg::ising_sc_pass<K, vg::kScShadow>;
     }
 }
 
@@ -387,33 +397,46 @@ int free_batch(vg_batch* B) {
     return VG_OK;
 }
 
-template <bool BITS>
-void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
+template <uint32_t MODE>
+void launch_sc_mode(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
     const dim3 grid(G.grid), block(vg::kBlock);
     uint8_t* arena = (uint8_t*)G.arena;
     int32_t* part = (int32_t*)G.partials;
+    const uint32_t opts = G.prio | (G.tie_cap << 8);
     switch (G.tile) {
         case 1:
-            hipLaunchKernelGGL((vg::ising_sc_pass<1, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8), G.uniform_nb);
+            hipLaunchKernelGGL((vg::ising_sc_pass<1, MODE>), grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid, opts,
+                               G.uniform_nb);
             break;
         case 2:
-            hipLaunchKernelGGL((vg::ising_sc_pass<2, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8), G.uniform_nb);
+            hipLaunchKernelGGL((vg::ising_sc_pass<2, MODE>), grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid, opts,
+                               G.uniform_nb);
             break;
         case 8:
-            hipLaunchKernelGGL((vg::ising_sc_pass<8, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8), G.uniform_nb);
+            hipLaunchKernelGGL((vg::ising_sc_pass<8, MODE>), grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid, opts,
+                               G.uniform_nb);
             break;
         default:
-            hipLaunchKernelGGL((vg::ising_sc_pass<4, BITS>), grid, block, 0, st, G.d_chains,
-                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid,
-                               G.prio | (G.tie_cap << 8), G.uniform_nb);
+            hipLaunchKernelGGL((vg::ising_sc_pass<4, MODE>), grid, block, 0, st, G.d_chains,
+                               G.d_block_chain, arena, part, c, s, G.total_blocks, G.grid, opts,
+                               G.uniform_nb);
             break;
     }
+}
+
+// One colour pass of an sc Ising group: bit-packed planes, or int8 planes
+// with or without shadows.
+template <bool BITS>
+void launch_ising_pass(const Group& G, hipStream_t st, uint32_t c, uint32_t s) {
+    if (BITS)
+        launch_sc_mode<vg::kScBits>(G, st, c, s);
+    else if (G.shadow)
+        launch_sc_mode<vg::kScShadow>(G, st, c, s);
+    else
+        launch_sc_mode<vg::kScInt8>(G, st, c, s);
 }
 
 template <bool BITS>
@@ -960,6 +983,16 @@ int create_impl(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, cons
                 d.off[c] = offset;
                 offset += colour_elems(g, d);
             }
+            if (g == kIsingSc) {  // bit-packed shadows of the two planes
+                d.spitch = (uint32_t)round_up((d.w + 31) / 32 * 4, 16);
+                for (uint32_t c = 0; c < 2; ++c) {
+                    offset = round_up(offset, elem_align);
+                    d.soff[c] = offset;
+                    offset += (uint64_t)d.rows * d.spitch;
+                }
+                d.shadow = 1;
+                G.shadow = true;
+            }
         }
         G.arena_bytes = std::max<uint64_t>(offset * elem_bytes(g), 256);
         hipError_t e = hipMalloc(&G.arena, G.arena_bytes);
@@ -1187,6 +1220,8 @@ int vg_batch_set_ising(vg_batch* batch, uint32_t chain, const int8_t* spins) {
             hipLaunchKernelGGL(vg::ising_scatter, dim3((uint32_t)((ch.n_sites + 255) / 256)),
                                dim3(256), 0, st, G.d_chains + ch.index, (uint8_t*)G.arena, tmp,
                                ch.n_sites);
+            if (ch.group == kIsingSc)
+                launch_shadow_build(G, G.host[ch.index], G.d_chains + ch.index, st);
         }
         e = hipGetLastError();
     }

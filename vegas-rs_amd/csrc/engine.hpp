@@ -20,9 +20,11 @@
 namespace vg {
 // sc pass args: (chains, block -> chain map, arena, partials, colour, sweep,
 // total blocks, grid, options, blocks per chain when uniform else 0).
-template <uint32_t K, bool BITS>
+template <uint32_t K, uint32_t MODE>
 __global__ void ising_sc_pass(const ChainDev*, const uint32_t*, uint8_t*, int32_t*, uint32_t,
                               uint32_t, uint32_t, uint32_t, uint32_t, uint32_t);
+// Packs the int8 sc planes of one chain into its shadows.
+__global__ void ising_shadow_build(const ChainDev*, uint8_t*);
 __global__ void ising_init_bits(const ChainDev*, uint8_t*);
 __global__ void ising_scatter_bits(const ChainDev*, uint8_t*, const int8_t*);
 __global__ void ising_gather_bits(const ChainDev*, const uint8_t*, int8_t*, uint64_t);
@@ -175,6 +177,9 @@ struct Group {
     uint32_t chains_cap = 0;
     uint64_t partial_cap = 0;
     bool shared_arena = false;  // arena is owned by someone else (the server's pool)
+    // int8 sc chains with bit-packed plane shadows (batches; the server's
+    // pooled chains have none)
+    bool shadow = false;
 };
 
 struct ChainHost {

@@ -203,7 +203,7 @@ __device__ __forceinline__ uint32_t settle(const uint32_t* thr, const ChainDev& 
     return flips;
 }
 
-template <uint32_t K, bool BITS>
+template <uint32_t K, uint32_t MODE>
 __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict__ chains,
                                                      const uint32_t* __restrict__ block_chain,
                                                      uint8_t* __restrict__ arena,
@@ -211,6 +211,11 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                                      uint32_t colour, uint32_t s,
                                                      uint32_t total_blocks, uint32_t grid,
                                                      uint32_t opts, uint32_t uniform_nb) {
+    // storage of the own rows and of the neighbour rows; int8 planes with
+    // shadows also store each updated row's bits in their own shadow
+    constexpr bool BITS = MODE == kScBits;
+    constexpr bool NB = MODE != kScInt8;
+    constexpr bool SH = MODE == kScShadow;
     // per wave: kTieRows records of {meta, tie mask, a0, a1, a2, site bits}
     __shared__ uint32_t tie_q[kWaves][kTieRows][kTieWords];
     // opts: bit 0 = falling wave priorities; bits 8-15 = records the tie
@@ -248,6 +253,9 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     const uint32_t segs = VG_FIELD(segs), segs_m = VG_FIELD(segs_m), segs_s = VG_FIELD(segs_s);
     const uint32_t hy = VG_FIELD(hy), hy_m = VG_FIELD(hy_m), hy_s = VG_FIELD(hy_s);
     const uint32_t plane = (uint32_t)VG_FIELD64(plane);
+    const uint32_t spitch = VG_FIELD(spitch), rows = VG_FIELD(rows);
+    const uint64_t soff_own = colour ? VG_FIELD64(soff[1]) : VG_FIELD64(soff[0]);
+    const uint64_t soff_oth = colour ? VG_FIELD64(soff[0]) : VG_FIELD64(soff[1]);
     const uint64_t off_own = colour ? VG_FIELD64(off[1]) : VG_FIELD64(off[0]);
     const uint64_t off_oth = colour ? VG_FIELD64(off[0]) : VG_FIELD64(off[1]);
     const bool rnd = VG_FIELD(proposal) != 0;
@@ -257,8 +265,13 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     uint8_t* const own_plane = arena + off_own;
     const uint8_t* const oth = arena + off_oth;
     const __amdgpu_buffer_rsrc_t rw_own = plane_rsrc(own_plane, plane);
-    const __amdgpu_buffer_rsrc_t ro = plane_rsrc(oth, plane);
-    const uint32_t zstep = ly * pitch;  // bytes between planes z and z+1
+    // neighbour rows: the other plane, or its shadow
+    const __amdgpu_buffer_rsrc_t ro =
+        SH ? plane_rsrc(arena + soff_oth, rows * spitch) : plane_rsrc(oth, plane);
+    const __amdgpu_buffer_rsrc_t rw_sh = plane_rsrc(arena + soff_own, SH ? rows * spitch : 0u);
+    const uint32_t zstep = ly * pitch;  // bytes between planes z and z+1 (own rows)
+    const uint32_t npitch = SH ? spitch : pitch;  // the same for neighbour rows
+    const uint32_t nzstep = ly * npitch;
     const uint32_t local = (b - block0) * kBlock + tid;
     const uint32_t wA = ctr_w(t, kIsingA, colour);
     uint32_t (*const q_wave)[kTieWords] = tie_q[wave];
@@ -284,8 +297,8 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         const bool ym_ok = y > 0 || py;
         const bool yp_ok = y + 1 < ly || py;
         // offsets of rows y-1 and y+1 relative to row y (wrapping when periodic)
-        const int32_t dym = y > 0 ? -(int32_t)pitch : (int32_t)((ly - 1) * pitch);
-        const int32_t dyp = y + 1 < ly ? (int32_t)pitch : -(int32_t)((ly - 1) * pitch);
+        const int32_t dym = y > 0 ? -(int32_t)npitch : (int32_t)((ly - 1) * npitch);
+        const int32_t dyp = y + 1 < ly ? (int32_t)npitch : -(int32_t)((ly - 1) * npitch);
         const uint32_t gy = (ym_ok ? 0u : 1u) + (yp_ok ? 0u : 1u);
         // x neighbour outside the 32 positions: i0-1 (q = 0) or i0+32 (q = 1)
         const bool xl_ok = i0 > 0 || px;
@@ -299,19 +312,21 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         const bool down = zc & 1u;
         const uint32_t zfirst = down ? z0 + nk - 1u : z0;
         const int32_t dstep = down ? -(int32_t)zstep : (int32_t)zstep;
+        const int32_t ndstep = down ? -(int32_t)nzstep : (int32_t)nzstep;
         uint32_t off = (y + ly * zfirst) * pitch + word_off<BITS>(i0);  // this segment's row
+        uint32_t noff = (y + ly * zfirst) * npitch + word_off<NB>(i0);  // the same, neighbour rows
         // offset and presence of the row in plane z + dz (dz = +-1) of the
         // other colour, for the row at offset of in plane z
         const auto zrow = [&](uint32_t z, uint32_t of, bool up, bool& ok) {
             ok = up ? (z + 1 < lz || pz) : (z > 0 || pz);
             if (!ok) return kNoRow;
-            return up ? (z + 1 < lz ? of + zstep : of - (lz - 1) * zstep)
-                      : (z > 0 ? of - zstep : of + (lz - 1) * zstep);
+            return up ? (z + 1 < lz ? of + nzstep : of - (lz - 1) * nzstep)
+                      : (z > 0 ? of - nzstep : of + (lz - 1) * nzstep);
         };
         // the row window along z: the plane behind the march and the current one
         bool behind_ok;
-        uint32_t behind = load_word<BITS>(ro, zrow(zfirst, off, down, behind_ok));
-        uint32_t here = load_word<BITS>(ro, off);
+        uint32_t behind = load_word<NB>(ro, zrow(zfirst, noff, down, behind_ok));
+        uint32_t here = load_word<NB>(ro, noff);
 
         const uint32_t wP = ctr_w(t, kIsingProposal, colour);
         // not unrolled: one plane's code is large, and the instruction
@@ -326,18 +341,18 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t z = down ? zfirst - k : z0 + k;
             const uint32_t row = y + ly * z;
             bool ahead_ok;
-            const uint32_t ahead_off = zrow(z, off, !down, ahead_ok);
+            const uint32_t ahead_off = zrow(z, noff, !down, ahead_ok);
             // x neighbours of position i: i-1, i (q = 0) or i, i+1 (q = 1) in
             // the other plane's same row.  The bit outside the segment is
             // position i0-1 (q = 0), or i0+32, or position 0 across the
             // periodic end (q = 1); it reads 0 when x is open there.
             const uint32_t q = (colour + y + z) & 1u;
             const RowRaw<BITS> c_o = load_raw<BITS>(rw_own, off);
-            const RowRaw<BITS> c_ym = load_raw<BITS>(ro, ym_ok ? off + dym : kNoRow);
-            const RowRaw<BITS> c_yp = load_raw<BITS>(ro, yp_ok ? off + dyp : kNoRow);
-            const RowRaw<BITS> c_ab = load_raw<BITS>(ro, ahead_off);
+            const RowRaw<NB> c_ym = load_raw<NB>(ro, ym_ok ? noff + dym : kNoRow);
+            const RowRaw<NB> c_yp = load_raw<NB>(ro, yp_ok ? noff + dyp : kNoRow);
+            const RowRaw<NB> c_ab = load_raw<NB>(ro, ahead_off);
             const uint32_t extra =
-                load_pos<BITS>(ro, off - word_off<BITS>(i0), q ? (xr_in ? i0 + kScSeg : 0u) : xl_pos,
+                load_pos<NB>(ro, noff - word_off<NB>(i0), q ? (xr_in ? i0 + kScSeg : 0u) : xl_pos,
                                q ? (xr_in || px) : xl_ok);
 
             // draws first: they do not wait for the loads
@@ -354,9 +369,9 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t d[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 
             const uint32_t o = site_bits<BITS>(c_o);
-            const uint32_t ym = site_bits<BITS>(c_ym);
-            const uint32_t yp = site_bits<BITS>(c_yp);
-            const uint32_t ahead = site_bits<BITS>(c_ab);
+            const uint32_t ym = site_bits<NB>(c_ym);
+            const uint32_t yp = site_bits<NB>(c_yp);
+            const uint32_t ahead = site_bits<NB>(c_ab);
             const uint32_t g_row = gy + (behind_ok ? 0u : 1u) + (ahead_ok ? 0u : 1u);
             const uint32_t xe = q ? (!xr_in && !px ? nvalid - 1u : 32u) : (!xl_ok ? 0u : 32u);
             const uint32_t eb = xe < 32u ? 1u << xe : 0u;  // site with an absent x neighbour
@@ -431,7 +446,10 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                 nrec += __builtin_popcountll(has);
             }
             const uint32_t n = o ^ acc;
-            if (acc) store_word<BITS>(rw_own, off, n);
+            if (acc) {
+                store_word<BITS>(rw_own, off, n);
+                if (SH) __builtin_amdgcn_raw_buffer_store_b32(n, rw_sh, noff, 0, 0);
+            }
             ups += __popc(n & vmask);
             if (colour == 1) {
                 // sum over the segment's sites of s_j * (sum of its present
@@ -451,6 +469,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             here = ahead;
             behind_ok = true;
             off += dstep;
+            noff += ndstep;
         }
         m_part += 2 * (int32_t)ups - (int32_t)(nk * nvalid);
     }
@@ -483,6 +502,9 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
                                           i0, row, t, colour, k0, k1, m_part, b_part);
             if (flips) {
                 // the owner stored the segment with the tied sites unchanged
+                if (SH)
+                    __builtin_amdgcn_raw_buffer_store_b32(bits ^ flips, rw_sh,
+                                                          row * spitch + (i0 >> 3), 0, 0);
                 if (BITS) {
                     __builtin_amdgcn_raw_buffer_store_b32(bits ^ flips, rw_own,
                                                           row * pitch + word_off<BITS>(i0), 0, 0);
@@ -508,30 +530,63 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     }
 }
 
-template __global__ void ising_sc_pass<1, false>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<1, kScInt8>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
                                                   uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<2, false>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<2, kScInt8>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
                                                   uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<4, false>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<4, kScInt8>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
                                                   uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<8, false>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<8, kScInt8>(const ChainDev*, const uint32_t*, uint8_t*,
                                                   int32_t*, uint32_t, uint32_t, uint32_t,
                                                   uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<1, true>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<1, kScBits>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
                                                  uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<2, true>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<2, kScBits>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
                                                  uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<4, true>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<4, kScBits>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
                                                  uint32_t, uint32_t, uint32_t);
-template __global__ void ising_sc_pass<8, true>(const ChainDev*, const uint32_t*, uint8_t*,
+template __global__ void ising_sc_pass<8, kScBits>(const ChainDev*, const uint32_t*, uint8_t*,
                                                  int32_t*, uint32_t, uint32_t, uint32_t,
                                                  uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<1, kScShadow>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<2, kScShadow>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<4, kScShadow>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t, uint32_t, uint32_t);
+template __global__ void ising_sc_pass<8, kScShadow>(const ChainDev*, const uint32_t*, uint8_t*,
+                                                     int32_t*, uint32_t, uint32_t, uint32_t,
+                                                     uint32_t, uint32_t, uint32_t);
+
+// Packs the int8 planes of an sc chain into its shadows: one thread per
+// shadow word (bit j of word k of a row = position 32k + j; positions past
+// the row read 0).
+__global__ void ising_shadow_build(const ChainDev* __restrict__ chain, uint8_t* __restrict__ arena) {
+    const ChainDev& C = *chain;
+    const uint32_t wpr = C.spitch / 4u;  // words per shadow row
+    const uint64_t per = (uint64_t)C.rows * wpr;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 2u * per) return;
+    const uint32_t c = (uint32_t)(idx / per);
+    const uint64_t e = idx - c * per;
+    const uint32_t row = (uint32_t)(e / wpr), k = (uint32_t)(e % wpr);
+    const uint8_t* src = arena + C.off[c] + (uint64_t)row * C.pitch;
+    uint32_t v = 0;
+    for (uint32_t j = 0; j < 32u; ++j) {
+        const uint32_t i = 32u * k + j;
+        if (i < C.w && src[i]) v |= 1u << j;
+    }
+    *reinterpret_cast<uint32_t*>(arena + C.soff[c] + (uint64_t)row * C.spitch + 4u * k) = v;
+}
 
 // Sums the pass partials of each (chain, sweep) over its ncol colour passes
 // into its record: record = {bond, 0, 0, mag, 0} (int64).  A pass adds the

@@ -129,6 +129,13 @@ struct ChainDev {
     uint32_t hy;           // (ly + 1) / 2, rows of one parity class per plane
     uint32_t hy_m, hy_s;
     uint32_t nzc;          // z chunks of one tile of planes
+    // int8 sc chains of a batch: bit-packed shadows of the two colour planes
+    // (bit j of word k of a row = position 32k + j; rows of spitch bytes at
+    // byte offsets soff[c] of the arena), equal to the int8 planes after
+    // every kernel that writes those; the colour passes read the neighbour
+    // rows from the other colour's shadow.  shadow = 0: no shadows.
+    uint32_t spitch, shadow;
+    uint64_t soff[2];
     // Ising acceptance: class (b, S) with b the current bit and S the sum over
     // neighbour slots of 2*b_j (present) or 1 (absent), S in [0, 2z].
     uint32_t thr[2][kThrS];  // accept iff draw < thr (32-bit draw)
@@ -167,6 +174,9 @@ struct ChainDev {
 constexpr uint32_t kBlock = 256;       // threads per workgroup of a pass
 constexpr uint32_t kIsingSeg = 16;     // positions per Ising row segment (bcc, fcc)
 constexpr uint32_t kScSeg = 32;        // positions per sc Ising work item (one word of site bits)
+// sc pass modes: int8 planes; bit-packed planes; int8 planes whose
+// neighbour rows are read from bit-packed shadows.
+enum ScMode : uint32_t { kScInt8 = 0, kScBits = 1, kScShadow = 2 };
 constexpr uint32_t kHeisSeg = 4;       // positions per Heisenberg work item
 constexpr uint32_t kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 
