@@ -131,7 +131,10 @@ __global__ __launch_bounds__(256) void graph_heis_pass(const ChainDev* __restric
                 pr[2] = -o[2];
             }
             float n[3];
-            heis_update(C, false, o, h, hb, u, pr, n, acc);
+            int32_t s32[4] = {0, 0, 0, 0};
+            int64_t bsum = 0;
+            heis_update(C, false, o, h, hb, u, pr, n, s32, bsum);
+            heis_fold(s32, bsum, acc);
             // the bond share can exceed the int32 fixed-point range of the
             // lattice sums when couplings are large: 64-bit conversion
             acc[0] = (int64_t)__float2ll_rn(fm(dot3(n[0], n[1], n[2], hb[0], hb[1], hb[2]), kFixScale));

@@ -31,6 +31,8 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
     const bool active = local < C.items;
     const uint64_t t = C.sweep0 + s;
     int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
+    int32_t s32[4] = {0, 0, 0, 0};     // mx, my, mz, axis2 of the lane's sites
+    int64_t bsum = 0;                  // bond of the lane's sites
 
     // Phase 1: neighbour fields, old spins and proposals of the item's sites.
     uint32_t row = 0, i0 = 0, nvalid = 0, pend = 0;
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             float n[3] = {os[j][0], os[j][1], os[j][2]};
-            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, acc);
+            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, s32, bsum);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];
@@ -147,6 +149,7 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
                 make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
     }
 
+    heis_fold(s32, bsum, acc);
     heis_partials(acc, partials, colour, 2u, s, total_blocks, b);
 }
 

@@ -28,6 +28,16 @@ __device__ __forceinline__ float4 ld4(const float* p, bool present) {
     return *reinterpret_cast<const float4*>(p);
 }
 
+// fp32 loads through a buffer resource (an offset past the plane reads 0).
+__device__ __forceinline__ float4 bload4f(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                       __uint_as_float(v.w));
+}
+__device__ __forceinline__ float bload1f(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 __device__ __forceinline__ float f4get(const float4& v, uint32_t k) {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
 }
@@ -188,14 +198,18 @@ __device__ __forceinline__ U4 flip_words(const ChainDev& C, uint32_t i0, uint32_
                   (uint32_t)C.seed, (uint32_t)(C.seed >> 32));
 }
 
+__device__ __forceinline__ int32_t fix27_32(float v) { return __float2int_rn(fm(v, kFixScale)); }
+
 // One site: old spin o, neighbour sum h, acceptance uniform u, proposal p.
-// Writes the new spin to n and adds the site's fixed-point sums
-// {bond, mx, my, mz, (s.axis)^2} to acc; the bond term n.hb is added when
-// `bond` is set (hb: the sum over the neighbours of lower colours; for the
+// Writes the new spin to n and adds the site's fixed-point sums: mx, my, mz
+// and (s.axis)^2 to s32 (each at most 2^27 in magnitude per site, so a lane
+// may collect up to 15 sites in int32) and, when `bond` is set, n.hb to
+// bsum (hb: the sum over the neighbours of lower colours; for the
 // two-colour lattices every neighbour of a colour-1 site).
 __device__ __forceinline__ void heis_update(const ChainDev& C, bool bond, const float o[3],
                                             const float h[3], const float hb[3], float u,
-                                            const float p[3], float n[3], int64_t acc[5]) {
+                                            const float p[3], float n[3], int32_t s32[4],
+                                            int64_t& bsum) {
     const float ox = o[0], oy = o[1], oz = o[2];
     const float px = p[0], py = p[1], pz = p[2];
     const float ax = C.ax, ay = C.ay, az = C.az;
@@ -211,12 +225,19 @@ __device__ __forceinline__ void heis_update(const ChainDev& C, bool bond, const 
     n[0] = accept ? px : ox;
     n[1] = accept ? py : oy;
     n[2] = accept ? pz : oz;
-    acc[1] += fix27(n[0]);
-    acc[2] += fix27(n[1]);
-    acc[3] += fix27(n[2]);
-    const float se = dot3(n[0], n[1], n[2], ax, ay, az);
-    acc[4] += fix27(fm(se, se));
-    if (bond) acc[0] += fix27(dot3(n[0], n[1], n[2], hb[0], hb[1], hb[2]));
+    s32[0] += fix27_32(n[0]);
+    s32[1] += fix27_32(n[1]);
+    s32[2] += fix27_32(n[2]);
+    const float se = accept ? pe : oe;  // n.axis, the same operations as dot3(n, a)
+    s32[3] += fix27_32(fm(se, se));
+    if (bond) bsum += fix27(dot3(n[0], n[1], n[2], hb[0], hb[1], hb[2]));
+}
+
+// The int32 site sums of a lane into the five int64 quantities.
+__device__ __forceinline__ void heis_fold(const int32_t s32[4], int64_t bsum, int64_t acc[5]) {
+    acc[0] += bsum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[1 + k] += (int64_t)s32[k];
 }
 
 // Phase 1 of a site: its acceptance uniform u and proposal p.  Random

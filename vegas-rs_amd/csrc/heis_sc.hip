@@ -37,10 +37,27 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
     const bool active = local < C.items;
     const uint64_t t = C.sweep0 + s;
     int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
+    int32_t s32[4] = {0, 0, 0, 0};     // mx, my, mz, axis2 of the lane's sites
+    int64_t bsum = 0;                  // bond of the lane's sites
 
     // Phase 1: neighbour fields, old spins and proposals of the item's sites.
     uint32_t row = 0, i0 = 0, nvalid = 0, pend = 0;
-    float* own = arena;
+    // Component planes through buffer resources: 32-bit byte offsets (a
+    // component plane is below 4 GiB, checked on the host), and kNoRow reads
+    // 0 for an absent neighbour row.
+    __amdgpu_buffer_rsrc_t rown[3], roth[3];
+    {
+        const uint64_t plane = C.plane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            rown[c] = plane_rsrc(reinterpret_cast<const uint8_t*>(arena + C.off[colour] + c * plane),
+                                 (uint32_t)(plane * 4u));
+            roth[c] = plane_rsrc(
+                reinterpret_cast<const uint8_t*>(arena + C.off[colour ^ 1u] + c * plane),
+                (uint32_t)(plane * 4u));
+        }
+    }
+    uint32_t own_off = 0;
     float os[4][3], hs[4][3], ps[4][3], us[4];
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
@@ -49,27 +66,24 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
         for (int c = 0; c < 3; ++c) os[j][c] = hs[j][c] = ps[j][c] = 0.0f;
     }
     if (active) {
-        row = local / C.segs;
+        row = fast_div(local, C.segs_m, C.segs_s);
         const uint32_t seg = local - row * C.segs;
         const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
-        const uint64_t plane = C.plane;
-        const uint32_t y = row % ly, z = row / ly;
+        const uint32_t z = fast_div(row, C.ly_m, C.ly_s), y = row - z * ly;
         const uint32_t q = (colour + y + z) & 1u;
         i0 = seg * kHeisSeg;
         nvalid = min(kHeisSeg, w - i0);
-        own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
-        const float* oth = arena + C.off[colour ^ 1u];
-        const float* orow = oth + (uint64_t)row * pitch;
+        own_off = (row * pitch + i0) * 4u;
+        const uint32_t orow = row * pitch * 4u;
 
         const bool ym_ok = (y > 0) || (pbc & 2u);
         const bool yp_ok = (y + 1 < ly) || (pbc & 2u);
         const bool zm_ok = (z > 0) || (pbc & 4u);
         const bool zp_ok = (z + 1 < lz) || (pbc & 4u);
-        const int64_t r_ym = y > 0 ? (int64_t)row - 1 : (int64_t)row + (ly - 1);
-        const int64_t r_yp = y + 1 < ly ? (int64_t)row + 1 : (int64_t)row - (ly - 1);
-        const int64_t r_zm = z > 0 ? (int64_t)row - ly : (int64_t)row + (int64_t)(lz - 1) * ly;
-        const int64_t r_zp =
-            z + 1 < lz ? (int64_t)row + ly : (int64_t)row - (int64_t)(lz - 1) * ly;
+        const uint32_t r_ym = y > 0 ? row - 1 : row + (ly - 1);
+        const uint32_t r_yp = y + 1 < ly ? row + 1 : row - (ly - 1);
+        const uint32_t r_zm = z > 0 ? row - ly : row + (lz - 1) * ly;
+        const uint32_t r_zp = z + 1 < lz ? row + ly : row - (lz - 1) * ly;
 
         float4 so[3], mid[3], ym[3], yp[3], zm[3], zp[3];
         float xe[3];  // the extra x neighbour (position i0-1 or i0+4)
@@ -82,21 +96,19 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
             xe_pos = i0 + kHeisSeg < w ? i0 + kHeisSeg : 0;
             xe_ok = i0 + kHeisSeg < w || (pbc & 1u);
         }
+        // Position w-1 with q = 1 wraps to position 0 (or has no +x bond).
+        const bool wrap_ok = q == 1 && i0 + kHeisSeg >= w && (pbc & 1u);
+        float wrap0[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            so[c] = ld4(own + c * plane, true);
-            mid[c] = ld4(orow + c * plane + i0, true);
-            ym[c] = ld4(oth + c * plane + r_ym * (int64_t)pitch + i0, ym_ok);
-            yp[c] = ld4(oth + c * plane + r_yp * (int64_t)pitch + i0, yp_ok);
-            zm[c] = ld4(oth + c * plane + r_zm * (int64_t)pitch + i0, zm_ok);
-            zp[c] = ld4(oth + c * plane + r_zp * (int64_t)pitch + i0, zp_ok);
-            xe[c] = xe_ok ? orow[c * plane + xe_pos] : 0.0f;
-        }
-        // Position w-1 with q = 1 wraps to position 0 (or has no +x bond).
-        float wrap0[3] = {0.0f, 0.0f, 0.0f};
-        if (q == 1 && i0 + kHeisSeg >= w && (pbc & 1u)) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) wrap0[c] = orow[c * plane];
+            so[c] = bload4f(rown[c], own_off);
+            mid[c] = bload4f(roth[c], orow + i0 * 4u);
+            ym[c] = bload4f(roth[c], ym_ok ? (r_ym * pitch + i0) * 4u : kNoRow);
+            yp[c] = bload4f(roth[c], yp_ok ? (r_yp * pitch + i0) * 4u : kNoRow);
+            zm[c] = bload4f(roth[c], zm_ok ? (r_zm * pitch + i0) * 4u : kNoRow);
+            zp[c] = bload4f(roth[c], zp_ok ? (r_zp * pitch + i0) * 4u : kNoRow);
+            xe[c] = bload1f(roth[c], xe_ok ? orow + xe_pos * 4u : kNoRow);
+            wrap0[c] = bload1f(roth[c], wrap_ok ? orow : kNoRow);
         }
 
         U4 fw = U4{0u, 0u, 0u, 0u};
@@ -136,22 +148,27 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
 
     // Phase 3: accept or reject, store, sums.
     if (active) {
-        const uint64_t plane = C.plane;
         float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             float n[3] = {os[j][0], os[j][1], os[j][2]};
-            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, acc);
+            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, s32, bsum);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            *reinterpret_cast<float4*>(own + c * plane) =
-                make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
+        for (int c = 0; c < 3; ++c) {
+            u32x4 v;
+            v.x = __float_as_uint(ns[c][0]);
+            v.y = __float_as_uint(ns[c][1]);
+            v.z = __float_as_uint(ns[c][2]);
+            v.w = __float_as_uint(ns[c][3]);
+            __builtin_amdgcn_raw_buffer_store_b128(v, rown[c], own_off, 0, 0);
+        }
     }
 
+    heis_fold(s32, bsum, acc);
     heis_partials(acc, partials, colour, 2u, s, total_blocks, b);
 }
 

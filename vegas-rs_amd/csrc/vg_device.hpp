@@ -278,11 +278,25 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return s;
 }
 
-// Sum over the 64 lanes of a wave.
+// Reductions over the 64 lanes of a wave (every lane active): DPP moves
+// within each row of 16 lanes -- swap within pairs and within quads
+// (quad_perm 1,0,3,2 and 2,3,0,1), then the half-row and row mirrors -- give
+// every lane its row's total; the four rows' totals are read from lanes 0,
+// 16, 32 and 48.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
 __device__ __forceinline__ int32_t wave_sum(int32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += (int32_t)dpp<kDppXor1>((uint32_t)v);
+    v += (int32_t)dpp<kDppXor2>((uint32_t)v);
+    v += (int32_t)dpp<kDppHalfMirror>((uint32_t)v);
+    v += (int32_t)dpp<kDppMirror>((uint32_t)v);
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) +
+           __builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48);
 }
 
 // Issue priority of the calling wave, 0 (lowest) .. 3; p must fold to a
@@ -296,11 +310,14 @@ __device__ __forceinline__ void set_prio(uint32_t p) {
     }
 }
 
-// Maximum over the 64 lanes of a wave (every lane must call it).
+// Maximum over the 64 lanes of a wave (every lane active).
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-    return v;
+    v = max(v, dpp<kDppXor1>(v));
+    v = max(v, dpp<kDppXor2>(v));
+    v = max(v, dpp<kDppHalfMirror>(v));
+    v = max(v, dpp<kDppMirror>(v));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
 // Ising colour planes are read and written through raw buffer resources:
