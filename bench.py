@@ -34,6 +34,8 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
+COUNTERS = ROOT / "profiles" / "pass_counters.txt"
+SIMDS, CLOCK_GHZ = 1024, 2.4  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
 METRIC = "spin-flip attempts/s (Ising sc 256³); batched vs plain; p50/p99 job latency"
 
 
@@ -47,6 +49,57 @@ def load_engine():
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def pass_counters(workload, kernels):
+    """Counter-measured bytes per dispatch (FETCH_SIZE + WRITE_SIZE, scaled by
+    a device copy of known size) and VALU instructions per dispatch of the
+    named pass kernels -- the mean over them -- from the table the rocprofv3
+    runs of this workload wrote (profiles/pass_counters.txt).  None when the
+    table has no row for one of them (another tile, say)."""
+    if not COUNTERS.exists():
+        return None
+    rows = {}
+    for line in COUNTERS.read_text().splitlines():
+        if line.startswith("#") or not line.strip():
+            continue
+        w, k, _, fetch, write, valu, _ = line.split("\t")
+        rows[(w, k)] = (float(fetch) + float(write), float(valu))
+    got = [rows.get((workload, k)) for k in kernels]
+    if any(g is None for g in got):
+        return None
+    return {"traffic": sum(g[0] for g in got) / len(got),
+            "valu": sum(g[1] for g in got) / len(got),
+            "source": f"profiles/pass_counters.txt, {workload}: {', '.join(kernels)}"}
+
+
+def with_counters(roofline, workload, kernels, mads_per_launch=None):
+    """Adds the counter-measured traffic (bytes per launch, null when not
+    profiled at this tile) and, given the v_mad_u64_u32 count of the launch,
+    its VALU ceiling: the time the measured VALU instructions take to issue
+    on every SIMD at the peak clock (4 cycles per wave64 instruction, 16 for
+    v_mad_u64_u32), as a fraction of the measured launch time."""
+    c = pass_counters(workload, kernels)
+    roofline["traffic"] = c["traffic"] if c else None
+    roofline["traffic_source"] = c["source"] if c else None
+    if c and mads_per_launch is not None:
+        busy_us = (4.0 * c["valu"] + 12.0 * mads_per_launch) / (SIMDS * CLOCK_GHZ * 1e3)
+        roofline["valu_ceiling"] = {
+            "valu_instructions_per_launch": c["valu"],
+            "mad_u64_per_launch": mads_per_launch,
+            "busy_us": busy_us,
+            "frac": busy_us / roofline["mean_launch_us"],
+            "peak": f"{SIMDS} SIMDs x {CLOCK_GHZ} GHz, one wave64 VALU instruction per 4 cycles "
+                    f"(v_mad_u64_u32: 16)",
+        }
+    return roofline
+
+
+def ising_mads(attempts_per_launch):
+    """v_mad_u64_u32 (wave instructions) of an Ising pass launch with flip
+    proposals: one Philox4x32-10 call (20 of them) per 16 sites, 64 lanes per
+    wave instruction (tie draws, a few per 10^3 sites, left out)."""
+    return 20.0 * attempts_per_launch / 1024.0
 
 
 def cpu_model():
@@ -237,6 +290,9 @@ def run_batched(vg, ctx, sweeps, baseline_sweeps, n_chains=512, cpu=None):
         "unit": "spin-flip attempts/s",
         "per_chain_launches": per_chain,
         "per_chain_sweeps": baseline_sweeps,
+        "per_chain_sampling": (f"each chain alone for {baseline_sweeps} of the batch's {sweeps} "
+                               f"sweeps (timed after 2 untimed ones): a chain's per-sweep cost "
+                               f"does not depend on how many sweeps it runs"),
         "speedup": batched / per_chain,
         "cpu_all_cores": cpu,
     }
@@ -253,7 +309,7 @@ def run_bitpacked(vg, ctx, side, sweeps):
         b.randomize()
         for i, T in enumerate(temps):
             b.set_thermostat(i, T)
-        tile = b.autotune(20)
+        tile = b.autotune(50)
         b.set_tile(tile)
         b.set_graph(True)
         b.run(10)
@@ -273,10 +329,11 @@ def run_bitpacked(vg, ctx, side, sweeps):
         "unit": "spin-flip attempts/s",
         "sweeps": sweeps,
         "tile": tile,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "kernel": "ising_sc_pass<bits>",
-                     "algorithmic_bytes_per_launch": per_launch // 4,
-                     "mean_launch_us": avg_ms * 1e3},
+        "roofline": with_counters(
+            {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "kernel": "ising_sc_pass<bits>",
+             "algorithmic_bytes_per_launch": per_launch // 4, "mean_launch_us": avg_ms * 1e3},
+            "bitpacked", [f"ising_sc_pass<{tile},1>"], ising_mads(per_launch)),
     }
 
 
@@ -310,11 +367,20 @@ def run_resident(vg, ctx, side, sweeps):
         "unit": "spin-flip attempts/s",
         "sweeps": sweeps,
         "tile": tile,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "kernel": "ising_sc_pass",
-                     "algorithmic_bytes_per_launch": 2 * per_launch,
-                     "mean_launch_us": avg_ms * 1e3},
+        "roofline": with_counters(
+            {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "kernel": "ising_sc_pass",
+             "algorithmic_bytes_per_launch": 2 * per_launch, "mean_launch_us": avg_ms * 1e3},
+            "resident", [f"ising_sc_pass<{tile},2>"], ising_mads(per_launch)),
     }
+
+
+def heis_mads(attempts_per_launch):
+    """v_mad_u64_u32 (wave instructions) of a Heisenberg pass launch with
+    uniform-sphere proposals: one Philox call (20) per site for the first two
+    Marsaglia tries, 64 lanes per wave instruction (the 4.6% of sites whose
+    tries both miss draw more, left out)."""
+    return 20.0 * attempts_per_launch / 64.0
 
 
 def run_heisenberg(vg, ctx, side, sweeps):
@@ -353,15 +419,27 @@ def run_heisenberg(vg, ctx, side, sweeps):
         "unit": "spin-flip attempts/s",
         "sweeps": sweeps,
         "ms_per_step": wall * 1e3 / sweeps,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "kernel": "heis_sc_pass",
-                     "algorithmic_bytes_per_launch": 24 * per_launch,
-                     "mean_launch_us": avg_ms * 1e3,
-                     "mean_launch_us_per_launch_events": ev_ms / ev_n * 1e3},
+        "roofline": with_counters(
+            {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "kernel": "heis_sc_pass",
+             "algorithmic_bytes_per_launch": 24 * per_launch, "mean_launch_us": avg_ms * 1e3,
+             "mean_launch_us_per_launch_events": ev_ms / ev_n * 1e3},
+            "heisenberg", ["heis_sc_pass"], heis_mads(per_launch)),
         "observables_last_sweep": {f"T={T}": {"E/N": float(e[i, -1]) / n,
                                               "|M|/N": float(m[i, -1]) / n}
                                    for i, T in enumerate(temps)},
     }
+
+
+def pass_kernels(lattice, spin, tile):
+    """The colour-pass kernels of a bcc or fcc line as the counter table names
+    them (fcc: one instantiation per colour)."""
+    bits = 1 if spin == "ising_bits" else 0
+    if lattice == "bcc":
+        return ["heis_bcc_pass"] if spin == "heisenberg" else [f"ising_bcc_pass<{tile},{bits}>"]
+    if spin == "heisenberg":
+        return [f"heis_fcc_pass<{a}>" for a in range(4)]
+    return [f"ising_fcc_pass<{a},{tile},{bits}>" for a in range(4)]
 
 
 def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
@@ -394,7 +472,7 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
             b.randomize()
             for i, T in enumerate(temps):
                 b.set_thermostat(i, T, 0.0 if ising else 0.05)
-            b.autotune(30)  # the Ising passes' z tile (a no-op for Heisenberg)
+            tile = b.autotune(30)  # the Ising passes' z tile (a no-op for Heisenberg)
             b.set_graph(True)
             b.run(5)
             b.prepare(sweeps)
@@ -416,12 +494,15 @@ def run_fcc(vg, ctx, cells, heis_cells, sweeps, kind=None):
             "unit": "spin-flip attempts/s",
             "sweeps": sweeps,
             "ms_per_step": wall * 1e3 / sweeps,
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "kernel": (f"{'ising' if ising else 'heis'}_{name}_pass"
-                                    + ("<bits>" if spin == "ising_bits" else "")),
-                         "algorithmic_bytes_per_launch": bpa * per_launch,
-                         "mean_launch_us": avg_ms * 1e3},
+            "roofline": with_counters(
+                {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                 "kernel": (f"{'ising' if ising else 'heis'}_{name}_pass"
+                            + ("<bits>" if spin == "ising_bits" else "")),
+                 "algorithmic_bytes_per_launch": bpa * per_launch,
+                 "mean_launch_us": avg_ms * 1e3},
+                name, pass_kernels(name, spin, tile),
+                ising_mads(per_launch) if ising else heis_mads(per_launch)),
             "observables_last_sweep": {f"T={T}": {"E/N": float(e[i, -1]) / n,
                                                   "|M|/N": float(m[i, -1]) / n}
                                        for i, T in enumerate(temps)},
@@ -606,10 +687,14 @@ def run_served(vg, ctx, n_jobs, rates, gen_seed=2025, budget=0, graph=False):
     seq_s = one_at_a_time_seconds(vg, ctx, jobs)
     attempts = sum(j[3] for j in jobs)
     for res in out:
-        res["one_at_a_time_throughput"] = attempts / seq_s
-        res["speedup_vs_one_at_a_time"] = res["throughput"] / (attempts / seq_s)
-    log(f"served: one job at a time with plain launches {attempts / seq_s:.3e} attempts/s "
-        f"({seq_s:.1f} s)")
+        res["one_at_a_time_throughput_estimated"] = attempts / seq_s
+        res["speedup_vs_one_at_a_time_estimated"] = res["throughput"] / (attempts / seq_s)
+        res["one_at_a_time_method"] = (
+            "estimated, not run end to end: the per-sweep wall time of a one-chain batch with "
+            "plain launches, measured over 60 sweeps for every (spin kind, L) in the job list, "
+            "times each job's sweeps, summed over the jobs")
+    log(f"served: one job at a time with plain launches, estimated {attempts / seq_s:.3e} "
+        f"attempts/s ({seq_s:.1f} s)")
     return out
 
 
@@ -631,8 +716,7 @@ def main():
                              "fcc", "bcc", "wolff", "resident"],
                     help="headline: sc 256^3 x 3 only; batched: the 512-chain batch only; "
                          "served: the Poisson job stream only")
-    ap.add_argument("--serve-jobs", type=int, default=400,
-                    help="jobs per rate (the full served workload is 2000)")
+    ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs per rate")
     ap.add_argument("--heis-side", type=int, default=128)
     ap.add_argument("--heis-sweeps", type=int, default=500)
     ap.add_argument("--fcc-cells", type=int, default=128, help="fcc Ising chains: cells per side")
@@ -646,13 +730,14 @@ def main():
     ap.add_argument("--resident-side", type=int, default=1024,
                     help="side of the HBM-resident Ising chain")
     ap.add_argument("--resident-sweeps", type=int, default=100)
-    ap.add_argument("--serve-rates", default="50,500")
+    ap.add_argument("--serve-rates", default="50,100,200,500")
     ap.add_argument("--serve-graph", action="store_true",
                     help="the server replays quanta from hipGraphs")
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="server per-sweep work budget (Ising-site units; 0 = none)")
     ap.add_argument("--batched-sweeps", type=int, default=10000)
-    ap.add_argument("--baseline-sweeps", type=int, default=100)
+    ap.add_argument("--baseline-sweeps", type=int, default=2000,
+                    help="sweeps of each chain in the batched set's per-chain leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -728,7 +813,7 @@ def main():
     batch.randomize()
     for i, T in enumerate(temps):
         batch.set_thermostat(i, T)
-    tile = batch.autotune(20) if args.tile == 0 else args.tile
+    tile = batch.autotune(50) if args.tile == 0 else args.tile
     batch.set_tile(tile)
     batch.set_graph(args.graph)
     log(f"bench: {len(temps)} chains sc {side}^3 int8, warmup {args.warmup}")
@@ -761,14 +846,6 @@ def main():
     attempts_per_launch = attempts_per_step // 2
     alg_bytes = 2 * attempts_per_launch  # read + write of each site once per sweep
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter passes
-    # (scaled by a device copy of known size), when profiled at this size.
-    pmc = ROOT / "profiles" / f"ising_sc{side}_traffic.txt"
-    if pmc.exists():
-        for line in pmc.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch:"):
-                traffic = float(line.split(":", 1)[1])
     log(f"bench: pass kernel {avg_ms * 1e3:.1f} us/launch -> {achieved:.0f} GB/s algorithmic")
 
     cpu = None
@@ -801,19 +878,19 @@ def main():
             "graph": bool(args.graph),
             "tile": tile,
         },
-        "roofline": {
+        "roofline": with_counters({
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
             "kernel": "ising_sc_pass",
             "algorithmic_bytes_per_launch": alg_bytes,
             "mean_launch_us": avg_ms * 1e3,
             "launches_timed": launches,
             "mean_launch_us_per_launch_events": ev_ms / ev_launches * 1e3,
-        },
+        }, "headline" if side == 256 else f"sc{side}", [f"ising_sc_pass<{tile},2>"],
+            ising_mads(attempts_per_launch)),
         "cpu_baseline": cpu,
         "observables_last_sweep": {
             f"T={T}": {"E/N": float(e[i, -1]) / n_sites, "|M|/N": float(m[i, -1]) / n_sites}

@@ -252,7 +252,7 @@ int vg_batch_prepare(vg_batch* batch, uint64_t sweeps);
 int vg_batch_set_tile(vg_batch* batch, uint32_t tile);
 int vg_batch_get_tile(const vg_batch* batch, uint32_t* tile);
 /* Times `trial_sweeps` sweeps with every tile (replays of graphs captured
- * beforehand, three rounds over the tiles, best of each) and keeps the
+ * beforehand, five rounds over the tiles, the median of each) and keeps the
  * fastest, for each of the six groups.  The chains' spins and sweep counters are
  * restored after every trial, so tuning does not change any trajectory. */
 int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen_tile);

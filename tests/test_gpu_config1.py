@@ -8,12 +8,17 @@ An ensemble of 512 independent chains runs the whole program on the engine
 (one batch, chain seeds 42 + k) and another of 512 runs it on the oracle's
 restatement of the reference's sequential MetropolisFlipIntegrator (seeds
 1000 + k, one thread per chain).  For every one of the 39 measured stages
-the ensemble means of the per-chain stage values <E>/N, <|M|>/N,
+the engine ensemble's means of the per-chain stage values <E>/N, <|M|>/N,
 C = Var(E)/(N T^2) and chi = Var(|M|)/(N T) (StatSensor,
-instrument.rs:113-123) agree within 3 sigma, sigma from the spread of each
-ensemble's chains, and the engine's agree within 3 sigma with the exact
-values of the 10x10 torus (row transfer matrix, tests/golden/
-make_ising10x10_exact.py).  Averaging over chains covers the temperatures
+instrument.rs:113-123) agree within 3 sigma with the exact values of the
+10x10 torus (row transfer matrix, tests/golden/make_ising10x10_exact.py),
+sigma from the spread of the ensemble's chains.  The engine ensemble and
+the sequential one agree as well, within the bound that 156 comparisons of
+two noisy ensembles call for: the largest of 156 standard normals exceeds
+4.0 in 1% of runs (and 3.0 in a third), and the sequential ensemble itself
+strays from the exact values by up to 3.3 of its sigmas at these seeds
+(T = 2.3, C); the long single-chain runs of tests/test_oracle.py pin the
+sequential restatement to the exact values.  Averaging over chains covers the temperatures
 just below T_c and the low ones, where a single 2D chain may spend long
 stretches in a two-domain state.  There the per-chain stage means are heavy
 tailed: at T = 2.1 the means of 64-chain blocks of the sequential
@@ -36,6 +41,9 @@ pytestmark = pytest.mark.gpu
 
 GPU_CHAINS = 512
 CPU_CHAINS = 512
+# the largest |z| of 156 standard normals stays below this in 99% of runs
+# (2 * (1 - Phi(4.0)) = 6.3e-5 per comparison, 156 * 6.3e-5 = 0.01)
+FAMILY_Z = 4.0
 EXACT = Path(__file__).resolve().parent / "golden" / "ising10x10_exact.json"
 N = 100
 
@@ -127,7 +135,7 @@ def test_config1_cli_program_ensemble(vg, orc, ctx):
             zx.append(zex)
             print(f"T={T:.2f} {lab:6s} engine {ma:.6f} +- {ea:.6f}  sequential {mb:.6f} +- "
                   f"{eb:.6f}  exact {ex:.6f}  z={z:.2f}  z_exact={zex:.2f}", flush=True)
-            if not (z < 3.0 and zex < 3.0):
+            if not (zex < 3.0 and z < FAMILY_Z):
                 fails.append((T, lab, ma, ea, mb, eb, ex, z, zex))
     print(f"config1: {len(zs)} comparisons with the sequential chains (max z {max(zs):.2f}, "
           f"mean {np.mean(zs):.2f}) and {len(zx)} with the exact values (max z {max(zx):.2f}, "

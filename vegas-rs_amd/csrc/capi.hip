@@ -1503,11 +1503,13 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
         if (G.members.empty() || rc != VG_OK) continue;
         const uint32_t start_tile = G.tile;
         uint32_t best = start_tile;
-        double best_ms = 1e300;
         const uint32_t tiles[4] = {1, 2, 4, 8};
-        // Three rounds over the tiles, each tile's best counted: a drift of
-        // the clocks during tuning then touches every tile alike.
-        for (int rep = 0; rep < 3 && rc == VG_OK; ++rep)
+        // Five rounds over the tiles, each tile's median kept: a drift of
+        // the clocks during tuning touches every tile alike, and one fast
+        // outlier does not pick a tile.
+        constexpr int kRounds = 5;
+        double ms[4][kRounds] = {};
+        for (int rep = 0; rep < kRounds && rc == VG_OK; ++rep)
             for (uint32_t ti = 0; ti < 4 && rc == VG_OK; ++ti) {
                 rc = layout_work(batch, g, tiles[ti]);
                 if (rc == VG_OK) rc = run_impl(batch, 2, nullptr, nullptr, nullptr);
@@ -1516,12 +1518,19 @@ int vg_batch_autotune(vg_batch* batch, uint32_t trial_sweeps, uint32_t* chosen) 
                 // launches of passes this short measure the launch rate).
                 if (rc == VG_OK) rc = vg_batch_prepare(batch, trial_sweeps);
                 if (rc == VG_OK) rc = run_impl(batch, trial_sweeps, nullptr, nullptr, nullptr);
-                if (rc == VG_OK && batch->last_ms < best_ms) {
-                    best_ms = batch->last_ms;
-                    best = tiles[ti];
-                }
+                if (rc == VG_OK) ms[ti][rep] = batch->last_ms;
                 if (rc == VG_OK) rc = restore();
             }
+        if (rc == VG_OK) {
+            double best_ms = 1e300;
+            for (uint32_t ti = 0; ti < 4; ++ti) {
+                std::sort(ms[ti], ms[ti] + kRounds);
+                if (ms[ti][kRounds / 2] < best_ms) {
+                    best_ms = ms[ti][kRounds / 2];
+                    best = tiles[ti];
+                }
+            }
+        }
         if (rc != VG_OK) {
             // A trial failed part way (graph capture, memory): put the spins
             // and sweep counters back as they were, best effort, keeping the
