@@ -803,10 +803,95 @@ static void input_documents() {
     EXPECT(p.unitcell_is_path && throws_kind([&] { p.lattice(); }, ErrorKind::NotImplemented));
 }
 
+// A Snappy block decoder written from the format description (varint
+// length; literals; copies with 1- and 2-byte offsets -- the compressor
+// emits no 4-byte-offset copies), for the round trips below.
+static bool snappy_decode(const std::string& in, std::string& out) {
+    size_t i = 0;
+    uint64_t n = 0;
+    for (int sh = 0;; sh += 7) {
+        if (i >= in.size() || sh > 35) return false;
+        const uint8_t b = (uint8_t)in[i++];
+        n |= (uint64_t)(b & 0x7F) << sh;
+        if (!(b & 0x80)) break;
+    }
+    out.clear();
+    while (i < in.size()) {
+        const uint8_t t = (uint8_t)in[i++];
+        if ((t & 3) == 0) {
+            uint64_t len = t >> 2;
+            if (len >= 60) {
+                const int nb = (int)len - 59;
+                if (i + nb > in.size()) return false;
+                len = 0;
+                for (int b = 0; b < nb; ++b) len |= (uint64_t)(uint8_t)in[i++] << (8 * b);
+            }
+            len += 1;
+            if (i + len > in.size()) return false;
+            out.append(in, i, len);
+            i += len;
+        } else {
+            uint32_t len, off;
+            if ((t & 3) == 1) {
+                if (i + 1 > in.size()) return false;
+                len = ((t >> 2) & 7u) + 4u;
+                off = ((uint32_t)(t >> 5) << 8) | (uint8_t)in[i++];
+            } else if ((t & 3) == 2) {
+                if (i + 2 > in.size()) return false;
+                len = (t >> 2) + 1u;
+                off = (uint8_t)in[i] | ((uint32_t)(uint8_t)in[i + 1] << 8);
+                i += 2;
+            } else {
+                return false;
+            }
+            if (off == 0 || off > out.size()) return false;
+            for (uint32_t k = 0; k < len; ++k) out.push_back(out[out.size() - off]);
+        }
+    }
+    return out.size() == n;
+}
+
+// The compressor (vegas/snappy.hpp) on 3000 generated inputs -- random bytes
+// over small and full alphabets, runs of every length up to 7, quadratic
+// residues -- of up to 200,000 bytes: every block decodes to its input.
+static void snappy_round_trips() {
+    std::mt19937_64 r(3);
+    int bad = 0;
+    for (int it = 0; it < 3000; ++it) {
+        const size_t n = r() % (it < 2000 ? 300 : 200000);
+        const int alpha = 1 + (int)(r() % 256);
+        const int mode = (int)(r() % 3);
+        std::string s(n, '\0');
+        for (size_t k = 0; k < n; ++k)
+            s[k] = mode == 0   ? (char)(r() % alpha)
+                   : mode == 1 ? (char)((k / (1 + it % 7)) % alpha)
+                               : (char)((k * k) % alpha);
+        const std::string c = vegas::snappy_compress(s.data(), s.size());
+        std::string d;
+        if (!snappy_decode(c, d) || d != s) ++bad;
+    }
+    EXPECT(bad == 0);
+    const std::string z(100000, 'z');
+    EXPECT(vegas::snappy_compress(z.data(), z.size()).size() < 100000 / 20);
+}
+
+// The Snappy block of a file's bytes, for tests/test_host_cpp.py to
+// decompress with pyarrow's Snappy codec.
+static int snappy_file(const char* in, const char* out) {
+    std::ifstream f(in, std::ios::binary);
+    if (!f) return 2;
+    const std::string data((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    const std::string packed = vegas::snappy_compress(data.data(), data.size());
+    std::ofstream o(out, std::ios::binary);
+    o.write(packed.data(), (std::streamsize)packed.size());
+    return o ? 0 : 2;
+}
+
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
     std::vector<std::pair<const char*, std::function<void()>>> tests;
     if (argc > 2 && std::strcmp(argv[1], "--toml-json") == 0) return toml_json(argv[2]);
+    if (argc > 3 && std::strcmp(argv[1], "--snappy") == 0) return snappy_file(argv[2], argv[3]);
     if (argc > 2 && std::strcmp(argv[1], "--parquet") == 0) {
         const std::string dir = argv[2];
         tests = {{"parquet_tables", [dir] { parquet_tables(dir); }}};
@@ -827,7 +912,8 @@ int main(int argc, char** argv) {
                  {"program_schedules", program_schedules},
                  {"program_validation", program_validation},
                  {"stat_line_format", stat_line_format},
-                 {"input_documents", input_documents}};
+                 {"input_documents", input_documents},
+                 {"snappy_round_trips", snappy_round_trips}};
     } else {
         tests = {{"machine_hooks_and_parity", machine_hooks_and_parity},
                  {"stat_sensor_lines", stat_sensor_lines},

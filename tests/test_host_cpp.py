@@ -102,6 +102,79 @@ def test_parquet_tables(tmp_path):
     empty = pq.read_table(tmp_path / "empty.parquet")
     assert len(empty) == 0 and empty.schema.names == obs.schema.names
 
+    # Snappy pages, as the reference writes them (output.rs:48,138), with
+    # the sizes the metadata reports
+    for name in ("obs", "state_ising", "big"):
+        md = pq.ParquetFile(tmp_path / f"{name}.parquet").metadata
+        for g in range(md.num_row_groups):
+            for c in range(md.num_columns):
+                col = md.row_group(g).column(c)
+                assert col.compression == "SNAPPY"
+                assert col.total_compressed_size > 0
+    kcol = pq.ParquetFile(tmp_path / "big.parquet").metadata.row_group(0).column(1)
+    # k = i * odd constant: no repeats, so the page stays near its 8 B/value
+    assert kcol.total_uncompressed_size >= 8 * n
+    fcol = pq.ParquetFile(tmp_path / "big.parquet").metadata.row_group(0).column(0)
+    # the flag column is a period-3 bit pattern: it compresses far down
+    assert fcol.total_compressed_size * 10 < fcol.total_uncompressed_size
+
+
+def _snappy_cases():
+    """Inputs for the Snappy compressor: empty, tiny, literal-length
+    boundaries (60, 61, 256, 65536 + 1), incompressible bytes, runs, short and
+    long periods, matches across the 64 KiB fragment boundary, and text."""
+    import numpy as np
+    rng = np.random.default_rng(11)
+    cases = {
+        "empty": b"",
+        "one": b"x",
+        "three": b"abc",
+        "four_same": b"aaaa",
+        "lit60": bytes(rng.integers(0, 256, 60, dtype=np.uint8)),
+        "lit61": bytes(rng.integers(0, 256, 61, dtype=np.uint8)),
+        "lit256": bytes(rng.integers(0, 256, 256, dtype=np.uint8)),
+        "random_70k": bytes(rng.integers(0, 256, 70000, dtype=np.uint8)),
+        "zeros_300k": bytes(300000),
+        "period3": b"abc" * 50000,
+        "period67": bytes(rng.integers(0, 256, 67, dtype=np.uint8)) * 3000,
+        "period5000": bytes(rng.integers(0, 256, 5000, dtype=np.uint8)) * 40,
+        "across_fragments": bytes(rng.integers(0, 256, 30000, dtype=np.uint8)) * 7,
+        "period65530": bytes(rng.integers(0, 256, 65530, dtype=np.uint8)) * 3,
+        "text": (b"The quick brown fox jumps over the lazy dog. " * 997)[:40000],
+        "doubles": (np.arange(200000, dtype=np.float64) * 0.5 - 3.0).tobytes(),
+        "low_entropy": bytes(rng.integers(0, 3, 200000, dtype=np.uint8)),
+    }
+    return cases
+
+
+def test_snappy_blocks(tmp_path):
+    """The host library's Snappy compressor (vegas/snappy.hpp): every block
+    decompresses, with pyarrow's Snappy codec, to its input, and repetitive
+    inputs shrink."""
+    import pyarrow as pa
+
+    if not BIN.exists():
+        pytest.fail(f"{BIN} is not built (make -C tests/cpp)")
+    sizes = {}
+    for name, data in _snappy_cases().items():
+        src, dst = tmp_path / f"{name}.in", tmp_path / f"{name}.snappy"
+        src.write_bytes(data)
+        r = subprocess.run([str(BIN), "--snappy", str(src), str(dst)], capture_output=True,
+                           timeout=60)
+        assert r.returncode == 0, (name, r.stderr)
+        packed = dst.read_bytes()
+        back = pa.decompress(packed, decompressed_size=len(data), codec="snappy",
+                             asbytes=True)
+        assert back == data, name
+        sizes[name] = len(packed)
+    assert sizes["empty"] == 1
+    assert sizes["zeros_300k"] < 300000 // 20
+    assert sizes["period3"] < 150000 // 20
+    # a period of 30000 repeats inside every 64 KiB fragment (matches do not
+    # reach into the previous fragment, as in the reference implementation)
+    assert sizes["across_fragments"] < 30000 * 7 * 3 // 5
+    assert sizes["random_70k"] <= 70000 + 70000 // 6 + 16
+
 
 @pytest.mark.gpu
 def test_machine_parquet_tables(tmp_path):

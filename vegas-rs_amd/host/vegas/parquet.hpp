@@ -4,16 +4,18 @@
 // The files hold what the reference's ArrowWriter holds for the same rows --
 // the same column names, order and types (Boolean, UInt64 as INT64 with the
 // unsigned 64-bit integer annotation, Float64 as DOUBLE), all columns
-// required -- so a reader sees the same table.  They differ in encoding only:
-// values are PLAIN and pages uncompressed, where the reference compresses
-// with Snappy.  Like the reference (output.rs:32-33, 101-113), rows go to
-// `<path with extension "parquet.tmp">`, renamed to `path` on close.
+// required -- and, like the reference's (output.rs:48,138), its pages are
+// Snappy-compressed, so a reader sees the same table.  Values are PLAIN
+// encoded (the reference's writer may also use dictionary pages; a reader
+// decodes both the same).  Like the reference (output.rs:32-33, 101-113),
+// rows go to `<path with extension "parquet.tmp">`, renamed to `path` on
+// close.
 //
 // Format: "PAR1", then per row group and column one or more data pages
-// (Thrift-compact PageHeader + PLAIN values; required top-level columns carry
-// no repetition or definition levels), then the Thrift-compact FileMetaData,
-// its length (4 bytes little-endian) and "PAR1" (Apache Parquet format
-// specification, parquet.thrift).
+// (Thrift-compact PageHeader + the Snappy block of the PLAIN values; required
+// top-level columns carry no repetition or definition levels), then the
+// Thrift-compact FileMetaData, its length (4 bytes little-endian) and "PAR1"
+// (Apache Parquet format specification, parquet.thrift).
 #pragma once
 
 #include <cstdint>
@@ -23,6 +25,7 @@
 #include <vector>
 
 #include "error.hpp"
+#include "snappy.hpp"
 
 namespace vegas {
 
@@ -117,7 +120,7 @@ enum : int32_t { kBoolean = 0, kInt64 = 2, kDouble = 5 };  // Type
 enum : int32_t { kRequired = 0 };                          // FieldRepetitionType
 enum : int32_t { kUint64 = 14 };                           // ConvertedType
 enum : int32_t { kPlain = 0, kRle = 3 };                   // Encoding
-enum : int32_t { kUncompressed = 0 };                      // CompressionCodec
+enum : int32_t { kSnappy = 1 };                            // CompressionCodec
 enum : int32_t { kDataPage = 0 };                          // PageType
 
 // Rust's Path::with_extension(ext): the file name's extension (after its
@@ -176,10 +179,11 @@ class ParquetWriter {
             for (size_t p0 = 0; p0 < n; p0 += kPageValues) {
                 const size_t m = std::min(kPageValues, n - p0);
                 const std::string values = plain(schema_[c].kind, cols[c], p0, m);
+                const std::string packed = snappy_compress(values.data(), values.size());
                 parquet_detail::Compact h;
                 h.i32(1, parquet_detail::kDataPage);
                 h.i32(2, (int32_t)values.size());
-                h.i32(3, (int32_t)values.size());
+                h.i32(3, (int32_t)packed.size());
                 h.begin_struct(5);
                 h.i32(1, (int32_t)m);
                 h.i32(2, parquet_detail::kPlain);
@@ -188,8 +192,9 @@ class ParquetWriter {
                 h.end_struct();
                 h.stop();
                 put(h.data().data(), h.data().size());
-                put(values.data(), values.size());
-                ch.bytes += h.data().size() + values.size();
+                put(packed.data(), packed.size());
+                ch.bytes += h.data().size() + packed.size();
+                ch.raw += h.data().size() + values.size();
             }
             rg.chunks.push_back(ch);
         }
@@ -232,7 +237,7 @@ class ParquetWriter {
             uint64_t total = 0;
             for (size_t c = 0; c < g.chunks.size(); ++c) {
                 const Chunk& ch = g.chunks[c];
-                total += ch.bytes;
+                total += ch.raw;  // total_byte_size: uncompressed
                 m.begin_elem_struct();
                 m.i64(2, (int64_t)ch.offset);
                 m.begin_struct(3);  // ColumnMetaData
@@ -241,10 +246,10 @@ class ParquetWriter {
                 m.elem_i32(parquet_detail::kPlain);
                 m.begin_list(3, Compact::kBinary, 1);
                 m.elem_str(schema_[c].name);
-                m.i32(4, parquet_detail::kUncompressed);
+                m.i32(4, parquet_detail::kSnappy);
                 m.i64(5, (int64_t)g.rows);
-                m.i64(6, (int64_t)ch.bytes);
-                m.i64(7, (int64_t)ch.bytes);
+                m.i64(6, (int64_t)ch.raw);    // total_uncompressed_size
+                m.i64(7, (int64_t)ch.bytes);  // total_compressed_size
                 m.i64(9, (int64_t)ch.offset);
                 m.end_struct();
                 m.end_elem_struct();
@@ -274,7 +279,7 @@ class ParquetWriter {
 
   private:
     struct Chunk {
-        uint64_t offset = 0, bytes = 0;
+        uint64_t offset = 0, bytes = 0, raw = 0;  // bytes as stored, and before compression
     };
     struct RowGroup {
         uint64_t rows = 0;

@@ -16,5 +16,6 @@
 #include "parquet.hpp"
 #include "program.hpp"
 #include "server.hpp"
+#include "snappy.hpp"
 #include "state.hpp"
 #include "thermostat.hpp"
