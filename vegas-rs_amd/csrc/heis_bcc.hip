@@ -30,7 +30,6 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
     const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
     const bool active = local < C.items;
     const uint64_t t = C.sweep0 + s;
-    int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
     int32_t s32[4] = {0, 0, 0, 0};     // mx, my, mz, axis2 of the lane's sites
     int64_t bsum = 0;                  // bond of the lane's sites
 
@@ -149,8 +148,7 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
                 make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
     }
 
-    heis_fold(s32, bsum, acc);
-    heis_partials(acc, partials, colour, 2u, s, total_blocks, b);
+    heis_partials_sites(s32, bsum, partials, colour, 2u, s, total_blocks, b);
 }
 
 }  // namespace vg

@@ -257,21 +257,18 @@ __device__ __forceinline__ void heis_propose(const ChainDev& C, uint32_t colour,
     }
 }
 
-// Workgroup sums of the five quantities -> partials[sweep][colour][block][5]
-// (ncol colours per sweep).
-__device__ __forceinline__ void heis_partials(int64_t acc[5], int64_t* __restrict__ partials,
-                                              uint32_t colour, uint32_t ncol, uint32_t s,
-                                              uint32_t total_blocks, uint32_t b) {
+// The workgroup's sums of the five quantities, from each wave's totals w
+// (equal in every lane) -> partials[sweep][colour][block][5] (ncol colours
+// per sweep).
+__device__ __forceinline__ void heis_block_partials(const int64_t w[5],
+                                                    int64_t* __restrict__ partials,
+                                                    uint32_t colour, uint32_t ncol, uint32_t s,
+                                                    uint32_t total_blocks, uint32_t b) {
     __shared__ int64_t red[5][kBlock / 64];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-    }
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) red[k][wave] = acc[k];
+        for (int k = 0; k < 5; ++k) red[k][wave] = w[k];
     }
     __syncthreads();
     if (threadIdx.x < 5) {
@@ -280,6 +277,43 @@ __device__ __forceinline__ void heis_partials(int64_t acc[5], int64_t* __restric
         for (uint32_t k = 0; k < kBlock / 64; ++k) v += red[threadIdx.x][k];
         partials[((uint64_t)(s * ncol + colour) * total_blocks + b) * 5u + threadIdx.x] = v;
     }
+}
+
+// Workgroup sums of the five quantities of any magnitude (the graph passes,
+// whose couplings enter the bond share): 64-bit shuffle reduction.
+__device__ __forceinline__ void heis_partials(int64_t acc[5], int64_t* __restrict__ partials,
+                                              uint32_t colour, uint32_t ncol, uint32_t s,
+                                              uint32_t total_blocks, uint32_t b) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    }
+    heis_block_partials(acc, partials, colour, ncol, s, total_blocks, b);
+}
+
+// Sum over the wave (every lane active) of a value below 2^40 in magnitude
+// in every lane: its low 16 bits and the rest (below 2^24) are summed in
+// int32 -- below 2^22 and 2^30 over 64 lanes, so exactly.
+__device__ __forceinline__ int64_t wave_sum_split(int64_t v) {
+    const int32_t lo = (int32_t)(v & 0xFFFF);
+    const int32_t hi = (int32_t)(v >> 16);
+    return (int64_t)wave_sum(hi) * 65536 + (int64_t)wave_sum(lo);
+}
+
+// Workgroup sums of the lattice passes (sc, bcc, fcc): a lane holds at most
+// four sites, each below 2^27 in every magnetisation sum and below
+// z * 2^27 <= 12 * 2^27 in its bond share, so every lane's five values are
+// below 2^40 and reduce with wave_sum_split.
+__device__ __forceinline__ void heis_partials_sites(const int32_t s32[4], int64_t bsum,
+                                                    int64_t* __restrict__ partials,
+                                                    uint32_t colour, uint32_t ncol, uint32_t s,
+                                                    uint32_t total_blocks, uint32_t b) {
+    int64_t w[5];
+    w[0] = wave_sum_split(bsum);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[1 + k] = wave_sum_split(s32[k]);
+    heis_block_partials(w, partials, colour, ncol, s, total_blocks, b);
 }
 
 }  // namespace vg

@@ -36,7 +36,6 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
     const uint32_t local = (b - C.block0) * kBlock + threadIdx.x;
     const bool active = local < C.items;
     const uint64_t t = C.sweep0 + s;
-    int64_t acc[5] = {0, 0, 0, 0, 0};  // bond, mx, my, mz, axis2
     int32_t s32[4] = {0, 0, 0, 0};     // mx, my, mz, axis2 of the lane's sites
     int64_t bsum = 0;                  // bond of the lane's sites
 
@@ -168,8 +167,7 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
         }
     }
 
-    heis_fold(s32, bsum, acc);
-    heis_partials(acc, partials, colour, 2u, s, total_blocks, b);
+    heis_partials_sites(s32, bsum, partials, colour, 2u, s, total_blocks, b);
 }
 
 // record = {bond, mx, my, mz, axis2}, summed over the ncol colour passes (a
