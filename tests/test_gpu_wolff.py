@@ -87,6 +87,46 @@ def test_wolff_set_state_and_batches_with_metropolis_chains(vg, orc, ctx):
             assert np.array_equal(b.get_ising(i), s), i
 
 
+def test_wolff_lds_and_global_paths(vg, orc, ctx):
+    """Chains of at most 2^19 sites keep spins and visit marks in LDS
+    (wolff_run_lds), larger ones run from global memory (wolff_run); both
+    kinds in one batch, and the small one alone, match the oracle's steps
+    bit for bit over two sweep blocks, at T_c (large clusters) and at a low
+    temperature (the cluster covers most of the lattice)."""
+    specs = [("sc", (82, 82, 80), 4.5115, 21),   # 537920 sites: global memory
+             ("sc", (80, 80, 80), 3.0, 22),      # 512000 sites: LDS, clusters of most sites
+             ("bcc", (64, 64, 65), 3.0, 23),     # 532480 sites: global memory
+             ("fcc", (8, 8, 8), 2.0, 24),        # 2048 sites at low T: LDS
+             ("sc", (46, 42, 34), 4.5115, 25)]   # LDS, a site count not a multiple of 32
+    descs, refs = [], []
+    for kind, dims, T, seed in specs:
+        descs.append(_chain(vg, kind, dims, (1, 1, 1), seed))
+        OL = orc.lattice(kind, *dims)
+        refs.append((OL, orc.model(terms=("exchange",), temperature=T), seed, T,
+                     orc.ising_init(OL, seed)))
+    with vg.Batch(ctx, descs) as b:
+        b.randomize()
+        for i, (_, _, _, T, _) in enumerate(refs):
+            b.set_thermostat(i, T)
+        step = 0
+        for n in (3, 40):
+            got = b.run_sums(n)
+            for i, (OL, OM, seed, _, st) in enumerate(refs):
+                want = orc.wolff_steps(OL, OM, seed, step, n, st)
+                assert np.array_equal(got[i], want), (i, n)
+                assert np.array_equal(b.get_ising(i), st), (i, n)
+            step += n
+    # the LDS chain alone continues as in the mix
+    OL, OM, seed, T, _ = refs[1]
+    st = orc.ising_init(OL, seed)
+    with vg.Batch(ctx, [descs[1]]) as b:
+        b.randomize()
+        b.set_thermostat(0, T)
+        got = b.run_sums(43)[0]
+        assert np.array_equal(got, orc.wolff_steps(OL, OM, seed, 0, 43, st))
+        assert np.array_equal(b.get_ising(0), st)
+
+
 def test_wolff_observables_follow_reference_total_energy(vg, orc, ctx):
     """E and |M| per step are the reference's Compound total energy and
     magnetization of the state (the field enters E, not the moves)."""

@@ -63,8 +63,9 @@ int validate_desc(const vg_chain_desc& d) {
         // integrator.rs:172-230 implements Wolff for Ising spins only
         if (d.spin != VG_SPIN_ISING)
             return fail(VG_ERR_NOT_IMPLEMENTED, "Wolff cluster steps run on int8 Ising chains");
-        if (sites_of(L) >= (1ull << 32))
-            return fail(VG_ERR_INVALID_ARGUMENT, "Wolff chains need fewer than 2^32 sites");
+        // site ids and cell coordinates by 31-bit magic division
+        if (sites_of(L) >= (1ull << 31))
+            return fail(VG_ERR_INVALID_ARGUMENT, "Wolff chains need fewer than 2^31 sites");
     }
     const vg_hamiltonian& H = d.hamiltonian;
     if (H.terms & ~15u) return fail(VG_ERR_INVALID_ARGUMENT, "unknown term bits 0x%x", H.terms);
@@ -219,6 +220,8 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
     d.nsites = sites_of(L);
     if (g == kIsingWolff) {  // natural order; no planes or work items
         d.plane = d.nsites;
+        magic_div(d.lx, d.lx_m, d.lx_s);
+        magic_div(d.ly, d.ly_m, d.ly_s);
         wolff_threshold(ch, d);
         return VG_OK;
     }
@@ -583,9 +586,26 @@ int enqueue_block(vg_batch* B, uint32_t nsw) {
         Group& G = B->groups[g];
         if (G.members.empty()) continue;
         dim3 grid((uint32_t)G.members.size(), nsw);
-        if (g == kIsingWolff)
-            hipLaunchKernelGGL(vg::wolff_run, dim3((uint32_t)G.members.size()), dim3(256), 0, st,
-                               G.d_chains, (uint8_t*)G.arena, nsw, B->d_records);
+        if (g == kIsingWolff) {
+            // chains of at most kWolffLdsSites sites in LDS, the rest in
+            // global memory; each kernel's blocks skip the other's chains
+            uint32_t lds_words = 0;
+            bool large = false;
+            for (const ChainDev& d : G.host) {
+                if (d.nsites <= vg::kWolffLdsSites)
+                    lds_words = std::max(lds_words, 2u * (uint32_t)((d.nsites + 31) / 32) +
+                                                        vg::kWolffLdsQueue);
+                else
+                    large = true;
+            }
+            const dim3 chains_grid((uint32_t)G.members.size());
+            if (lds_words)
+                hipLaunchKernelGGL(vg::wolff_run_lds, chains_grid, dim3(256), lds_words * 4u, st,
+                                   G.d_chains, (uint8_t*)G.arena, nsw, B->d_records);
+            if (large)
+                hipLaunchKernelGGL(vg::wolff_run, chains_grid, dim3(256), 0, st, G.d_chains,
+                                   (uint8_t*)G.arena, nsw, B->d_records);
+        }
         else if (is_ising(g) && !is_graph(g))
             hipLaunchKernelGGL(vg::ising_reduce, grid, dim3(vg::kBlock), 0, st, G.d_chains,
                                (const int32_t*)G.partials, G.total_blocks, colours_of(g), nsw,
@@ -1010,6 +1030,16 @@ int create_impl(vg_ctx* ctx, uint32_t n_chains, const vg_chain_desc* descs, cons
             return fail(e == hipErrorOutOfMemory ? VG_ERR_OUT_OF_MEMORY : VG_ERR_DEVICE,
                         "allocating the chain arena (%llu bytes): %s",
                         (unsigned long long)G.arena_bytes, hipGetErrorString(e));
+        }
+        if (g == kIsingWolff) {
+            // the LDS Wolff kernel's dynamic LDS goes past the 64 KiB default
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(vg::wolff_run_lds),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(vg::kWolffLdsWordsMax * 4u));
+            if (e != hipSuccess) {
+                free_batch(B);
+                return fail(VG_ERR_DEVICE, "Wolff LDS size: %s", hipGetErrorString(e));
+            }
         }
         if (is_graph(g)) {
             for (size_t k = 0; k < G.members.size(); ++k) {

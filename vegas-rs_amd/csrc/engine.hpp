@@ -52,6 +52,7 @@ template <uint32_t A>
 __global__ void heis_fcc_pass(const ChainDev*, const uint32_t*, float*, int64_t*, uint32_t,
                               uint32_t, uint32_t);
 __global__ void wolff_run(const ChainDev*, uint8_t*, uint32_t, int64_t*);
+__global__ void wolff_run_lds(const ChainDev*, uint8_t*, uint32_t, int64_t*);
 __global__ void wolff_init(const ChainDev*, uint8_t*);
 __global__ void heis_clear(const ChainDev*, float*);
 __global__ void heis_init(const ChainDev*, float*);

@@ -160,6 +160,7 @@ struct ChainDev {
     // a bond is active iff its 32-bit draw < wolff_thr, or always.
     uint64_t nsites;
     uint32_t wolff_thr, wolff_all;
+    uint32_t lx_m, lx_s;   // n / lx == (n * lx_m) >> lx_s for n < 2^31 (Wolff cells)
     // Graph chains (byte offsets in the arena): off[0] spins (Ising 1 byte
     // per position; Heisenberg x, y, z planes of nsites floats), off[1]
     // merged-row offsets (u32, nsites + 1), off[2] neighbour positions
@@ -179,6 +180,12 @@ constexpr uint32_t kScSeg = 32;        // positions per sc Ising work item (one 
 enum ScMode : uint32_t { kScInt8 = 0, kScBits = 1, kScShadow = 2 };
 constexpr uint32_t kHeisSeg = 4;       // positions per Heisenberg work item
 constexpr uint32_t kNumXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
+// Wolff chains of at most this many sites keep their spins and visit marks
+// in LDS, one bit per site each (2 x 64 KiB of the 160 KiB of a CU), and the
+// first kWolffLdsQueue entries of the cluster queue (31 KiB).
+constexpr uint32_t kWolffLdsSites = 1u << 19;
+constexpr uint32_t kWolffLdsQueue = 7936;
+constexpr uint32_t kWolffLdsWordsMax = 2u * (kWolffLdsSites / 32u) + kWolffLdsQueue;
 
 // Exact unsigned division by a per-chain constant d for n < 2^31:
 // m = ceil(2^s / d) with s = 31 + ceil(log2 d) (computed on the host).
