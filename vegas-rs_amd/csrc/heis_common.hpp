@@ -205,11 +205,12 @@ __device__ __forceinline__ int32_t fix27_32(float v) { return __float2int_rn(fm(
 // and (s.axis)^2 to s32 (each at most 2^27 in magnitude per site, so a lane
 // may collect up to 15 sites in int32) and, when `bond` is set, n.hb to
 // bsum (hb: the sum over the neighbours of lower colours; for the
-// two-colour lattices every neighbour of a colour-1 site).
+// two-colour lattices every neighbour of a colour-1 site).  A site that is
+// not `valid` (zero padding) keeps o and adds nothing.
 __device__ __forceinline__ void heis_update(const ChainDev& C, bool bond, const float o[3],
                                             const float h[3], const float hb[3], float u,
                                             const float p[3], float n[3], int32_t s32[4],
-                                            int64_t& bsum) {
+                                            int64_t& bsum, bool valid = true) {
     const float ox = o[0], oy = o[1], oz = o[2];
     const float px = p[0], py = p[1], pz = p[2];
     const float ax = C.ax, ay = C.ay, az = C.az;
@@ -220,8 +221,8 @@ __device__ __forceinline__ void heis_update(const ChainDev& C, bool bond, const 
     const float e_an = fm(C.kk, fs(fm(pe, pe), fm(oe, oe)));
     const float e_ze = fm(C.hm, dot3(dx, dy, dz, C.fx, C.fy, C.fz));
     const float dE = fa(fa(e_ex, e_an), e_ze);
-    bool accept = dE < 0.0f;
-    if (!accept) accept = u < exp_nonpos(-fm(dE, C.beta));
+    bool accept = valid && dE < 0.0f;
+    if (valid && !accept) accept = u < exp_nonpos(-fm(dE, C.beta));
     n[0] = accept ? px : ox;
     n[1] = accept ? py : oy;
     n[2] = accept ? pz : oz;

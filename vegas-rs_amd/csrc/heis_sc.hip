@@ -22,7 +22,7 @@
 
 namespace vg {
 
-__global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__ chains,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void heis_sc_pass(const ChainDev* __restrict__ chains,
                                                     const uint32_t* __restrict__ block_chain,
                                                     float* __restrict__ arena,
                                                     int64_t* __restrict__ partials,
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
     int64_t bsum = 0;                  // bond of the lane's sites
 
     // Phase 1: neighbour fields, old spins and proposals of the item's sites.
-    uint32_t row = 0, i0 = 0, nvalid = 0, pend = 0;
+    uint32_t row, i0, nvalid, pend = 0;
     // Component planes through buffer resources: 32-bit byte offsets (a
     // component plane is below 4 GiB, checked on the host), and kNoRow reads
     // 0 for an absent neighbour row.
@@ -56,25 +56,22 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
                 (uint32_t)(plane * 4u));
         }
     }
-    uint32_t own_off = 0;
+    // Every lane runs the same code: a lane past the chain's items (the
+    // tail of its last block) has no valid site and its own row offset is
+    // kNoRow, so it reads zeros and its stores are dropped; a site j >=
+    // nvalid of a short last segment is padding, which every kernel keeps at
+    // zero.  Neither is ever accepted, so both add zero to the sums.
+    row = fast_div(local, C.segs_m, C.segs_s);
+    const uint32_t seg = local - row * C.segs;
+    const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
+    const uint32_t z = fast_div(row, C.ly_m, C.ly_s), y = row - z * ly;
+    const uint32_t q = (colour + y + z) & 1u;
+    i0 = seg * kHeisSeg;
+    nvalid = active ? min(kHeisSeg, w - i0) : 0u;
+    const uint32_t own_off = active ? (row * pitch + i0) * 4u : kNoRow;
+    const uint32_t orow = row * pitch * 4u;
     float os[4][3], hs[4][3], ps[4][3], us[4];
-#pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-        us[j] = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) os[j][c] = hs[j][c] = ps[j][c] = 0.0f;
-    }
-    if (active) {
-        row = fast_div(local, C.segs_m, C.segs_s);
-        const uint32_t seg = local - row * C.segs;
-        const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
-        const uint32_t z = fast_div(row, C.ly_m, C.ly_s), y = row - z * ly;
-        const uint32_t q = (colour + y + z) & 1u;
-        i0 = seg * kHeisSeg;
-        nvalid = min(kHeisSeg, w - i0);
-        own_off = (row * pitch + i0) * 4u;
-        const uint32_t orow = row * pitch * 4u;
-
+    {
         const bool ym_ok = (y > 0) || (pbc & 2u);
         const bool yp_ok = (y + 1 < ly) || (pbc & 2u);
         const bool zm_ok = (z > 0) || (pbc & 4u);
@@ -116,7 +113,6 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
         for (uint32_t j = 0; j < 4; ++j) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) os[j][c] = f4get(so[c], j);
-            if (j >= nvalid) continue;
             const uint32_t pos = i0 + j;
             // x neighbours, -x then +x: positions pos-1, pos (q = 0) or
             // pos, pos+1 (q = 1) of the other plane's row.
@@ -140,18 +136,20 @@ __global__ __launch_bounds__(256) void heis_sc_pass(const ChainDev* __restrict__
             }
             heis_propose(C, colour, pos, row, t, fw, j, os[j], us[j], ps[j], pend);
         }
+        pend &= (1u << nvalid) - 1u;  // no further calls for sites that do not exist
     }
 
     // Phase 2: the wave's sites whose first proposal call missed.
     if (C.proposal) heis_resolve(C, hq[threadIdx.x >> 6], pend, i0, row, t, colour, ps);
 
     // Phase 3: accept or reject, store, sums.
-    if (active) {
+    {
         float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
-            float n[3] = {os[j][0], os[j][1], os[j][2]};
-            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, s32, bsum);
+            float n[3];
+            heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, s32, bsum,
+                        j < nvalid);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];
