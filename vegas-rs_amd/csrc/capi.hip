@@ -257,7 +257,7 @@ int init_chain_dev(const ChainHost& ch, int g, uint32_t slot, ChainDev& d) {
         return fail(VG_ERR_UNSUPPORTED_LATTICE,
                     "chain %u: an Ising colour plane of %llu bytes exceeds 4 GiB", slot,
                     (unsigned long long)d.plane);
-    if (g == kHeisSc && 4 * d.plane > 0xFFFFF000ull)
+    if ((g == kHeisSc || g == kHeisBcc || g == kHeisFcc) && 4 * d.plane > 0xFFFFF000ull)
         return fail(VG_ERR_UNSUPPORTED_LATTICE,
                     "chain %u: a Heisenberg colour plane component of %llu bytes exceeds 4 GiB",
                     slot, (unsigned long long)(4 * d.plane));

@@ -16,7 +16,7 @@
 
 namespace vg {
 
-__global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict__ chains,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void heis_bcc_pass(const ChainDev* __restrict__ chains,
                                                      const uint32_t* __restrict__ block_chain,
                                                      float* __restrict__ arena,
                                                      int64_t* __restrict__ partials,
@@ -34,28 +34,38 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
     int64_t bsum = 0;                  // bond of the lane's sites
 
     // Phase 1: neighbour fields, old spins and proposals of the item's sites.
-    uint32_t row = 0, i0 = 0, nvalid = 0, pend = 0;
-    float* own = arena;
-    float os[4][3], hs[4][3], ps[4][3], us[4];
-#pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-        us[j] = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) os[j][c] = hs[j][c] = ps[j][c] = 0.0f;
-    }
-    if (active) {
-        const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
+    // Component planes through buffer resources: 32-bit byte offsets (a
+    // component plane is below 4 GiB, checked on the host), and kNoRow reads
+    // 0 for an absent neighbour row.
+    __amdgpu_buffer_rsrc_t rown[3], roth[3];
+    {
         const uint64_t plane = C.plane;
-        row = fast_div(local, C.segs_m, C.segs_s);
-        const uint32_t seg = local - row * C.segs;
-        const uint32_t z = fast_div(row, C.ly_m, C.ly_s);
-        const uint32_t y = row - z * ly;
-        const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
-        i0 = seg * kHeisSeg;
-        nvalid = min(kHeisSeg, w - i0);
-        own = arena + C.off[colour] + (uint64_t)row * pitch + i0;
-        const float* oth = arena + C.off[colour ^ 1u];
-
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            rown[c] = plane_rsrc(reinterpret_cast<const uint8_t*>(arena + C.off[colour] + c * plane),
+                                 (uint32_t)(plane * 4u));
+            roth[c] = plane_rsrc(
+                reinterpret_cast<const uint8_t*>(arena + C.off[colour ^ 1u] + c * plane),
+                (uint32_t)(plane * 4u));
+        }
+    }
+    // Every lane runs the same code: a lane past the chain's items (the
+    // tail of its last block) has no valid site and its own row offset is
+    // kNoRow, so it reads zeros and its stores are dropped; a site j >=
+    // nvalid of a short last segment is padding, which every kernel keeps at
+    // zero.  Neither is ever accepted, so both add zero to the sums.
+    const uint32_t ly = C.ly, lz = C.lz, w = C.w, pitch = C.pitch, pbc = C.pbc;
+    const uint32_t row = fast_div(local, C.segs_m, C.segs_s);
+    const uint32_t seg = local - row * C.segs;
+    const uint32_t z = fast_div(row, C.ly_m, C.ly_s);
+    const uint32_t y = row - z * ly;
+    const bool px = pbc & 1u, py = pbc & 2u, pz = pbc & 4u;
+    const uint32_t i0 = seg * kHeisSeg;
+    const uint32_t nvalid = active ? min(kHeisSeg, w - i0) : 0u;
+    const uint32_t own_off = active ? (row * pitch + i0) * 4u : kNoRow;
+    uint32_t pend = 0;
+    float os[4][3], hs[4][3], ps[4][3], us[4];
+    {
         bool yok, zok;
         uint32_t y2, z2;
         if (colour == 0) {
@@ -82,21 +92,20 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
             xe_pos = i0 + kHeisSeg < w ? i0 + kHeisSeg : 0;
             xe_ok = i0 + kHeisSeg < w || px;
         }
-        const bool last = i0 + kHeisSeg >= w;
 
         float4 so[3], v[4][3];
-        float xe[4][3], wrap0[4][3];
+        // colour 1: the last segment's extra position is 0, the +x slot of
+        // position w-1 across the periodic end (0 when x is open)
+        float xe[4][3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) so[c] = ld4(own + c * plane, true);
+        for (int c = 0; c < 3; ++c) so[c] = bload4f(rown[c], own_off);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float* rp = oth + (uint64_t)rows[r] * pitch;
+            const uint32_t ro = rows[r] * pitch * 4u;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                v[r][c] = ld4(rp + c * plane + i0, rok[r]);
-                xe[r][c] = rok[r] && xe_ok ? rp[c * plane + xe_pos] : 0.0f;
-                // colour 1: position w-1 bonds to position 0 across the end
-                wrap0[r][c] = colour == 1 && last && px && rok[r] ? rp[c * plane] : 0.0f;
+                v[r][c] = bload4f(roth[c], rok[r] ? ro + i0 * 4u : kNoRow);
+                xe[r][c] = bload1f(roth[c], rok[r] && xe_ok ? ro + xe_pos * 4u : kNoRow);
             }
         }
 
@@ -106,7 +115,6 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
         for (uint32_t j = 0; j < 4; ++j) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) os[j][c] = f4get(so[c], j);
-            if (j >= nvalid) continue;
             const uint32_t pos = i0 + j;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -118,34 +126,40 @@ __global__ __launch_bounds__(256) void heis_bcc_pass(const ChainDev* __restrict_
                     if (colour == 0)
                         bx = j == 0 ? xe[r][c] : f4get(v[r][c], j - 1);
                     else
-                        bx = pos + 1 == w ? wrap0[r][c] : (j == 3 ? xe[r][c] : f4get(v[r][c], j + 1));
+                        bx = (pos + 1 == w || j == 3) ? xe[r][c] : f4get(v[r][c], j + 1);
                     acc_h = r == 0 ? fa(a, bx) : fa(fa(acc_h, a), bx);
                 }
                 hs[j][c] = acc_h;
             }
             heis_propose(C, colour, pos, row, t, fw, j, os[j], us[j], ps[j], pend);
         }
+        pend &= (1u << nvalid) - 1u;  // no further calls for sites that do not exist
     }
 
     // Phase 2: the wave's sites whose first proposal call missed.
     if (C.proposal) heis_resolve(C, hq[threadIdx.x >> 6], pend, i0, row, t, colour, ps);
 
     // Phase 3: accept or reject, store, sums.
-    if (active) {
-        const uint64_t plane = C.plane;
+    {
         float ns[3][4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
-            float n[3] = {os[j][0], os[j][1], os[j][2]};
-            if (j < nvalid) heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, s32, bsum);
+            float n[3];
+            heis_update(C, colour == 1, os[j], hs[j], hs[j], us[j], ps[j], n, s32, bsum,
+                        j < nvalid);
             ns[0][j] = n[0];
             ns[1][j] = n[1];
             ns[2][j] = n[2];
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            *reinterpret_cast<float4*>(own + c * plane) =
-                make_float4(ns[c][0], ns[c][1], ns[c][2], ns[c][3]);
+        for (int c = 0; c < 3; ++c) {
+            u32x4 vv;
+            vv.x = __float_as_uint(ns[c][0]);
+            vv.y = __float_as_uint(ns[c][1]);
+            vv.z = __float_as_uint(ns[c][2]);
+            vv.w = __float_as_uint(ns[c][3]);
+            __builtin_amdgcn_raw_buffer_store_b128(vv, rown[c], own_off, 0, 0);
+        }
     }
 
     heis_partials_sites(s32, bsum, partials, colour, 2u, s, total_blocks, b);

@@ -92,9 +92,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
             xe_pos = i0 + kHeisSeg < w ? i0 + kHeisSeg : 0;
             xe_ok = i0 + kHeisSeg < w || (pbc & 1u);
         }
-        // Position w-1 with q = 1 wraps to position 0 (or has no +x bond).
-        const bool wrap_ok = q == 1 && i0 + kHeisSeg >= w && (pbc & 1u);
-        float wrap0[3];
+        // With q = 1 the last segment's extra neighbour is position 0: the
+        // +x neighbour of position w-1 across the periodic end (0 when x is
+        // open), whichever j that position has.
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             so[c] = bload4f(rown[c], own_off);
@@ -104,7 +104,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
             zm[c] = bload4f(roth[c], zm_ok ? (r_zm * pitch + i0) * 4u : kNoRow);
             zp[c] = bload4f(roth[c], zp_ok ? (r_zp * pitch + i0) * 4u : kNoRow);
             xe[c] = bload1f(roth[c], xe_ok ? orow + xe_pos * 4u : kNoRow);
-            wrap0[c] = bload1f(roth[c], wrap_ok ? orow : kNoRow);
         }
 
         U4 fw = U4{0u, 0u, 0u, 0u};
@@ -125,7 +124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void h
                     npx = here;
                 } else {
                     nmx = here;
-                    npx = pos + 1 == w ? wrap0[c] : (j == 3 ? xe[c] : f4get(mid[c], j + 1));
+                    npx = (pos + 1 == w || j == 3) ? xe[c] : f4get(mid[c], j + 1);
                 }
                 float v = fa(nmx, npx);
                 v = fa(v, f4get(ym[c], j));
