@@ -130,6 +130,7 @@ typedef struct vg_hamiltonian {
 /* "sweep" of vg_batch_run is one Wolff step, i.e. one cluster grown from */
 /* a random site with bond probability 1 - exp(-2J/T) over equal spins    */
 /* and flipped (the move ignores the field, as the reference's does).     */
+/* Chains of fewer than 2^31 sites (VG_ERR_INVALID_ARGUMENT otherwise).   */
 #define VG_PROPOSAL_WOLFF 2
 
 typedef struct vg_chain_desc {
