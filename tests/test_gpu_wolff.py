@@ -175,6 +175,12 @@ def test_wolff_refused_combinations(vg, ctx):
         with pytest.raises(vg.VegasError) as e:
             vg.Batch(ctx, [vg.chain(L, spin, vg.WOLFF, vg.hamiltonian(exchange=1.0), 1)])
         assert e.value.kind == "NotImplemented"
+    # 2^31 sites: past the 31-bit site ids of the Wolff kernels (refused
+    # before anything is allocated)
+    big = vg.lattice(vg.SC, 2048, 1024, 1024)
+    with pytest.raises(vg.VegasError) as e:
+        vg.Batch(ctx, [vg.chain(big, vg.ISING, vg.WOLFF, vg.hamiltonian(exchange=1.0), 1)])
+    assert e.value.kind == "InvalidArgument"
     with vg.Server(ctx, memory_bytes=1 << 20, max_jobs=2, quantum_sweeps=10) as srv:
         with pytest.raises(vg.VegasError) as e:
             srv.submit(vg.chain(L, vg.ISING, vg.WOLFF, vg.hamiltonian(exchange=1.0), 1),
