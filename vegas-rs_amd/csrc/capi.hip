@@ -349,7 +349,10 @@ int layout_work(vg_batch* B, int g, uint32_t tile) {
             items = d.items;  // one thread per site of the largest colour class (set at creation)
         } else if (is_zmarch(g)) {
             d.nzc = (d.lz + tile - 1) / tile;
-            items = (uint64_t)d.nzc * 2 * d.hy * d.segs;
+            // sc: rows in natural order; bcc, fcc: the rows of one y parity,
+            // then the other (2 * hy rows, the last one past an odd ly idle)
+            const bool sc = g == kIsingSc || g == kIsingBits;
+            items = (uint64_t)d.nzc * (sc ? d.ly : 2 * d.hy) * d.segs;
         } else {
             items = (uint64_t)d.rows * d.segs;
         }

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     const uint32_t pitch = VG_FIELD(pitch), pbc = VG_FIELD(pbc);
     const uint32_t items = VG_FIELD(items), block0 = VG_FIELD(block0);
     const uint32_t segs = VG_FIELD(segs), segs_m = VG_FIELD(segs_m), segs_s = VG_FIELD(segs_s);
-    const uint32_t hy = VG_FIELD(hy), hy_m = VG_FIELD(hy_m), hy_s = VG_FIELD(hy_s);
+    const uint32_t ly_m = VG_FIELD(ly_m), ly_s = VG_FIELD(ly_s);
     const uint32_t plane = (uint32_t)VG_FIELD64(plane);
     const uint32_t spitch = VG_FIELD(spitch), rows = VG_FIELD(rows);
     const uint64_t soff_own = colour ? VG_FIELD64(soff[1]) : VG_FIELD64(soff[0]);
@@ -281,11 +281,13 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
     uint32_t ups = 0;  // up sites after each plane's update, summed
     uint32_t seg = 0, y = 0, zc = 0;
     if (active && local < items) {
+        // work items in natural order: segments of a row, rows y of a z
+        // chunk, chunks; rows y and y+1 of one plane are neighbours in
+        // memory (one 128-byte line for rows of 64 bytes) and in the grid
         const uint32_t r = fast_div(local, segs_m, segs_s);
         seg = local - r * segs;
-        const uint32_t r2 = fast_div(r, hy_m, hy_s);
-        y = 2u * (r - r2 * hy) + (r2 & 1u);
-        zc = r2 >> 1;
+        zc = fast_div(r, ly_m, ly_s);
+        y = r - zc * ly;
     }
     if (active && local < items && y < ly) {
         const uint32_t z0 = zc * K;
@@ -492,9 +494,9 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t l = (b - block0) * kBlock + owner;
             const uint32_t r = fast_div(l, segs_m, segs_s);
             const uint32_t sg = l - r * segs;
-            const uint32_t r2 = fast_div(r, hy_m, hy_s);
-            const uint32_t yy = 2u * (r - r2 * hy) + (r2 & 1u);
-            const uint32_t zz = (r2 >> 1) * K + kz;
+            const uint32_t zq = fast_div(r, ly_m, ly_s);
+            const uint32_t yy = r - zq * ly;
+            const uint32_t zz = zq * K + kz;
             const uint32_t row = yy + ly * zz;
             const uint32_t i0 = sg * kScSeg;
             const uint32_t bits = rec[5];

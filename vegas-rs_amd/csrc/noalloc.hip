@@ -100,7 +100,7 @@ extern "C" int vg_run_sums_noalloc(const vg_chain_desc* desc, double temperature
     for (uint32_t c = 0; c < d.ncol; ++c) d.off[c] = c * plane_bytes;
     const uint32_t tile = 4;
     d.nzc = (d.lz + tile - 1) / tile;
-    const uint64_t items = (uint64_t)d.nzc * 2 * d.hy * d.segs;
+    const uint64_t items = (uint64_t)d.nzc * d.ly * d.segs;  // the sc pass's natural row order
     if (items > 0x7FFFFFFFull - vg::kBlock)
         return fail(VG_ERR_INVALID_ARGUMENT, "chain too large");
     d.items = (uint32_t)items;

@@ -125,7 +125,8 @@ struct ChainDev {
     uint32_t limit;        // sweeps of the current block this chain runs (the rest skip it)
     // n / d == (n * m) >> s for n < 2^31 (segs, ly and hy divisors)
     uint32_t segs_m, segs_s, ly_m, ly_s;
-    // Ising work order: rows of one y parity, a tile of z planes per lane
+    // Ising work order (bcc, fcc): rows of one y parity, then the other, a
+    // tile of z planes per lane (the sc pass takes the rows in natural order)
     uint32_t hy;           // (ly + 1) / 2, rows of one parity class per plane
     uint32_t hy_m, hy_s;
     uint32_t nzc;          // z chunks of one tile of planes
