@@ -24,6 +24,7 @@ core), and the served Poisson job stream with p50/p99 latency (--serve-jobs
 jobs per rate).
 """
 import argparse
+import hashlib
 import importlib.util
 import json
 import os
@@ -35,6 +36,17 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 COUNTERS = ROOT / "profiles" / "pass_counters.txt"
+CSRC = ROOT / "vegas-rs_amd" / "csrc"
+# The sources each pass kernel is compiled from (the file and the headers it
+# includes): a counter row applies to the build only while they are unchanged.
+KERNEL_SOURCES = {
+    "ising_sc_pass": ["ising_sc.hip", "ising_swar.hpp", "vg_device.hpp"],
+    "ising_bcc_pass": ["ising_bcc.hip", "ising_swar.hpp", "vg_device.hpp"],
+    "ising_fcc_pass": ["ising_fcc.hip", "fcc_common.hpp", "ising_swar.hpp", "vg_device.hpp"],
+    "heis_sc_pass": ["heis_sc.hip", "heis_common.hpp", "vg_device.hpp"],
+    "heis_bcc_pass": ["heis_bcc.hip", "heis_common.hpp", "vg_device.hpp"],
+    "heis_fcc_pass": ["heis_fcc.hip", "fcc_common.hpp", "heis_common.hpp", "vg_device.hpp"],
+}
 SIMDS, CLOCK_GHZ = 1024, 2.4  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
 METRIC = "spin-flip attempts/s (Ising sc 256³); batched vs plain; p50/p99 job latency"
 
@@ -51,22 +63,37 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def source_digest(kernel):
+    """SHA-1 (12 hex digits) of the sources a pass kernel is compiled from."""
+    h = hashlib.sha1()
+    for f in KERNEL_SOURCES[kernel.split("<")[0]]:
+        h.update((CSRC / f).read_bytes())
+    return h.hexdigest()[:12]
+
+
 def pass_counters(workload, kernels):
     """Counter-measured bytes per dispatch (FETCH_SIZE + WRITE_SIZE, scaled by
     a device copy of known size) and VALU instructions per dispatch of the
     named pass kernels -- the mean over them -- from the table the rocprofv3
     runs of this workload wrote (profiles/pass_counters.txt).  None when the
-    table has no row for one of them (another tile, say)."""
+    table has no row for one of them (another tile, say) or when a kernel's
+    sources differ from those the table was measured with (its "# sources"
+    line holds their digests)."""
     if not COUNTERS.exists():
         return None
-    rows = {}
+    rows, digests = {}, {}
     for line in COUNTERS.read_text().splitlines():
+        if line.startswith("# sources:"):
+            digests = dict(kv.split("=") for kv in line[len("# sources:"):].split())
+            continue
         if line.startswith("#") or not line.strip():
             continue
         w, k, _, fetch, write, valu, _ = line.split("\t")
         rows[(w, k)] = (float(fetch) + float(write), float(valu))
     got = [rows.get((workload, k)) for k in kernels]
     if any(g is None for g in got):
+        return None
+    if any(digests.get(k.split("<")[0]) != source_digest(k) for k in kernels):
         return None
     return {"traffic": sum(g[0] for g in got) / len(got),
             "valu": sum(g[1] for g in got) / len(got),
