@@ -65,14 +65,18 @@ struct RowRaw<false> {
     u32x4 a, b;
 };
 
+// full = false: the segment's positions 16..31 are all padding (the last
+// segment of a row of at most 16 more positions); their bytes are neither
+// read (they read 0) nor written.
 template <bool BITS>
-__device__ __forceinline__ RowRaw<BITS> load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+__device__ __forceinline__ RowRaw<BITS> load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                 bool full = true) {
     RowRaw<BITS> v;
     if constexpr (BITS) {
         v.w = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
     } else {
         v.a = bload16(r, off);
-        v.b = bload16(r, off + 16u);
+        v.b = bload16(r, full ? off + 16u : kNoRow);
     }
     return v;
 }
@@ -86,12 +90,14 @@ __device__ __forceinline__ uint32_t site_bits(const RowRaw<BITS>& v) {
 }
 
 template <bool BITS>
-__device__ __forceinline__ uint32_t load_word(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return site_bits<BITS>(load_raw<BITS>(r, off));
+__device__ __forceinline__ uint32_t load_word(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                              bool full = true) {
+    return site_bits<BITS>(load_raw<BITS>(r, off, full));
 }
 
 template <bool BITS>
-__device__ __forceinline__ void store_word(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
+__device__ __forceinline__ void store_word(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v,
+                                           bool full = true) {
     if (BITS) {
         __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
         return;
@@ -106,7 +112,7 @@ __device__ __forceinline__ void store_word(__amdgpu_buffer_rsrc_t r, uint32_t of
     b.z = nibble_bytes(v >> 24);
     b.w = nibble_bytes(v >> 28);
     __builtin_amdgcn_raw_buffer_store_b128(a, r, off, 0, 0);
-    __builtin_amdgcn_raw_buffer_store_b128(b, r, off + 16u, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(b, r, full ? off + 16u : kNoRow, 0, 0);  // kNoRow: dropped
 }
 
 template <bool BITS>
@@ -327,8 +333,9 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         };
         // the row window along z: the plane behind the march and the current one
         bool behind_ok;
-        uint32_t behind = load_word<NB>(ro, zrow(zfirst, noff, down, behind_ok));
-        uint32_t here = load_word<NB>(ro, noff);
+        const bool full = nvalid > 16u;  // the segment's upper 16 positions hold sites
+        uint32_t behind = load_word<NB>(ro, zrow(zfirst, noff, down, behind_ok), full);
+        uint32_t here = load_word<NB>(ro, noff, full);
 
         const uint32_t wP = ctr_w(t, kIsingProposal, colour);
         // not unrolled: one plane's code is large, and the instruction
@@ -349,10 +356,10 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             // position i0-1 (q = 0), or i0+32, or position 0 across the
             // periodic end (q = 1); it reads 0 when x is open there.
             const uint32_t q = (colour + y + z) & 1u;
-            const RowRaw<BITS> c_o = load_raw<BITS>(rw_own, off);
-            const RowRaw<NB> c_ym = load_raw<NB>(ro, ym_ok ? noff + dym : kNoRow);
-            const RowRaw<NB> c_yp = load_raw<NB>(ro, yp_ok ? noff + dyp : kNoRow);
-            const RowRaw<NB> c_ab = load_raw<NB>(ro, ahead_off);
+            const RowRaw<BITS> c_o = load_raw<BITS>(rw_own, off, full);
+            const RowRaw<NB> c_ym = load_raw<NB>(ro, ym_ok ? noff + dym : kNoRow, full);
+            const RowRaw<NB> c_yp = load_raw<NB>(ro, yp_ok ? noff + dyp : kNoRow, full);
+            const RowRaw<NB> c_ab = load_raw<NB>(ro, ahead_off, full);
             const uint32_t extra =
                 load_pos<NB>(ro, noff - word_off<NB>(i0), q ? (xr_in ? i0 + kScSeg : 0u) : xl_pos,
                                q ? (xr_in || px) : xl_ok);
@@ -449,7 +456,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             }
             const uint32_t n = o ^ acc;
             if (acc) {
-                store_word<BITS>(rw_own, off, n);
+                store_word<BITS>(rw_own, off, n, full);
                 if (SH) __builtin_amdgcn_raw_buffer_store_b32(n, rw_sh, noff, 0, 0);
             }
             ups += __popc(n & vmask);
