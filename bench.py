@@ -125,7 +125,9 @@ def with_counters(roofline, workload, kernels, mads_per_launch=None):
 def ising_mads(attempts_per_launch):
     """v_mad_u64_u32 (wave instructions) of an Ising pass launch with flip
     proposals: one Philox4x32-10 call (20 of them) per 16 sites, 64 lanes per
-    wave instruction (tie draws, a few per 10^3 sites, left out)."""
+    wave instruction (tie draws, a few per 10^3 sites, left out).  An upper
+    bound for the sc pass: the multiplies of the first two rounds that do not
+    depend on the row are computed once per z march, not once per plane."""
     return 20.0 * attempts_per_launch / 1024.0
 
 

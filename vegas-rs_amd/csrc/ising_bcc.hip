@@ -164,6 +164,11 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
             uint32_t e_hi;
             load_pair(zu, hi_ok, q_hi, e_hi);
             const u32x4 o4 = load_seg<BITS>(rw_own, off);
+            // The draws do not wait for the loads.  The compiler would sink
+            // them past the branches below, behind the wait for the loads:
+            // an empty asm statement taking them keeps them ahead.
+            U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, colour)}, k0, k1);
+            asm volatile("" : "+v"(ra.x), "+v"(ra.y), "+v"(ra.z), "+v"(ra.w));
             // present rows of the four; absent slots 8 - 2 nr (8 - nr at gx)
             const uint32_t nr = (lo_ok ? 1u : 0u) + (lo_ok && yok ? 1u : 0u) + (hi_ok ? 1u : 0u) +
                                 (hi_ok && yok ? 1u : 0u);
@@ -196,7 +201,6 @@ __global__ __launch_bounds__(256) void ising_bcc_pass(const ChainDev* __restrict
             for (int kk = 0; kk < 4; ++kk) cnt[kk] = P[kk] + xs[kk];  // 0..8 per byte
 
             const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
-            const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, colour)}, k0, k1);
             const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
             uint32_t live[4] = {valid[0], valid[1], valid[2], valid[3]};
             if (C.proposal) {

@@ -254,7 +254,11 @@ __global__ __launch_bounds__(256) void ising_fcc_pass(const ChainDev* __restrict
             }
 
             const uint32_t o[4] = {o4.x, o4.y, o4.z, o4.w};
-            const U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, A)}, k0, k1);
+            // The draws do not wait for the loads.  The compiler would sink
+            // them past the branch below, behind the wait for the loads: an
+            // empty asm statement taking them keeps them ahead of it.
+            U4 ra = philox(U4{i0 >> 4, row, (uint32_t)t, ctr_w(t, kIsingA, A)}, k0, k1);
+            asm volatile("" : "+v"(ra.x), "+v"(ra.y), "+v"(ra.z), "+v"(ra.w));
             const uint32_t rw[4] = {ra.x, ra.y, ra.z, ra.w};
             uint32_t live[4] = {valid[0], valid[1], valid[2], valid[3]};
             if (C.proposal) {

@@ -338,6 +338,22 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
         uint32_t here = load_word<NB>(ro, noff, full);
 
         const uint32_t wP = ctr_w(t, kIsingProposal, colour);
+        // Along the march: the plane ahead of plane z is z + 1 (up) or z - 1
+        // (down), or across the periodic end from the lattice's last (up) or
+        // first (down) plane, where it is absent when z is open.
+        const uint32_t zend = down ? 0u : lz - 1u;
+        const uint32_t nwrap = down ? (lz - 1) * nzstep : 0u - (lz - 1) * nzstep;
+        const uint32_t rstep = down ? 0u - ly : ly;
+        // x neighbours of position i: i-1, i (q = 0) or i, i+1 (q = 1) in the
+        // other plane's same row; q alternates from plane to plane.  The bit
+        // outside the segment is position i0-1 (q = 0), or i0+32, or position
+        // 0 across the periodic end (q = 1); it reads 0 when x is open there,
+        // and then xe is its site, the one with an absent x slot (32: none).
+        const uint32_t xpos0 = xl_pos, xpos1 = xr_in ? i0 + kScSeg : 0u;
+        const bool xok0 = xl_ok, xok1 = xr_in || px;
+        const uint32_t xe0 = !xl_ok ? 0u : 32u, xe1 = !xr_in && !px ? nvalid - 1u : 32u;
+        uint32_t z = zfirst, row = y + ly * zfirst;
+        uint32_t q = (colour + y + zfirst) & 1u;
         // not unrolled: one plane's code is large, and the instruction
         // cache is shared by the waves of two CUs
 #pragma unroll 1
@@ -347,42 +363,29 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             // marches, so the waves of a SIMD progress together and the pass
             // does not end in a tail of a few young waves.
             if (prio) set_prio(((K - 1u - k) * 4u) / K);
-            const uint32_t z = down ? zfirst - k : z0 + k;
-            const uint32_t row = y + ly * z;
-            bool ahead_ok;
-            const uint32_t ahead_off = zrow(z, noff, !down, ahead_ok);
-            // x neighbours of position i: i-1, i (q = 0) or i, i+1 (q = 1) in
-            // the other plane's same row.  The bit outside the segment is
-            // position i0-1 (q = 0), or i0+32, or position 0 across the
-            // periodic end (q = 1); it reads 0 when x is open there.
-            const uint32_t q = (colour + y + z) & 1u;
+            const bool at_end = z == zend;
+            const bool ahead_ok = !at_end || pz;
+            const uint32_t ahead_off = !at_end ? noff + ndstep : (pz ? noff + nwrap : kNoRow);
             const RowRaw<BITS> c_o = load_raw<BITS>(rw_own, off, full);
             const RowRaw<NB> c_ym = load_raw<NB>(ro, ym_ok ? noff + dym : kNoRow, full);
             const RowRaw<NB> c_yp = load_raw<NB>(ro, yp_ok ? noff + dyp : kNoRow, full);
             const RowRaw<NB> c_ab = load_raw<NB>(ro, ahead_off, full);
             const uint32_t extra =
-                load_pos<NB>(ro, noff - word_off<NB>(i0), q ? (xr_in ? i0 + kScSeg : 0u) : xl_pos,
-                               q ? (xr_in || px) : xl_ok);
+                load_pos<NB>(ro, noff - word_off<NB>(i0), q ? xpos1 : xpos0, q ? xok1 : xok0);
 
-            // draws first: they do not wait for the loads
-            uint32_t prop = 0;
-            if (rnd) {
-                // Spin::rand proposes Up when its draw is below 1/2
-                // (state.rs:76-84): proposal bit 0 -> Up.  A proposal equal
-                // to the current spin changes nothing.
-                const U4 rp = philox(U4{i0 >> 7, row, (uint32_t)t, wP}, k0, k1);
-                prop = u4_word(rp, (i0 >> 5) & 3u);
-            }
+            // draws first: they do not wait for the loads.  Nothing branches
+            // between the loads and the count planes, so the draws and the
+            // loads' uses share one block and the wait falls after the draws.
             const U4 r0 = philox(U4{i0 >> 4, row, (uint32_t)t, wA}, k0, k1);
             const U4 r1 = philox(U4{(i0 >> 4) + 1u, row, (uint32_t)t, wA}, k0, k1);
-            const uint32_t d[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+            uint32_t d[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 
             const uint32_t o = site_bits<BITS>(c_o);
             const uint32_t ym = site_bits<NB>(c_ym);
             const uint32_t yp = site_bits<NB>(c_yp);
             const uint32_t ahead = site_bits<NB>(c_ab);
             const uint32_t g_row = gy + (behind_ok ? 0u : 1u) + (ahead_ok ? 0u : 1u);
-            const uint32_t xe = q ? (!xr_in && !px ? nvalid - 1u : 32u) : (!xl_ok ? 0u : 32u);
+            const uint32_t xe = q ? xe1 : xe0;
             const uint32_t eb = xe < 32u ? 1u << xe : 0u;  // site with an absent x neighbour
             const uint32_t no = ~o;
             const uint32_t mrow = here & vmask;
@@ -403,7 +406,21 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const uint32_t a0 = s1 ^ s2, h = s1 & s2;
             const uint32_t a1 = bop3<kXor3>(c1, c2, h);
             const uint32_t a2 = bop3<kMaj3>(c1, c2, h);
-            const uint32_t live = rnd ? vmask & ~(prop ^ o) : vmask;
+            // The compiler would otherwise sink the draws past the branch
+            // below, behind the wait for the loads: an empty asm statement
+            // taking them keeps them ahead of it.
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) asm volatile("" : "+v"(d[k]));
+            uint32_t live = vmask;
+            if (rnd) {
+                // Spin::rand proposes Up when its draw is below 1/2
+                // (state.rs:76-84): proposal bit 0 -> Up.  A proposal equal
+                // to the current spin changes nothing.
+                const U4 rp = philox(U4{i0 >> 7, row, (uint32_t)t, wP}, k0, k1);
+                const uint32_t wsel = (i0 >> 5) & 3u;
+                const uint32_t lo = wsel & 1u ? rp.y : rp.x, hi = wsel & 1u ? rp.w : rp.z;
+                live &= ~((wsel & 2u ? hi : lo) ^ o);
+            }
 
             // Sites of this row segment take the plan of their absent count:
             // g_row, and g_row + 1 at position xe.  Usually every lane of the
@@ -479,6 +496,9 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             behind_ok = true;
             off += dstep;
             noff += ndstep;
+            z = down ? z - 1u : z + 1u;
+            row += rstep;
+            q ^= 1u;
         }
         m_part += 2 * (int32_t)ups - (int32_t)(nk * nvalid);
     }
