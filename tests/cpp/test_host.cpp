@@ -552,6 +552,144 @@ static void job_server_programs() {
 // Output tables (output.rs): the sensors' parquet files, written here by
 // driving the hooks directly, are read back and checked by pyarrow in
 // tests/test_host_cpp.py.
+// A program stopped after a stage and resumed from its checkpoint file in a
+// fresh Machine (other initial spins, a fresh rng) repeats the uninterrupted
+// run: the same StatSensor lines, per-sweep energies and |M| (bit for bit)
+// and final spins.  And run_stages from stage 0 is the program's run().
+template <class H, class I, class S>
+struct ResumeRun {
+    std::string lines;
+    std::vector<SweepRecord> records;
+    State<S> final_state;
+};
+
+template <class H, class I, class S, class Make>
+static ResumeRun<H, I, S> run_program(Make make, const std::vector<Stage>& stages, size_t first,
+                                      const Checkpoint<S>* from, uint64_t rng_seed,
+                                      size_t save_after, const std::string& path) {
+    std::ostringstream out;
+    typename Machine<H, I, S>::Instruments ins;
+    ins.emplace_back(new StatSensor<H, S>(out));
+    auto* rec = new Recorder<H, S>();
+    ins.emplace_back(rec);
+    Machine<H, I, S> m = make(std::move(ins));
+    if (from) m.restore(*from);
+    std::mt19937_64 rng(rng_seed);
+    run_stages(rng, m, stages, first, [&](uint64_t next) {
+        if (next == save_after) save_checkpoint(path, m.checkpoint(next));
+    });
+    return ResumeRun<H, I, S>{out.str(), rec->records, m.state()};
+}
+
+template <class S>
+static bool same_state(const State<S>& a, const State<S>& b) {
+    if (a.len() != b.len()) return false;
+    for (size_t i = 0; i < a.len(); ++i)
+        if (a.at(i) != b.at(i)) return false;
+    return true;
+}
+
+static size_t count_lines(const std::string& s) {
+    size_t n = 0;
+    for (char c : s) n += c == '\n';
+    return n;
+}
+
+static void checkpoint_resume(const std::string& dir) {
+    {  // Ising CoolDown, flip proposals, checkpoint after stage 2 of 5
+        using H = Compound<Exchange, Zeeman<IsingSpin>>;
+        using I = MetropolisFlipIntegrator;
+        const Lattice l = Lattice::sc().expand(12, 12, 12);
+        const CoolDown prog(4.5, 3.5, 0.25, 20, 50);
+        const auto stages = prog.schedule();
+        EXPECT(stages.size() == 5);
+        const auto make = [&](State<IsingSpin> st) {
+            return [&, st](typename Machine<H, I, IsingSpin>::Instruments ins) {
+                return Machine<H, I, IsingSpin>(
+                    Thermostat<IsingSpin>(4.5, Field<IsingSpin>(IsingSpin::up(), 0.1)),
+                    hamiltonian(Exchange::from_lattice(1.0, l), Zeeman<IsingSpin>()), I(),
+                    std::move(ins), st);
+            };
+        };
+        std::mt19937_64 init(3);
+        const auto st0 = State<IsingSpin>::rand_with_size(init, l.num_sites());
+        const std::string path = dir + "/ising.ckpt";
+        const auto full = run_program<H, I, IsingSpin>(make(st0), stages, 0, nullptr, 11, 2, path);
+        // the program's own run() on the same machine and rng: the same lines
+        {
+            std::ostringstream out;
+            Machine<H, I, IsingSpin>::Instruments ins;
+            ins.emplace_back(new StatSensor<H, IsingSpin>(out));
+            auto m = make(st0)(std::move(ins));
+            std::mt19937_64 rng(11);
+            prog.run(rng, m);
+            EXPECT(out.str() == full.lines);
+        }
+        const auto ck = load_checkpoint<IsingSpin>(path);
+        EXPECT(ck.started && ck.next_stage == 2 && ck.sweeps == 2 * (20 + 50));
+        const auto resumed = run_program<H, I, IsingSpin>(
+            make(State<IsingSpin>::down_with_size(l.num_sites())), stages, ck.next_stage, &ck,
+            99, 0, path);
+        EXPECT(count_lines(full.lines) == 5 && count_lines(resumed.lines) == 3);
+        size_t cut = 0;
+        for (int k = 0; k < 2; ++k) cut = full.lines.find('\n', cut) + 1;
+        EXPECT(resumed.lines == full.lines.substr(cut));
+        const size_t skip = 2 * (20 + 50);
+        bool same = resumed.records.size() + skip == full.records.size();
+        for (size_t k = 0; same && k < resumed.records.size(); ++k)
+            same = resumed.records[k].energy == full.records[skip + k].energy &&
+                   resumed.records[k].magnetization == full.records[skip + k].magnetization;
+        EXPECT(same);
+        EXPECT(same_state(resumed.final_state, full.final_state));
+        // resuming at the end runs nothing; past the end is refused
+        const auto done = run_program<H, I, IsingSpin>(make(st0), stages, 5, &ck, 1, 0, path);
+        EXPECT(done.lines.empty() && done.records.empty());
+        EXPECT(throws_kind(
+            [&] { run_program<H, I, IsingSpin>(make(st0), stages, 6, &ck, 1, 0, path); },
+            ErrorKind::InvalidArgument));
+    }
+    {  // Heisenberg HysteresisLoop, random proposals and anisotropy
+        using H = Compound<Compound<Exchange, UniaxialAnisotropy<HeisenbergSpin>>,
+                           Zeeman<HeisenbergSpin>>;
+        using I = MetropolisIntegrator;
+        const Lattice l = Lattice::sc().expand_all(8);
+        const HysteresisLoop prog(30, 10, 1.5, 0.2, 0.1);
+        const auto stages = prog.schedule();
+        const size_t at = stages.size() / 2;
+        const auto make = [&](State<HeisenbergSpin> st) {
+            return [&, st](typename Machine<H, I, HeisenbergSpin>::Instruments ins) {
+                return Machine<H, I, HeisenbergSpin>(
+                    Thermostat<HeisenbergSpin>(1.5, Field<HeisenbergSpin>::zero()),
+                    hamiltonian(Exchange::from_lattice(1.0, l),
+                                UniaxialAnisotropy<HeisenbergSpin>(HeisenbergSpin::up(), 0.2),
+                                Zeeman<HeisenbergSpin>()),
+                    I(), std::move(ins), st);
+            };
+        };
+        std::mt19937_64 init(4);
+        const auto st0 = State<HeisenbergSpin>::rand_with_size(init, l.num_sites());
+        const std::string path = dir + "/heis.ckpt";
+        const auto full =
+            run_program<H, I, HeisenbergSpin>(make(st0), stages, 0, nullptr, 12, at, path);
+        const auto ck = load_checkpoint<HeisenbergSpin>(path);
+        EXPECT(ck.next_stage == at && ck.sweeps == at * (10 + 30));
+        const auto resumed = run_program<H, I, HeisenbergSpin>(
+            make(State<HeisenbergSpin>::up_with_size(l.num_sites())), stages, at, &ck, 5, 0,
+            path);
+        size_t cut = 0;
+        for (size_t k = 0; k < at; ++k) cut = full.lines.find('\n', cut) + 1;
+        EXPECT(count_lines(resumed.lines) == stages.size() - at);
+        EXPECT(resumed.lines == full.lines.substr(cut));
+        const size_t skip = at * (10 + 30);
+        bool same = resumed.records.size() + skip == full.records.size();
+        for (size_t k = 0; same && k < resumed.records.size(); ++k)
+            same = resumed.records[k].energy == full.records[skip + k].energy &&
+                   resumed.records[k].magnetization == full.records[skip + k].magnetization;
+        EXPECT(same);
+        EXPECT(same_state(resumed.final_state, full.final_state));
+    }
+}
+
 static void parquet_tables(const std::string& dir) {
     using vegas::parquet_detail::with_extension;
     EXPECT(with_extension("a/b.parquet", "parquet.tmp") == "a/b.parquet.tmp");
@@ -806,6 +944,61 @@ static void input_documents() {
 // A Snappy block decoder written from the format description (varint
 // length; literals; copies with 1- and 2-byte offsets -- the compressor
 // emits no 4-byte-offset copies), for the round trips below.
+// Checkpoint files: encode -> decode gives back every field (Heisenberg
+// spins as the same doubles), and damaged or foreign files are refused.
+static void checkpoint_files(const std::string& dir) {
+    std::mt19937_64 rng(9);
+    Checkpoint<HeisenbergSpin> h;
+    h.started = true;
+    h.seed = 0x0123456789abcdefull;
+    h.sweeps = 4242;
+    h.next_stage = 7;
+    h.thermostat = Thermostat<HeisenbergSpin>(
+        1.25, Field<HeisenbergSpin>(HeisenbergSpin(0.6, 0.0, 0.8), 0.3));
+    h.state = State<HeisenbergSpin>::rand_with_size(rng, 33);
+    const std::string path = dir + "/h.ckpt";
+    save_checkpoint(path, h);
+    const auto g = load_checkpoint<HeisenbergSpin>(path);
+    EXPECT(g.started && g.seed == h.seed && g.sweeps == 4242 && g.next_stage == 7);
+    EXPECT(g.thermostat.temperature() == 1.25 && g.thermostat.field().magnitude() == 0.3);
+    EXPECT(g.thermostat.field().orientation() == HeisenbergSpin(0.6, 0.0, 0.8));
+    bool same = g.state.len() == 33;
+    for (size_t i = 0; same && i < 33; ++i) same = g.state.at(i) == h.state.at(i);
+    EXPECT(same);
+    std::FILE* t = std::fopen((path + ".tmp").c_str(), "rb");
+    EXPECT(t == nullptr);  // renamed into place
+    if (t) std::fclose(t);
+
+    Checkpoint<IsingSpin> i;
+    i.next_stage = 1;
+    i.thermostat = Thermostat<IsingSpin>(2.0, Field<IsingSpin>(IsingSpin::down(), -0.5));
+    i.state = State<IsingSpin>::rand_with_size(rng, 20);
+    const std::string bytes = encode_checkpoint(i);
+    EXPECT(bytes.size() == 8 + 4 + 4 + 3 * 8 + 5 * 8 + 8 + 20);
+    const auto j = decode_checkpoint<IsingSpin>(bytes);
+    EXPECT(!j.started && j.next_stage == 1 && j.state.len() == 20);
+    EXPECT(j.thermostat.field().orientation() == IsingSpin::down());
+    EXPECT(j.thermostat.field().magnitude() == 0.5);
+    same = true;
+    for (size_t k = 0; k < 20; ++k) same &= j.state.at(k) == i.state.at(k);
+    EXPECT(same);
+    // refusals: truncated, trailing bytes, bad magic, another spin kind, bad spin
+    EXPECT(throws_kind([&] { decode_checkpoint<IsingSpin>(bytes.substr(0, bytes.size() - 1)); },
+                       ErrorKind::InvalidArgument));
+    EXPECT(throws_kind([&] { decode_checkpoint<IsingSpin>(bytes + "x"); },
+                       ErrorKind::InvalidArgument));
+    std::string bad = bytes;
+    bad[0] = 'X';
+    EXPECT(throws_kind([&] { decode_checkpoint<IsingSpin>(bad); }, ErrorKind::InvalidArgument));
+    EXPECT(throws_kind([&] { decode_checkpoint<HeisenbergSpin>(bytes); },
+                       ErrorKind::InvalidArgument));
+    bad = bytes;
+    bad[bad.size() - 1] = 3;
+    EXPECT(throws_kind([&] { decode_checkpoint<IsingSpin>(bad); }, ErrorKind::InvalidArgument));
+    EXPECT(throws_kind([&] { load_checkpoint<IsingSpin>(dir + "/missing.ckpt"); },
+                       ErrorKind::Io));
+}
+
 static bool snappy_decode(const std::string& in, std::string& out) {
     size_t i = 0;
     uint64_t n = 0;
@@ -898,6 +1091,15 @@ int main(int argc, char** argv) {
     } else if (argc > 2 && std::strcmp(argv[1], "--gpu-parquet") == 0) {
         const std::string dir = argv[2];
         tests = {{"sensors_write_parquet", [dir] { sensors_write_parquet(dir); }}};
+    } else if (argc > 2 && std::strcmp(argv[1], "--checkpoint") == 0) {
+        const std::string dir = argv[2];
+        tests = {{"checkpoint_files", [dir] { checkpoint_files(dir); }}};
+    } else if (argc > 2 && std::strcmp(argv[1], "--gpu-checkpoint") == 0) {
+        const std::string dir = argv[2];
+        tests = {{"checkpoint_resume", [dir] { checkpoint_resume(dir); }}};
+    } else if (argc > 1 && !gpu && std::strcmp(argv[1], "--cpu") != 0) {
+        std::fprintf(stderr, "unknown mode %s\n", argv[1]);
+        return 2;
     } else if (!gpu) {
         tests = {{"ising_spin_multiplies_correctly", ising_spin_multiplies_correctly},
                  {"ising_magnetization_can_be_added_with_spin", ising_magnetization_can_be_added_with_spin},

@@ -208,3 +208,24 @@ def _run_parquet(out_dir, flag="--parquet"):
     print(r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
+
+
+def test_checkpoint_files(tmp_path):
+    """Checkpoint files (host/vegas/checkpoint.hpp): every field back from
+    the file (Heisenberg spins as the same doubles), written through a .tmp
+    file renamed into place; truncated, over-long, foreign-magic,
+    other-spin-kind and bad-spin files refused; a missing file an I/O
+    error."""
+    _run_parquet(tmp_path, "--checkpoint")
+
+
+@pytest.mark.gpu
+def test_checkpoint_resume(tmp_path):
+    """A CoolDown (Ising, flip proposals, field) and a HysteresisLoop
+    (Heisenberg, random proposals, anisotropy) checkpointed after a stage
+    and resumed from the file in a fresh Machine with other initial spins
+    and rng: StatSensor lines, per-sweep energy and |M| and the final spins
+    equal the uninterrupted run's bit for bit; run_stages from stage 0
+    prints the program's own run() lines; resuming at the end runs nothing
+    and past it is refused."""
+    _run_parquet(tmp_path, "--gpu-checkpoint")

@@ -96,6 +96,13 @@ class Chain {
         for (size_t k = 0; k < sweeps; ++k) out[k] = SweepRecord{e[k], m[k]};
     }
     vg_batch* handle() { return batch_; }
+    // Sweeps run so far: the chain's Philox counter.
+    uint64_t sweep_count() const {
+        uint64_t n = 0;
+        check(vg_batch_sweep_count(batch_, 0, &n));
+        return n;
+    }
+    void set_sweep_count(uint64_t n) { check(vg_batch_set_sweep_count(batch_, 0, n)); }
 
   private:
     void upload_impl(const State<S>& st);
@@ -167,6 +174,19 @@ State<S> step(const I&, Rng& rng, const Thermostat<S>& th, const H& h, const Sta
     return c.download();
 }
 
+// A Machine's chain between two stages (checkpoint.hpp saves and loads
+// these): the chain's Philox key and counter, its spins and thermostat, and
+// the index of the next stage of the program being run.
+template <class S>
+struct Checkpoint {
+    bool started = false;  // the chain exists (seed and sweeps are meaningful)
+    uint64_t seed = 0;
+    uint64_t sweeps = 0;
+    uint64_t next_stage = 0;
+    Thermostat<S> thermostat = Thermostat<S>::near_zero();
+    State<S> state;
+};
+
 template <class H, class I, class S>
 class Machine {
   public:
@@ -199,16 +219,48 @@ class Machine {
     }
     Chain<S>& chain() { return *chain_; }
 
+    // The machine's chain now, with next_stage as the program cursor.
+    Checkpoint<S> checkpoint(uint64_t next_stage) const {
+        Checkpoint<S> ck;
+        ck.next_stage = next_stage;
+        ck.thermostat = thermostat_;
+        if (chain_) {
+            ck.started = true;
+            ck.seed = seed_;
+            ck.sweeps = chain_->sweep_count();
+            ck.state = chain_->download();
+        } else {
+            ck.state = state_;
+        }
+        return ck;
+    }
+    // Continues from a checkpoint: the chain gets the checkpoint's seed,
+    // counter and spins (no seed is drawn from the caller's rng), so its
+    // next sweeps are the ones the checkpointed run would have made.
+    void restore(const Checkpoint<S>& ck) {
+        if (ck.state.len() != state_.len())
+            throw Error(ErrorKind::InvalidArgument, "checkpoint size does not match the machine");
+        thermostat_ = ck.thermostat;
+        state_ = ck.state;
+        chain_.reset();
+        if (ck.started) {
+            make_chain(ck.seed);
+            chain_->set_sweep_count(ck.sweeps);
+        }
+    }
+
   private:
+    void make_chain(uint64_t seed) {
+        seed_ = seed;
+        chain_ = std::make_unique<Chain<S>>(lattice_of(hamiltonian_, state_.len()), I::proposal,
+                                            describe(hamiltonian_), seed, default_context(),
+                                            hamiltonian_.matrix());
+        chain_->upload(state_);
+    }
+
     template <class Rng>
     void run(Rng& rng, size_t steps) {
-        if (!chain_) {
-            chain_ = std::make_unique<Chain<S>>(lattice_of(hamiltonian_, state_.len()),
-                                                I::proposal, describe(hamiltonian_),
-                                                (uint64_t)rng(), default_context(),
-                                                hamiltonian_.matrix());
-            chain_->upload(state_);
-        }
+        if (!chain_) make_chain((uint64_t)rng());
         chain_->set_thermostat(thermostat_);
         std::vector<SweepRecord> rec;
         size_t done = 0;
@@ -249,6 +301,7 @@ class Machine {
     Instruments instruments_;
     State<S> state_;
     std::unique_ptr<Chain<S>> chain_;
+    uint64_t seed_ = 0;  // the chain's Philox key
 };
 
 }  // namespace vegas

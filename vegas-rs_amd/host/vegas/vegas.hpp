@@ -7,6 +7,7 @@
 #include "vegas_hip.h"
 
 #include "accumulator.hpp"
+#include "checkpoint.hpp"
 #include "energy.hpp"
 #include "error.hpp"
 #include "instrument.hpp"
