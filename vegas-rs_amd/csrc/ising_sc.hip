@@ -366,7 +366,14 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const bool at_end = z == zend;
             const bool ahead_ok = !at_end || pz;
             const uint32_t ahead_off = !at_end ? noff + ndstep : (pz ? noff + nwrap : kNoRow);
-            const RowRaw<BITS> c_o = load_raw<BITS>(rw_own, off, full);
+            // The own sites: their row as stored, or with shadows the own
+            // colour's shadow word, which every write of the row keeps equal
+            // to it (the int8 row is then only written here).
+            uint32_t o_word = 0;
+            if constexpr (SH)
+                o_word = __builtin_amdgcn_raw_buffer_load_b32(rw_sh, noff, 0, 0);
+            else
+                o_word = site_bits<BITS>(load_raw<BITS>(rw_own, off, full));
             const RowRaw<NB> c_ym = load_raw<NB>(ro, ym_ok ? noff + dym : kNoRow, full);
             const RowRaw<NB> c_yp = load_raw<NB>(ro, yp_ok ? noff + dyp : kNoRow, full);
             const RowRaw<NB> c_ab = load_raw<NB>(ro, ahead_off, full);
@@ -380,7 +387,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             const U4 r1 = philox(U4{(i0 >> 4) + 1u, row, (uint32_t)t, wA}, k0, k1);
             uint32_t d[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 
-            const uint32_t o = site_bits<BITS>(c_o);
+            const uint32_t o = o_word;
             const uint32_t ym = site_bits<NB>(c_ym);
             const uint32_t yp = site_bits<NB>(c_yp);
             const uint32_t ahead = site_bits<NB>(c_ab);
