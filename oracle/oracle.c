@@ -1,7 +1,8 @@
 /*
  * CPU oracle -- TEST INFRASTRUCTURE ONLY (see oracle.h for who may use it).
  * Build: oracle/Makefile (gcc, -ffp-contract=off so every fp32 operation is
- * the single IEEE operation the checkerboard semantics name).
+ * the single IEEE operation the checkerboard semantics name; the fused
+ * multiply-adds they name are explicit fmaf calls).
  */
 #include "oracle.h"
 
@@ -408,7 +409,7 @@ void or_ising_sweeps_sc_fast(const or_lattice* L, const or_model* M, uint64_t se
 }
 
 /* ------------------------------------------------------------------------ */
-/* Heisenberg checkerboard (fp32; every line is one IEEE operation)          */
+/* Heisenberg checkerboard (fp32; every line is one IEEE operation or fmaf) */
 /* ------------------------------------------------------------------------ */
 float or_exp_nonpos(float x) {
     if (x < -86.0f) return 0.0f;
@@ -425,12 +426,10 @@ float or_exp_nonpos(float x) {
     return ldexpf(p, (int)n);
 }
 
+/* a . b as one product and two fused multiply-adds, x then y then z. */
 static float dot3f(const float* a, const float* b) {
-    float t = a[0] * b[0];
-    float u = a[1] * b[1];
-    float v = t + u;
-    float w = a[2] * b[2];
-    return v + w;
+    const float t = a[0] * b[0];
+    return fmaf(a[2], b[2], fmaf(a[1], b[1], t));
 }
 
 static float unit24f(uint32_t w) { return (float)(w >> 8) * 5.9604644775390625e-08f; }
@@ -578,8 +577,8 @@ void or_heis_sweeps(const or_lattice* L, const or_model* M, uint64_t seed, uint6
                         const float d[3] = {p[0] - s[0], p[1] - s[1], p[2] - s[2]};
                         const float e_ex = jn * dot3f(d, h);
                         const float pe = dot3f(p, axis), oe = dot3f(s, axis);
-                        const float pe2 = pe * pe, oe2 = oe * oe;
-                        const float e_an = kk * (pe2 - oe2);
+                        const float oe2 = oe * oe;
+                        const float e_an = kk * fmaf(pe, pe, -oe2);
                         const float e_ze = hm * dot3f(d, fdir);
                         const float e1 = e_ex + e_an;
                         const float dE = e1 + e_ze;
@@ -1266,8 +1265,8 @@ void or_graph_heis_sweeps(const or_graph* G, const or_model* M, uint64_t seed, u
                 const float d[3] = {p[0] - s[0], p[1] - s[1], p[2] - s[2]};
                 const float e_ex = jn * dot3f(d, h);
                 const float pe = dot3f(p, axis), oe = dot3f(s, axis);
-                const float pe2 = pe * pe, oe2 = oe * oe;
-                const float e_an = kk * (pe2 - oe2);
+                const float oe2 = oe * oe;
+                const float e_an = kk * fmaf(pe, pe, -oe2);
                 const float e_ze = hm * dot3f(d, fdir);
                 const float e1 = e_ex + e_an;
                 const float dE = e1 + e_ze;

@@ -5,8 +5,8 @@
 // marsaglia, util.rs:21-34) and MetropolisFlipIntegrator::step (109-138:
 // proposal -s) with the local energy of Exchange + UniaxialAnisotropy +
 // Zeeman + Gauge (src/energy.rs:64-214).  Every arithmetic step is a single
-// IEEE fp32 operation in a fixed order (no contraction), so the CPU oracle
-// reproduces every bit.
+// IEEE fp32 operation or an explicit fused multiply-add, in a fixed order
+// (no implicit contraction), so the CPU oracle reproduces every bit.
 #pragma once
 
 #include "vg_device.hpp"
@@ -16,8 +16,11 @@ namespace vg {
 __device__ __forceinline__ float fm(float a, float b) { return __fmul_rn(a, b); }
 __device__ __forceinline__ float fa(float a, float b) { return __fadd_rn(a, b); }
 __device__ __forceinline__ float fs(float a, float b) { return __fsub_rn(a, b); }
+// Fused multiply-add, one rounding (v_fma_f32; the oracle's fmaf).
+__device__ __forceinline__ float ff(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// a . b as one product and two fused multiply-adds, x then y then z.
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
-    return fa(fa(fm(ax, bx), fm(ay, by)), fm(az, bz));
+    return ff(az, bz, ff(ay, by, fm(ax, bx)));
 }
 __device__ __forceinline__ int64_t fix27(float v) {
     return (int64_t)__float2int_rn(fm(v, kFixScale));
@@ -55,8 +58,9 @@ __device__ __forceinline__ uint32_t v24(const U4& c, int m) {
 }
 // A 24-bit value as a uniform in [0,1) and in [-1,1) (exact in fp32).
 __device__ __forceinline__ float f24u(uint32_t v) { return fm((float)v, 5.9604644775390625e-08f); }
+// (the product is exact, so the fused form rounds once, as fm then fs do)
 __device__ __forceinline__ float f24s(uint32_t v) {
-    return fs(fm((float)v, 1.1920928955078125e-07f), 1.0f);
+    return ff((float)v, 1.1920928955078125e-07f, -1.0f);
 }
 
 struct Try {
@@ -75,7 +79,7 @@ __device__ __forceinline__ void sphere_point(const Try& a, float p[3]) {
     const float root = sqrt_rn(fs(1.0f, a.q));
     p[0] = fm(fm(2.0f, a.x1), root);
     p[1] = fm(fm(2.0f, a.x2), root);
-    p[2] = fs(1.0f, fm(2.0f, a.q));
+    p[2] = ff(-2.0f, a.q, 1.0f);  // 2q is exact: the same rounding as fs(1, 2q)
 }
 
 // Fresh uniform-sphere proposal of one site (Spin::rand -> marsaglia,
@@ -218,7 +222,7 @@ __device__ __forceinline__ void heis_update(const ChainDev& C, bool bond, const 
     const float e_ex = fm(C.jn, dot3(dx, dy, dz, h[0], h[1], h[2]));
     const float pe = dot3(px, py, pz, ax, ay, az);
     const float oe = dot3(ox, oy, oz, ax, ay, az);
-    const float e_an = fm(C.kk, fs(fm(pe, pe), fm(oe, oe)));
+    const float e_an = fm(C.kk, ff(pe, pe, -fm(oe, oe)));
     const float e_ze = fm(C.hm, dot3(dx, dy, dz, C.fx, C.fy, C.fz));
     const float dE = fa(fa(e_ex, e_an), e_ze);
     bool accept = valid && dE < 0.0f;
