@@ -17,7 +17,9 @@
 // downwards in odd ones.  Along z the row behind is the previous row's
 // neighbour ahead, so each plane loads four rows (own, y-1, y+1, the next
 // plane's) instead of six.  Int8 rows are packed to words on load (v_dot4 with bit
-// weights) and expanded on store; bit-packed rows are one 32-bit load.
+// weights) and expanded on store; bit-packed rows are one 32-bit load.  With
+// bit-packed shadows of the two colour planes (int8 chains of a batch) every
+// row is read as a shadow word, and the int8 row is only written.
 //
 // The decision works on whole words.  Each neighbour word is replaced by
 // the complement of the site bits where that neighbour is absent (open
@@ -417,7 +419,7 @@ __global__ __launch_bounds__(256) void ising_sc_pass(const ChainDev* __restrict_
             // below, behind the wait for the loads: an empty asm statement
             // taking them keeps them ahead of it.
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) asm volatile("" : "+v"(d[k]));
+            for (uint32_t j = 0; j < 8; ++j) asm volatile("" : "+v"(d[j]));
             uint32_t live = vmask;
             if (rnd) {
                 // Spin::rand proposes Up when its draw is below 1/2
