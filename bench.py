@@ -286,7 +286,7 @@ def run_batched(vg, ctx, sweeps, baseline_sweeps, n_chains=512, cpu=None):
         b.randomize()
         for i, (T, h, d) in enumerate(therm):
             b.set_thermostat(i, T, h, d)
-        tile = b.autotune(5)
+        tile = b.autotune(30)  # 30 trial sweeps per tile: a stable pick
         b.set_tile(tile)
         b.set_graph(True)
         b.run(10)
