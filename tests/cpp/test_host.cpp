@@ -648,6 +648,32 @@ static void checkpoint_resume(const std::string& dir) {
             [&] { run_program<H, I, IsingSpin>(make(st0), stages, 6, &ck, 1, 0, path); },
             ErrorKind::InvalidArgument));
     }
+    {  // Ising CoolDown with Wolff cluster steps: the step counter is the chain's
+        using H = Exchange;
+        using I = WolffIntegrator;
+        const Lattice l = Lattice::sc().expand(10, 10, 10);
+        const CoolDown prog(5.0, 4.0, 0.5, 15, 40);
+        const auto stages = prog.schedule();
+        const auto make = [&](State<IsingSpin> st) {
+            return [&, st](typename Machine<H, I, IsingSpin>::Instruments ins) {
+                return Machine<H, I, IsingSpin>(
+                    Thermostat<IsingSpin>(5.0, Field<IsingSpin>::zero()),
+                    Exchange::from_lattice(1.0, l), I(), std::move(ins), st);
+            };
+        };
+        std::mt19937_64 init(6);
+        const auto st0 = State<IsingSpin>::rand_with_size(init, l.num_sites());
+        const std::string path = dir + "/wolff.ckpt";
+        const auto full = run_program<H, I, IsingSpin>(make(st0), stages, 0, nullptr, 13, 1, path);
+        const auto ck = load_checkpoint<IsingSpin>(path);
+        EXPECT(ck.next_stage == 1 && ck.sweeps == 15 + 40);
+        const auto resumed = run_program<H, I, IsingSpin>(
+            make(State<IsingSpin>::up_with_size(l.num_sites())), stages, 1, &ck, 8, 0, path);
+        size_t cut = full.lines.find('\n') + 1;
+        EXPECT(count_lines(resumed.lines) == stages.size() - 1);
+        EXPECT(resumed.lines == full.lines.substr(cut));
+        EXPECT(same_state(resumed.final_state, full.final_state));
+    }
     {  // Heisenberg HysteresisLoop, random proposals and anisotropy
         using H = Compound<Compound<Exchange, UniaxialAnisotropy<HeisenbergSpin>>,
                            Zeeman<HeisenbergSpin>>;

@@ -222,7 +222,7 @@ def test_checkpoint_files(tmp_path):
 @pytest.mark.gpu
 def test_checkpoint_resume(tmp_path):
     """A CoolDown (Ising, flip proposals, field) and a HysteresisLoop
-    (Heisenberg, random proposals, anisotropy) checkpointed after a stage
+    (Heisenberg, random proposals, anisotropy) and a Wolff CoolDown checkpointed after a stage
     and resumed from the file in a fresh Machine with other initial spins
     and rng: StatSensor lines, per-sweep energy and |M| and the final spins
     equal the uninterrupted run's bit for bit; run_stages from stage 0
